@@ -1,0 +1,163 @@
+/*
+ * hvc_engine.h -- C ABI of the MI355X (gfx950) HVC growth-trial engine.
+ *
+ * The engine replaces the per-trial inner loop of the reference growth
+ * simulator (paths relative to the reference checkout, "NGS" =
+ * c++/src/NetworkGrowthSimulator.cpp, "NGS.h" = c++/include/NetworkGrowthSimulator.h):
+ * neuron integration, spike delivery through axonal-delay queues into synaptic
+ * conductances, and the STDP weight / synapse-state machine.  The host keeps the
+ * growth driver, file formats and replacement bookkeeping and calls these
+ * entry points.  Plain C types only; all pointers are borrowed for the call.
+ *
+ * Errors: every call returns hvc_status; hvc_last_error() gives a message.
+ * Threading: a context is not thread-safe; separate contexts are independent.
+ */
+#ifndef HVC_ENGINE_H
+#define HVC_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hvc_ctx hvc_ctx;
+
+typedef enum {
+    HVC_OK = 0,
+    HVC_E_ARG = 1,      /* invalid argument or network description */
+    HVC_E_OOM = 2,      /* device or host allocation failed */
+    HVC_E_HIP = 3,      /* HIP runtime error */
+    HVC_E_NUMERIC = 4,  /* check_bad_values flagged an HVC-RA neuron (HH2_test.cpp:244-260) */
+    HVC_E_STATE = 5,    /* call out of order (e.g. run before upload) */
+    HVC_E_CAPACITY = 6  /* an engine buffer overflowed; result discarded */
+} hvc_status;
+
+/* SynapticParameters, c++/include/ConfigurationNetworkGrowth.h:11-34 */
+typedef struct {
+    int Nss;
+    double A_P, A_D, T_0, T_P, T_D, TAU_P, TAU_D;
+    double BETA, BETA_SUPERSYNAPSE;
+    double ACTIVATION_THRESHOLD, SUPERSYNAPSE_THRESHOLD, WEIGHT_MAX;
+} hvc_synaptic_params;
+
+typedef struct {
+    int ra_model;       /* 0: HH2_test + Debraband_step (NGS:8943), 1: HH2_buffer + Runge4_step */
+    int i_model;        /* 0: HHI_buffer + DP8_step (NGS:9223),    1: HHI_buffer + Runge4_step  */
+    int precision;      /* 0: FP64 state (bit-exact form); 1: FP32 neuron state */
+    double dt;          /* TIMESTEP (NGS:15) */
+    double trial_ms;    /* TRIAL_DURATION (NGS:17) */
+    double sync_ms;     /* NETWORK_UPDATE_FREQUENCY (NGS:16) */
+    double mu_s, sigma_s, mu_d, sigma_d;  /* NoiseParameters, ConfigurationNetworkGrowth.h:77-89 */
+    int ra_poisson;     /* Poisson conductance noise on HVC-RA (RK4 form only) */
+    int i_poisson;      /* Poisson conductance noise on HVC-I (NGS:2061-2075 enables it) */
+    uint32_t seed_lo, seed_hi;   /* noise-stream key */
+    hvc_synaptic_params syn;
+    int spike_capacity; /* max spikes per trial per kind (0 = default) */
+} hvc_params;
+
+/* Network.  RA->I and I->RA are CSR by presynaptic neuron (syn_ID_*, weights_*,
+ * axonal_delays_* of NGS.h); RA->RA is a candidate set, CSR by presynaptic
+ * neuron with ascending targets (the reference's dense N_RA x N_RA arrays,
+ * NGS.h:142-172, are the candidate set of all targets).  Synapse state: weight
+ * per candidate, active / super lists in insertion order, remodeled flags. */
+typedef struct {
+    int n_ra, n_i;
+    const int* ra2i_ptr; const int* ra2i_tgt; const double* ra2i_w; const double* ra2i_d;
+    const int* i2ra_ptr; const int* i2ra_tgt; const double* i2ra_w; const double* i2ra_d;
+    const int* rr_ptr; const int* rr_tgt; const double* rr_d;
+    const double* rr_w;
+    const int* act_cnt; const int* act_ids;
+    const int* sup_cnt; const int* sup_ids;
+    const int* remodeled;
+    const double* erest;   /* per HVC-RA: set_Erest (NGS:2141) */
+    const double* gca;     /* per HVC-RA: set_GCa  (NGS:2142) */
+    int n_train; const int* train_ids; const double* spread;
+} hvc_network;
+
+typedef struct {
+    int mode;                   /* 0: growth trial, 1: chain-test trial */
+    int training;               /* growth: apply the training kick (NGS:8933) */
+    uint32_t trial_index;       /* selects the noise streams of this trial */
+    int first_trial;            /* first trial after initialize_network (NGS:849-850) */
+    double mean_injection_time; /* NGS:8898 (growth) / 12785 (chain test: 100) */
+} hvc_trial;
+
+/* Outputs in caller-owned buffers.  *_counts has one entry per neuron; *_times
+ * (may be NULL) holds the spike times neuron by neuron (capacity *_cap); *_total
+ * is the spike count.  With *_times NULL only counts are written; the times stay
+ * available through hvc_trial_spikes until the next trial. */
+typedef struct {
+    int* soma_counts; double* soma_times; int soma_cap; int soma_total;
+    int* dend_counts; double* dend_times; int dend_cap; int dend_total;
+    int* i_counts; double* i_times; int i_cap; int i_total;
+    int bad_values;             /* -1 when check_bad_values fails, else 0 */
+    int warnings[3];            /* LTP dt<=T_0, LTD dt>T_0, list entry missing (stderr in the reference) */
+    double* ra_state;           /* optional: 12 doubles per HVC-RA (Vs Vd n l h r c Ca Ge_s Gi_s Ge_d Gi_d) */
+    double* i_state;            /* optional: 7 doubles per HVC-I (V n m h w Ge Gi) */
+} hvc_trial_out;
+
+typedef struct {
+    double* w;                  /* per candidate edge */
+    int* act_cnt; int* act_ids; /* act_ids: concatenated lists, capacity = candidate count */
+    int* sup_cnt; int* sup_ids;
+    int* remodeled;
+} hvc_synapses;
+
+/* Create a context on a device.  Replaces NetworkGrowthSimulator's constructor
+ * and set_dynamics / set_noise (NGS:48-82, 2061-2095). */
+hvc_status hvc_create(const hvc_params* params, int device, hvc_ctx** out);
+
+/* Upload the network and synapse state.  Replaces initialize_network and the
+ * send_* scatters (NGS:745-890, 3536-4232). */
+hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net);
+
+/* Run one trial.  mode 0 replaces trial_1stSoma_pre_1stSoma_post_delays_fixedSpread
+ * (NGS:8839-9571, called at NGS:5792) including potentiation_decay and
+ * update_all_synapses; mode 1 replaces trial_no_stdp_fixedSpread (NGS:12747-13118,
+ * called at NGS:1406).  Every trial starts from rest (setToRest_afterEpoch,
+ * NGS:4591-4621).  Returns HVC_E_NUMERIC when check_bad_values fails (NGS:4538-4551). */
+hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* trial, hvc_trial_out* out);
+
+/* Spike times of the last trial: kind 0 somatic, 1 dendritic (HVC-RA), 2 HVC-I.
+ * counts gets one entry per neuron; times (capacity cap, may be NULL) the spike
+ * times neuron by neuron; *total the number of spikes.  The data of
+ * spikes_in_trial_{soma,dend,interneuron} (NGS.h:202-204). */
+hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, double* times, int cap, int* total);
+
+/* Read back weights, active / super lists and remodeled flags (the data of
+ * gather_full_state_data / write_full_network_state, NGS:18552-18593). */
+hvc_status hvc_download_synapses(hvc_ctx* ctx, hvc_synapses* syn);
+
+/* Replace the synapse state (after replace_neurons, NGS:15757-15881). */
+hvc_status hvc_upload_synapses(hvc_ctx* ctx, const hvc_synapses* syn);
+
+/* Set Erest / GCa of n neurons (set_neuron_properties and
+ * update_neuron_properties_sameDendrite_diffMaturationRate, NGS:2136-2183). */
+hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, const double* erest, const double* gca);
+
+/* Single-neuron traces on the device: the neuron seam of the reference
+ * (HH2_test::Debraband_step_no_target_update HH2_test.cpp:535-545,
+ * HH2_buffer::R4_step_no_target_update HH2_buffer.cpp:521-528,
+ * HHI_buffer::DP8_step_no_target_update / R4_step_no_target_update
+ * HHI_buffer.cpp:323-339, with raiseE / raiseI / raiseExcWeight between steps).
+ * Arrays are host memory laid out [step][neuron]; out_state [n_steps+1][n][12|7]. */
+hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const double* erest, const double* gca,
+                        uint32_t trial, const double* kick, const double* exc_w, const double* inh_g,
+                        double* out_state, uint8_t* out_fired);
+hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32_t trial,
+                       const double* exc_g, const double* inh_g, double* out_state, uint8_t* out_fired);
+
+/* Kernel timing of the last run_trial: average device duration (ms) per launch
+ * of each kernel class, measured with HIP events on the engine's stream.
+ * names: "integrate_ra", "integrate_i", "deliver", "sync", "decay". */
+hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches);
+hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled);
+
+void hvc_destroy(hvc_ctx* ctx);
+const char* hvc_last_error(const hvc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

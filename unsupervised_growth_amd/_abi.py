@@ -1,0 +1,116 @@
+"""ctypes view of include/hvc_engine.h and the loader of libhvc_engine.so.
+
+The engine is the compiled HIP library built in-tree by
+``unsupervised_growth_amd.build``; there is no CPU fallback, so a missing or
+unloadable library raises ``EngineNotBuilt``.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhvc_engine.so")
+
+c_int_p = ctypes.POINTER(ctypes.c_int)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_u8_p = ctypes.POINTER(ctypes.c_uint8)
+
+HVC_OK, HVC_E_ARG, HVC_E_OOM, HVC_E_HIP, HVC_E_NUMERIC, HVC_E_STATE, HVC_E_CAPACITY = range(7)
+STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: "HVC_E_NUMERIC",
+                5: "HVC_E_STATE", 6: "HVC_E_CAPACITY"}
+
+# every entry point declared in include/hvc_engine.h
+EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
+           "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
+           "hvc_kernel_timing", "hvc_set_timing", "hvc_destroy", "hvc_last_error"]
+
+
+class EngineNotBuilt(RuntimeError):
+    pass
+
+
+class SynapticParams(ctypes.Structure):
+    _fields_ = [("Nss", ctypes.c_int), ("A_P", ctypes.c_double), ("A_D", ctypes.c_double),
+                ("T_0", ctypes.c_double), ("T_P", ctypes.c_double), ("T_D", ctypes.c_double),
+                ("TAU_P", ctypes.c_double), ("TAU_D", ctypes.c_double), ("BETA", ctypes.c_double),
+                ("BETA_SUPERSYNAPSE", ctypes.c_double), ("ACTIVATION_THRESHOLD", ctypes.c_double),
+                ("SUPERSYNAPSE_THRESHOLD", ctypes.c_double), ("WEIGHT_MAX", ctypes.c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("ra_model", ctypes.c_int), ("i_model", ctypes.c_int), ("precision", ctypes.c_int),
+                ("dt", ctypes.c_double), ("trial_ms", ctypes.c_double), ("sync_ms", ctypes.c_double),
+                ("mu_s", ctypes.c_double), ("sigma_s", ctypes.c_double), ("mu_d", ctypes.c_double),
+                ("sigma_d", ctypes.c_double), ("ra_poisson", ctypes.c_int), ("i_poisson", ctypes.c_int),
+                ("seed_lo", ctypes.c_uint32), ("seed_hi", ctypes.c_uint32), ("syn", SynapticParams),
+                ("spike_capacity", ctypes.c_int)]
+
+
+class NetworkDesc(ctypes.Structure):
+    _fields_ = [("n_ra", ctypes.c_int), ("n_i", ctypes.c_int),
+                ("ra2i_ptr", c_int_p), ("ra2i_tgt", c_int_p), ("ra2i_w", c_double_p), ("ra2i_d", c_double_p),
+                ("i2ra_ptr", c_int_p), ("i2ra_tgt", c_int_p), ("i2ra_w", c_double_p), ("i2ra_d", c_double_p),
+                ("rr_ptr", c_int_p), ("rr_tgt", c_int_p), ("rr_d", c_double_p), ("rr_w", c_double_p),
+                ("act_cnt", c_int_p), ("act_ids", c_int_p), ("sup_cnt", c_int_p), ("sup_ids", c_int_p),
+                ("remodeled", c_int_p), ("erest", c_double_p), ("gca", c_double_p),
+                ("n_train", ctypes.c_int), ("train_ids", c_int_p), ("spread", c_double_p)]
+
+
+class Trial(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("training", ctypes.c_int), ("trial_index", ctypes.c_uint32),
+                ("first_trial", ctypes.c_int), ("mean_injection_time", ctypes.c_double)]
+
+
+class TrialOut(ctypes.Structure):
+    _fields_ = [("soma_counts", c_int_p), ("soma_times", c_double_p), ("soma_cap", ctypes.c_int),
+                ("soma_total", ctypes.c_int),
+                ("dend_counts", c_int_p), ("dend_times", c_double_p), ("dend_cap", ctypes.c_int),
+                ("dend_total", ctypes.c_int),
+                ("i_counts", c_int_p), ("i_times", c_double_p), ("i_cap", ctypes.c_int),
+                ("i_total", ctypes.c_int),
+                ("bad_values", ctypes.c_int), ("warnings", ctypes.c_int * 3),
+                ("ra_state", c_double_p), ("i_state", c_double_p)]
+
+
+class Synapses(ctypes.Structure):
+    _fields_ = [("w", c_double_p), ("act_cnt", c_int_p), ("act_ids", c_int_p), ("sup_cnt", c_int_p),
+                ("sup_ids", c_int_p), ("remodeled", c_int_p)]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libhvc_engine.so (once) and declare the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineNotBuilt("HIP engine library not built: %s (run python -m unsupervised_growth_amd.build)" % path)
+    try:
+        lib = ctypes.CDLL(path)
+    except OSError as e:
+        raise EngineNotBuilt("cannot load %s: %s" % (path, e))
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    lib.hvc_create.argtypes = [P(Params), ctypes.c_int, P(vp)]
+    lib.hvc_upload_network.argtypes = [vp, P(NetworkDesc)]
+    lib.hvc_run_trial.argtypes = [vp, P(Trial), P(TrialOut)]
+    lib.hvc_download_synapses.argtypes = [vp, P(Synapses)]
+    lib.hvc_upload_synapses.argtypes = [vp, P(Synapses)]
+    lib.hvc_patch_neurons.argtypes = [vp, c_int_p, ctypes.c_int, c_double_p, c_double_p]
+    lib.hvc_trace_ra.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, ctypes.c_uint32,
+                                 c_double_p, c_double_p, c_double_p, c_double_p, c_u8_p]
+    lib.hvc_trace_i.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                c_double_p, c_double_p, c_double_p, c_u8_p]
+    lib.hvc_trial_spikes.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
+    lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
+    lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
+    lib.hvc_destroy.argtypes = [vp]
+    lib.hvc_destroy.restype = None
+    lib.hvc_last_error.argtypes = [vp]
+    lib.hvc_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("hvc_destroy", "hvc_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib

@@ -1,0 +1,802 @@
+// hvc_engine.cpp -- host side of the HVC engine: the C ABI of include/hvc_engine.h.
+//
+// A context owns one HIP stream and every device buffer of one network.  The
+// trial clock (NGS:8906-8925), the synchronisation steps (NGS:9291, 9563), the
+// training-kick steps (NGS:8933-8940) and the stage-time offsets of the
+// steppers are evaluated here in IEEE binary64 exactly as the reference
+// evaluates them; the kernels in hvc_kernels.hip consume them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hvc_internal.h"
+#include "hvc_neuron.hip.h"
+
+namespace hvc {
+void launch_decay_table(hipStream_t st, const double* clock, const double* offsets, int num_steps, void* rows);
+void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
+                     const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
+                     uint8_t* fired);
+void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
+                    const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
+void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_trial, int E);
+void launch_integrate_ra(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
+void launch_integrate_i(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial);
+void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
+void launch_sync(hipStream_t st, const NetDev& nd);
+void launch_decay(hipStream_t st, const NetDev& nd);
+void launch_chain_edges(hipStream_t st, const NetDev& nd);
+}  // namespace hvc
+
+using namespace hvc;
+
+namespace {
+
+enum TimerClass { kTInt = 0, kTIntI, kTDeliver, kTSync, kTDecay, kNumTimers };
+const char* kTimerNames[kNumTimers] = {"integrate_ra", "integrate_i", "deliver", "sync", "decay"};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct hvc_ctx {
+    hvc_params prm;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<DevBuf> bufs;
+
+    int num_steps = 0;
+    std::vector<double> clock;
+    std::vector<std::pair<int, int>> windows;  // (t0, t1) inclusive, t1 = sync step
+    double offsets[kNumOffsets];
+    double* d_clock = nullptr;
+    void* d_decay = nullptr;
+    double* d_offsets = nullptr;
+
+    bool uploaded = false;
+    NetDev nd{};
+    int n_ra = 0, n_i = 0, E = 0;
+    std::vector<int> rr_ptr, rr_tgt;
+    std::vector<int> train_ids;
+    std::vector<double> spread;
+    int* d_kick = nullptr;
+    std::vector<int> h_kick;
+
+    bool trial_done = false;
+    int log_counts[3] = {0, 0, 0};
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev[2 * kNumTimers] = {};
+    double t_sum[kNumTimers] = {};
+    int t_cnt[kNumTimers] = {};
+};
+
+#define HVC_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+            return HVC_E_HIP;                                                                  \
+        }                                                                                      \
+    } while (0)
+
+static hvc_status fail(hvc_ctx* ctx, hvc_status s, const std::string& msg)
+{
+    ctx->err = msg;
+    return s;
+}
+
+template <class T>
+static T* dalloc(hvc_ctx* ctx, size_t count, hvc_status* st)
+{
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        *st = HVC_E_OOM;
+        ctx->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
+        return nullptr;
+    }
+    hipMemsetAsync(p, 0, bytes, ctx->stream);
+    ctx->bufs.push_back(DevBuf{p, bytes});
+    return (T*)p;
+}
+
+static void free_bufs(hvc_ctx* ctx)
+{
+    for (auto& b : ctx->bufs) hipFree(b.p);
+    ctx->bufs.clear();
+}
+
+extern "C" const char* hvc_last_error(const hvc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** out)
+{
+    if (!prm || !out) return HVC_E_ARG;
+    *out = nullptr;
+    hvc_ctx* ctx = new hvc_ctx();
+    ctx->prm = *prm;
+    ctx->device = device;
+    auto bad = [&](const char* m) {
+        ctx->err = m;
+        return HVC_E_ARG;
+    };
+    if (!(prm->dt > 0) || !(prm->trial_ms > 0) || !(prm->sync_ms > 0)) { hvc_status s = bad("dt, trial_ms, sync_ms must be > 0"); *out = ctx; return s; }
+    if (prm->ra_model < 0 || prm->ra_model > 1 || prm->i_model < 0 || prm->i_model > 1) { hvc_status s = bad("model must be 0 or 1"); *out = ctx; return s; }
+    if (prm->precision != 0) { hvc_status s = bad("only precision 0 (FP64) is available in this build"); *out = ctx; return s; }
+    if (prm->ra_poisson && prm->ra_model != 1) { hvc_status s = bad("RA Poisson noise is part of the RK4 (HH2_buffer) form only"); *out = ctx; return s; }
+    if (!(prm->syn.ACTIVATION_THRESHOLD > 0) || !(prm->syn.SUPERSYNAPSE_THRESHOLD > 0) || prm->syn.Nss < 0) {
+        hvc_status s = bad("synaptic thresholds must be > 0 and Nss >= 0");
+        *out = ctx;
+        return s;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        ctx->err = "no HIP device";
+        *out = ctx;
+        return HVC_E_HIP;
+    }
+    if (device < 0 || device >= ndev) { hvc_status s = bad("device index out of range"); *out = ctx; return s; }
+    *out = ctx;
+    HVC_TRY(hipSetDevice(device));
+    HVC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int k = 0; k < 2 * kNumTimers; ++k) HVC_TRY(hipEventCreate(&ctx->ev[k]));
+
+    // trial clock and synchronisation points (NGS:8906-8925, 9291, 9563)
+    const double dt = prm->dt;
+    ctx->num_steps = (int)(prm->trial_ms / dt);
+    if (ctx->num_steps < 2) return fail(ctx, HVC_E_ARG, "trial shorter than two steps");
+    ctx->clock.assign(ctx->num_steps, 0.0);
+    double internal_time = 0, network_time = prm->sync_ms;
+    int t0 = 1;
+    for (int t = 1; t < ctx->num_steps; ++t) {
+        internal_time += dt;
+        ctx->clock[t] = internal_time;
+        if ((internal_time > network_time) || (t == ctx->num_steps - 1)) {
+            if (t - t0 + 1 > 64) return fail(ctx, HVC_E_ARG, "more than 64 steps between synchronisation points");
+            ctx->windows.emplace_back(t0, t);
+            t0 = t + 1;
+            network_time += prm->sync_ms;
+        }
+    }
+    // stage-time offsets of the steppers, written as the reference writes them
+    double* o = ctx->offsets;
+    o[kOffDt] = dt;
+    o[kOffHalf] = 0.5 * dt;
+    o[kOffThird] = dt / 3;
+    o[kOffTwoThird] = 2 * dt / 3;
+    o[kOffDp2] = dt / 18;
+    o[kOffDp3] = dt / 12;
+    o[kOffDp4] = dt / 8;
+    o[kOffDp5] = 5 * dt / 16;
+    o[kOffDp6] = 3 * dt / 8;
+    o[kOffDp7] = 59 * dt / 400;
+    o[kOffDp8] = 93 * dt / 200;
+    o[kOffDp9] = 5490023248 * dt / 9719169821;
+    o[kOffDp10] = 13 * dt / 20;
+    o[kOffDp11] = 1201146811 * dt / 12990119798;
+
+    hvc_status st = HVC_OK;
+    ctx->d_clock = dalloc<double>(ctx, ctx->num_steps, &st);
+    ctx->d_decay = dalloc<DecayRow>(ctx, ctx->num_steps, &st);
+    ctx->d_offsets = dalloc<double>(ctx, kNumOffsets, &st);
+    if (st != HVC_OK) return st;
+    HVC_TRY(hipMemcpyAsync(ctx->d_clock, ctx->clock.data(), sizeof(double) * ctx->num_steps, hipMemcpyHostToDevice,
+                           ctx->stream));
+    HVC_TRY(hipMemcpyAsync(ctx->d_offsets, ctx->offsets, sizeof(ctx->offsets), hipMemcpyHostToDevice, ctx->stream));
+    launch_decay_table(ctx->stream, ctx->d_clock, ctx->d_offsets, ctx->num_steps, ctx->d_decay);
+    HVC_TRY(hipGetLastError());
+    HVC_TRY(hipStreamSynchronize(ctx->stream));
+
+    NetDev& nd = ctx->nd;
+    nd.ra_model = prm->ra_model;
+    nd.i_model = prm->i_model;
+    nd.dt = dt;
+    nd.num_steps = ctx->num_steps;
+    nd.clock = ctx->d_clock;
+    nd.decay = ctx->d_decay;
+    nd.sq_sig_s = std::sqrt(dt) * prm->sigma_s;
+    nd.sq_sig_d = std::sqrt(dt) * prm->sigma_d;
+    nd.mu_term_s = 100000 * (0.0 + prm->mu_s) / 5000.0;
+    nd.mu_term_d = 100000 * (0.0 + prm->mu_d) / 10000.0;
+    nd.ra_poisson = prm->ra_poisson;
+    nd.i_poisson = prm->i_poisson;
+    nd.seed_lo = prm->seed_lo;
+    nd.seed_hi = prm->seed_hi;
+    nd.syn = prm->syn;
+    return HVC_OK;
+}
+
+extern "C" void hvc_destroy(hvc_ctx* ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) {
+        hipStreamSynchronize(ctx->stream);
+        free_bufs(ctx);
+        for (int k = 0; k < 2 * kNumTimers; ++k)
+            if (ctx->ev[k]) hipEventDestroy(ctx->ev[k]);
+        hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+// ---------------------------------------------------------------------------
+// network upload
+// ---------------------------------------------------------------------------
+static bool check_csr(const int* ptr, const int* tgt, const double* d, int n_pre, int n_post, bool ascending,
+                      std::string& why)
+{
+    if (!ptr || ptr[0] != 0) { why = "CSR ptr[0] must be 0"; return false; }
+    for (int i = 0; i < n_pre; ++i) {
+        if (ptr[i + 1] < ptr[i]) { why = "CSR ptr not monotone"; return false; }
+        std::vector<int> seen;
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+            if (tgt[e] < 0 || tgt[e] >= n_post) { why = "CSR target out of range"; return false; }
+            if (!(d[e] >= 0) || !std::isfinite(d[e])) { why = "axonal delays must be finite and >= 0"; return false; }
+            if (ascending && e > ptr[i] && tgt[e] <= tgt[e - 1]) { why = "RA->RA targets must be strictly ascending"; return false; }
+            seen.push_back(tgt[e]);
+        }
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end()) { why = "duplicate connection"; return false; }
+    }
+    return true;
+}
+
+// edges of each row sorted by delay (stable), for the delay-queue walk
+static void sort_fixed(const int* ptr, const int* tgt, const double* w, const double* d, int n, std::vector<int>& otgt,
+                       std::vector<double>& ow, std::vector<double>& od)
+{
+    int E = ptr[n];
+    otgt.resize(E); ow.resize(E); od.resize(E);
+    std::vector<int> idx;
+    for (int i = 0; i < n; ++i) {
+        idx.resize(ptr[i + 1] - ptr[i]);
+        for (size_t k = 0; k < idx.size(); ++k) idx[k] = ptr[i] + (int)k;
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+        for (size_t k = 0; k < idx.size(); ++k) {
+            otgt[ptr[i] + k] = tgt[idx[k]];
+            ow[ptr[i] + k] = w[idx[k]];
+            od[ptr[i] + k] = d[idx[k]];
+        }
+    }
+}
+
+static hvc_status set_synapse_state(hvc_ctx* ctx, const double* w, const int* act_cnt, const int* act_ids,
+                                    const int* sup_cnt, const int* sup_ids, const int* remodeled)
+{
+    const int n = ctx->n_ra, E = ctx->E, Nss = ctx->prm.syn.Nss;
+    std::vector<uint8_t> act(E + 1, 0), sup(E + 1, 0);
+    std::vector<int> alist(E + 1, 0), slist((size_t)n * std::max(Nss, 1) + 1, 0);
+    std::vector<int> acnt(n), scnt(n), rem(n);
+    auto edge_of = [&](int i, int j) {
+        auto b = ctx->rr_tgt.begin() + ctx->rr_ptr[i], e = ctx->rr_tgt.begin() + ctx->rr_ptr[i + 1];
+        auto it = std::lower_bound(b, e, j);
+        return (it != e && *it == j) ? (int)(it - ctx->rr_tgt.begin()) : -1;
+    };
+    int oa = 0, os = 0;
+    for (int i = 0; i < n; ++i) {
+        int len = ctx->rr_ptr[i + 1] - ctx->rr_ptr[i];
+        int a = act_cnt ? act_cnt[i] : 0, s = sup_cnt ? sup_cnt[i] : 0;
+        if (a < 0 || a > len || s < 0 || s > len || s > Nss) return fail(ctx, HVC_E_ARG, "active/super list size out of range");
+        acnt[i] = a; scnt[i] = s;
+        for (int k = 0; k < a; ++k) {
+            int e = edge_of(i, act_ids[oa++]);
+            if (e < 0 || act[e]) return fail(ctx, HVC_E_ARG, "active synapse is not a candidate or repeated");
+            act[e] = 1;
+            alist[ctx->rr_ptr[i] + k] = ctx->rr_tgt[e];
+        }
+        for (int k = 0; k < s; ++k) {
+            int e = edge_of(i, sup_ids[os++]);
+            if (e < 0 || sup[e]) return fail(ctx, HVC_E_ARG, "supersynapse is not a candidate or repeated");
+            sup[e] = 1;
+            slist[(size_t)i * Nss + k] = ctx->rr_tgt[e];
+        }
+        rem[i] = remodeled ? remodeled[i] : 0;
+    }
+    NetDev& nd = ctx->nd;
+    hipStream_t st = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(nd.w, w, sizeof(double) * E, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.act, act.data(), E, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.sup, sup.data(), E, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.act_list, alist.data(), sizeof(int) * E, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.sup_list, slist.data(), sizeof(int) * (size_t)n * Nss, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.act_cnt, acnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.sup_cnt, scnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.remodeled, rem.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipStreamSynchronize(st));
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
+{
+    if (!ctx || !net) return HVC_E_ARG;
+    if (ctx->uploaded) return fail(ctx, HVC_E_STATE, "network already uploaded to this context");
+    const int nra = net->n_ra, ni = net->n_i;
+    if (nra <= 0 || ni < 0) return fail(ctx, HVC_E_ARG, "n_ra must be > 0 and n_i >= 0");
+    std::string why;
+    if (!check_csr(net->ra2i_ptr, net->ra2i_tgt, net->ra2i_d, nra, ni, false, why)) return fail(ctx, HVC_E_ARG, "RA->I: " + why);
+    if (!check_csr(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_d, ni, nra, false, why)) return fail(ctx, HVC_E_ARG, "I->RA: " + why);
+    if (!check_csr(net->rr_ptr, net->rr_tgt, net->rr_d, nra, nra, true, why)) return fail(ctx, HVC_E_ARG, "RA->RA: " + why);
+    for (int k = 0; k < net->n_train; ++k)
+        if (net->train_ids[k] < 0 || net->train_ids[k] >= nra) return fail(ctx, HVC_E_ARG, "training id out of range");
+
+    ctx->n_ra = nra;
+    ctx->n_i = ni;
+    ctx->E = net->rr_ptr[nra];
+    ctx->rr_ptr.assign(net->rr_ptr, net->rr_ptr + nra + 1);
+    ctx->rr_tgt.assign(net->rr_tgt, net->rr_tgt + ctx->E);
+    ctx->train_ids.assign(net->train_ids, net->train_ids + net->n_train);
+    ctx->spread.assign(net->spread, net->spread + net->n_train);
+
+    // per row: candidates sorted by (delay, target).  Two different delays whose
+    // sums with a spike time could round to the same delivery time would make
+    // that order differ from the reference's (time, target) order; reject them.
+    std::vector<int> perm(ctx->E);
+    for (int i = 0; i < nra; ++i) {
+        int b = net->rr_ptr[i], e = net->rr_ptr[i + 1];
+        for (int k = b; k < e; ++k) perm[k] = k;
+        std::sort(perm.begin() + b, perm.begin() + e, [&](int x, int y) {
+            if (net->rr_d[x] != net->rr_d[y]) return net->rr_d[x] < net->rr_d[y];
+            return net->rr_tgt[x] < net->rr_tgt[y];
+        });
+        for (int k = b + 1; k < e; ++k) {
+            double d0 = net->rr_d[perm[k - 1]], d1 = net->rr_d[perm[k]];
+            if (d0 != d1 && d1 - d0 < 1e-9 * std::max(1.0, ctx->prm.trial_ms))
+                return fail(ctx, HVC_E_ARG, "RA->RA delays of one row differ by less than 1e-9 * trial length");
+        }
+    }
+    int dense = 1;
+    for (int i = 0; i < nra && dense; ++i) {
+        if (net->rr_ptr[i + 1] - net->rr_ptr[i] != nra) { dense = 0; break; }
+        for (int j = 0; j < nra; ++j)
+            if (net->rr_tgt[net->rr_ptr[i] + j] != j) { dense = 0; break; }
+    }
+
+    std::vector<int> a_tgt, b_tgt;
+    std::vector<double> a_w, a_d, b_w, b_d;
+    sort_fixed(net->ra2i_ptr, net->ra2i_tgt, net->ra2i_w, net->ra2i_d, nra, a_tgt, a_w, a_d);
+    sort_fixed(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_w, net->i2ra_d, ni, b_tgt, b_w, b_d);
+
+    // inbox capacities: largest in-degree per connection type, plus slack
+    auto max_indeg = [](const int* ptr, const int* tgt, int n_pre, int n_post, int skip_self) {
+        std::vector<int> deg(std::max(n_post, 1), 0);
+        for (int i = 0; i < n_pre; ++i)
+            for (int e = ptr[i]; e < ptr[i + 1]; ++e)
+                if (!(skip_self && tgt[e] == i)) deg[tgt[e]]++;
+        return *std::max_element(deg.begin(), deg.end());
+    };
+    int cap_gi_ra = max_indeg(net->i2ra_ptr, net->i2ra_tgt, ni, nra, 0) + 8;
+    int cap_ge_ra = max_indeg(net->rr_ptr, net->rr_tgt, nra, nra, 1) + 8;
+    int cap_ge_i = max_indeg(net->ra2i_ptr, net->ra2i_tgt, nra, ni, 0) + 8;
+
+    hvc_status st = HVC_OK;
+    NetDev& nd = ctx->nd;
+    nd.n_ra = nra;
+    nd.n_i = ni;
+    nd.dense = dense;
+    const int E = ctx->E;
+    const int Nss = ctx->prm.syn.Nss;
+    for (int k = 0; k < 12; ++k) nd.ra[k] = dalloc<double>(ctx, nra, &st);
+    nd.ra_flags = dalloc<int>(ctx, nra, &st);
+    nd.ra_noise = dalloc<double>(ctx, 4 * (size_t)nra, &st);
+    nd.ra_draw = dalloc<uint32_t>(ctx, nra, &st);
+    double* d_EL = dalloc<double>(ctx, nra, &st);
+    double* d_GCa = dalloc<double>(ctx, nra, &st);
+    nd.EL = d_EL;
+    nd.GCa = d_GCa;
+    ctx->d_kick = dalloc<int>(ctx, nra, &st);
+    nd.kick_step = ctx->d_kick;
+    for (int k = 0; k < 7; ++k) nd.in[k] = dalloc<double>(ctx, ni, &st);
+    nd.i_flag = dalloc<int>(ctx, ni, &st);
+    nd.i_noise = dalloc<double>(ctx, 2 * (size_t)ni, &st);
+    nd.i_draw = dalloc<uint32_t>(ctx, ni, &st);
+    nd.soma_mask = dalloc<uint64_t>(ctx, nra, &st);
+    nd.dend_mask = dalloc<uint64_t>(ctx, nra, &st);
+    nd.i_mask = dalloc<uint64_t>(ctx, ni, &st);
+    int cap = ctx->prm.spike_capacity > 0 ? ctx->prm.spike_capacity : std::max(1 << 16, 256 * (nra + ni));
+    nd.log_cap = cap;
+    nd.soma_log = dalloc<int2>(ctx, cap, &st);
+    nd.dend_log = dalloc<int2>(ctx, cap, &st);
+    nd.i_log = dalloc<int2>(ctx, cap, &st);
+    nd.log_cnt = dalloc<int>(ctx, 4, &st);
+    nd.post_ring = dalloc<double>(ctx, (size_t)nra * kPostRing, &st);
+    nd.post_cnt = dalloc<int>(ctx, nra, &st);
+    nd.post_pend = dalloc<int>(ctx, nra, &st);
+    nd.post_list = dalloc<uint64_t>(ctx, kPostListCap, &st);
+    nd.post_list_cnt = dalloc<int>(ctx, 1, &st);
+    nd.blocks_done = dalloc<unsigned int>(ctx, 1, &st);
+    int* d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
+    int* d_rr_tgt = dalloc<int>(ctx, E, &st);
+    double* d_rr_d = dalloc<double>(ctx, E, &st);
+    int* d_perm = dalloc<int>(ctx, E, &st);
+    nd.rr_ptr = d_rr_ptr; nd.rr_tgt = d_rr_tgt; nd.rr_d = d_rr_d; nd.rr_perm = d_perm;
+    nd.w = dalloc<double>(ctx, E, &st);
+    nd.act = dalloc<uint8_t>(ctx, E, &st);
+    nd.sup = dalloc<uint8_t>(ctx, E, &st);
+    nd.act_list = dalloc<int>(ctx, E, &st);
+    nd.act_cnt = dalloc<int>(ctx, nra, &st);
+    nd.sup_list = dalloc<int>(ctx, (size_t)nra * std::max(Nss, 1), &st);
+    nd.sup_cnt = dalloc<int>(ctx, nra, &st);
+    nd.remodeled = dalloc<int>(ctx, nra, &st);
+    nd.pre_ring = dalloc<double>(ctx, (size_t)E * kPreRing, &st);
+    nd.pre_cnt = dalloc<int>(ctx, E, &st);
+    nd.st_step = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_mode = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_cursor = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_n = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_count = dalloc<int>(ctx, nra, &st);
+    nd.st_items = dalloc<int>(ctx, (size_t)nra * kMaxStreams * std::max(Nss, 1), &st);
+    int* d_a_ptr = dalloc<int>(ctx, nra + 1, &st);
+    int* d_a_tgt = dalloc<int>(ctx, a_tgt.size(), &st);
+    double* d_a_w = dalloc<double>(ctx, a_w.size(), &st);
+    double* d_a_d = dalloc<double>(ctx, a_d.size(), &st);
+    int* d_b_ptr = dalloc<int>(ctx, ni + 1, &st);
+    int* d_b_tgt = dalloc<int>(ctx, b_tgt.size(), &st);
+    double* d_b_w = dalloc<double>(ctx, b_w.size(), &st);
+    double* d_b_d = dalloc<double>(ctx, b_d.size(), &st);
+    nd.ra2i = FixedEdges{d_a_ptr, d_a_tgt, d_a_w, d_a_d};
+    nd.i2ra = FixedEdges{d_b_ptr, d_b_tgt, d_b_w, d_b_d};
+    auto mk_inflight = [&](int n) {
+        InFlight f;
+        f.step = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
+        f.cursor = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
+        f.head = dalloc<int>(ctx, n, &st);
+        f.count = dalloc<int>(ctx, n, &st);
+        return f;
+    };
+    nd.fl_ra2i = mk_inflight(nra);
+    nd.fl_i2ra = mk_inflight(ni);
+    nd.fl_rr = mk_inflight(nra);
+    nd.chain_e = dalloc<int>(ctx, E, &st);
+    auto mk_inbox = [&](int n, int c) {
+        Inbox b;
+        b.cap = c;
+        b.rec = dalloc<Record>(ctx, (size_t)std::max(n, 1) * c, &st);
+        b.cnt = dalloc<int>(ctx, n, &st);
+        return b;
+    };
+    nd.in_gi_ra = mk_inbox(nra, cap_gi_ra);
+    nd.in_ge_ra = mk_inbox(nra, cap_ge_ra);
+    nd.in_ge_i = mk_inbox(ni, cap_ge_i);
+    nd.err = dalloc<unsigned int>(ctx, 1, &st);
+    nd.warn = dalloc<int>(ctx, 4, &st);
+    if (st != HVC_OK) {
+        free_bufs(ctx);
+        return st;
+    }
+
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(d_EL, net->erest, sizeof(double) * nra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_GCa, net->gca, sizeof(double) * nra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_rr_ptr, net->rr_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_a_ptr, net->ra2i_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_a_tgt, a_tgt.data(), sizeof(int) * a_tgt.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_a_w, a_w.data(), sizeof(double) * a_w.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_a_d, a_d.data(), sizeof(double) * a_d.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_b_ptr, net->i2ra_ptr, sizeof(int) * (ni + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_b_tgt, b_tgt.data(), sizeof(int) * b_tgt.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_b_w, b_w.data(), sizeof(double) * b_w.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_b_d, b_d.data(), sizeof(double) * b_d.size(), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    ctx->uploaded = true;
+    return set_synapse_state(ctx, net->rr_w, net->act_cnt, net->act_ids, net->sup_cnt, net->sup_ids, net->remodeled);
+}
+
+extern "C" hvc_status hvc_upload_synapses(hvc_ctx* ctx, const hvc_synapses* syn)
+{
+    if (!ctx || !syn) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    return set_synapse_state(ctx, syn->w, syn->act_cnt, syn->act_ids, syn->sup_cnt, syn->sup_ids, syn->remodeled);
+}
+
+extern "C" hvc_status hvc_download_synapses(hvc_ctx* ctx, hvc_synapses* syn)
+{
+    if (!ctx || !syn) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    const int n = ctx->n_ra, E = ctx->E, Nss = ctx->prm.syn.Nss;
+    NetDev& nd = ctx->nd;
+    std::vector<int> alist(E), slist((size_t)n * std::max(Nss, 1)), acnt(n), scnt(n), rem(n);
+    hipStream_t s = ctx->stream;
+    if (syn->w) HVC_TRY(hipMemcpyAsync(syn->w, nd.w, sizeof(double) * E, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(alist.data(), nd.act_list, sizeof(int) * E, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(slist.data(), nd.sup_list, sizeof(int) * (size_t)n * Nss, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(acnt.data(), nd.act_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(scnt.data(), nd.sup_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(rem.data(), nd.remodeled, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    int oa = 0, os = 0;
+    for (int i = 0; i < n; ++i) {
+        if (syn->act_cnt) syn->act_cnt[i] = acnt[i];
+        if (syn->sup_cnt) syn->sup_cnt[i] = scnt[i];
+        if (syn->remodeled) syn->remodeled[i] = rem[i];
+        if (syn->act_ids) for (int k = 0; k < acnt[i]; ++k) syn->act_ids[oa++] = alist[ctx->rr_ptr[i] + k];
+        if (syn->sup_ids) for (int k = 0; k < scnt[i]; ++k) syn->sup_ids[os++] = slist[(size_t)i * Nss + k];
+    }
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, const double* erest, const double* gca)
+{
+    if (!ctx || (n > 0 && (!ids || !erest || !gca))) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    for (int k = 0; k < n; ++k) {
+        if (ids[k] < 0 || ids[k] >= ctx->n_ra) return fail(ctx, HVC_E_ARG, "neuron id out of range");
+        HVC_TRY(hipMemcpyAsync((double*)ctx->nd.EL + ids[k], erest + k, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HVC_TRY(hipMemcpyAsync((double*)ctx->nd.GCa + ids[k], gca + k, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    HVC_TRY(hipStreamSynchronize(ctx->stream));
+    return HVC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// trials
+// ---------------------------------------------------------------------------
+static void tstart(hvc_ctx* ctx, int k, bool on)
+{
+    if (on) hipEventRecord(ctx->ev[2 * k], ctx->stream);
+}
+static void tstop(hvc_ctx* ctx, int k, bool on)
+{
+    if (!on) return;
+    hipEventRecord(ctx->ev[2 * k + 1], ctx->stream);
+    hipEventSynchronize(ctx->ev[2 * k + 1]);
+    float ms = 0;
+    hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]);
+    ctx->t_sum[k] += ms;
+    ctx->t_cnt[k] += 1;
+}
+
+static hvc_status collect_log(hvc_ctx* ctx, const int2* d_log, int count, int n, int* counts, double* times, int cap,
+                              int* total)
+{
+    *total = count;
+    if (!counts) return HVC_OK;
+    if (times && count > cap) return fail(ctx, HVC_E_ARG, "spike-time buffer too small");
+    std::vector<int2> log(count);
+    if (count > 0) HVC_TRY(hipMemcpy(log.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
+    std::sort(log.begin(), log.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    std::fill(counts, counts + n, 0);
+    for (int k = 0; k < count; ++k) {
+        counts[log[k].x]++;
+        if (times) times[k] = ctx->clock[log[k].y];
+    }
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled)
+{
+    if (!ctx) return HVC_E_ARG;
+    ctx->timing = enabled != 0;
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches)
+{
+    if (!ctx || !name) return HVC_E_ARG;
+    for (int k = 0; k < kNumTimers; ++k) {
+        if (std::strcmp(name, kTimerNames[k]) == 0) {
+            if (avg_ms) *avg_ms = ctx->t_cnt[k] ? ctx->t_sum[k] / ctx->t_cnt[k] : 0.0;
+            if (launches) *launches = ctx->t_cnt[k];
+            return HVC_OK;
+        }
+    }
+    return fail(ctx, HVC_E_ARG, std::string("unknown kernel class ") + name);
+}
+
+extern "C" hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, double* times, int cap, int* total)
+{
+    if (!ctx || kind < 0 || kind > 2 || !total) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    if (!ctx->trial_done) return fail(ctx, HVC_E_STATE, "no trial has run");
+    const int2* log = kind == 0 ? ctx->nd.soma_log : (kind == 1 ? ctx->nd.dend_log : ctx->nd.i_log);
+    int n = kind == 2 ? ctx->n_i : ctx->n_ra;
+    return collect_log(ctx, log, ctx->log_counts[kind], n, counts, times, cap, total);
+}
+
+extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial_out* out)
+{
+    if (!ctx || !tr) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    if (tr->mode != 0 && tr->mode != 1) return fail(ctx, HVC_E_ARG, "mode must be 0 (growth) or 1 (chain test)");
+    HVC_TRY(hipSetDevice(ctx->device));
+    const int growth = tr->mode == 0;
+    const int training = growth ? tr->training : 1;
+    NetDev& nd = ctx->nd;
+    hipStream_t s = ctx->stream;
+
+    // training kick: first step with internal_time > spread + mean_injection_time (NGS:8933-8940)
+    ctx->h_kick.assign(ctx->n_ra, -1);
+    if (training) {
+        for (size_t k = 0; k < ctx->train_ids.size(); ++k) {
+            double thr = ctx->spread[k] + tr->mean_injection_time;
+            for (int t = 1; t < ctx->num_steps; ++t)
+                if (ctx->clock[t] > thr) {
+                    ctx->h_kick[ctx->train_ids[k]] = t;
+                    break;
+                }
+        }
+    }
+    HVC_TRY(hipMemcpyAsync(ctx->d_kick, ctx->h_kick.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < kNumTimers; ++k) { ctx->t_sum[k] = 0; ctx->t_cnt[k] = 0; }
+
+    launch_reset(s, nd, tr->trial_index, tr->first_trial, ctx->E);
+    if (!growth) launch_chain_edges(s, nd);
+    const int nw = (int)ctx->windows.size();
+    for (int w = 0; w < nw; ++w) {
+        int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
+        bool tm = ctx->timing && (w % 16 == 7);
+        tstart(ctx, kTInt, tm);
+        launch_integrate_ra(s, nd, t0, t1, tr->trial_index, growth);
+        tstop(ctx, kTInt, tm);
+        tstart(ctx, kTIntI, tm);
+        launch_integrate_i(s, nd, t0, t1, tr->trial_index);
+        tstop(ctx, kTIntI, tm);
+        tstart(ctx, kTDeliver, tm);
+        launch_deliver(s, nd, t0, t1, growth);
+        tstop(ctx, kTDeliver, tm);
+        tstart(ctx, kTSync, tm);
+        launch_sync(s, nd);
+        tstop(ctx, kTSync, tm);
+    }
+    if (growth) {
+        tstart(ctx, kTDecay, ctx->timing);
+        launch_decay(s, nd);
+        tstop(ctx, kTDecay, ctx->timing);
+    }
+    HVC_TRY(hipGetLastError());
+    unsigned int err = 0;
+    int logc[4] = {0, 0, 0, 0}, warn[4] = {0, 0, 0, 0};
+    HVC_TRY(hipMemcpyAsync(&err, nd.err, sizeof(err), hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(logc, nd.log_cnt, sizeof(logc), hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(warn, nd.warn, sizeof(warn), hipMemcpyDeviceToHost, s));
+    std::vector<double> vs(ctx->n_ra), vd(ctx->n_ra);
+    HVC_TRY(hipMemcpyAsync(vs.data(), nd.ra[0], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(vd.data(), nd.ra[1], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    ctx->trial_done = true;
+    for (int k = 0; k < 3; ++k) ctx->log_counts[k] = logc[k];
+    if (err) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "engine capacity exceeded (flags 0x%x); trial result discarded", err);
+        return fail(ctx, HVC_E_CAPACITY, buf);
+    }
+    // check_bad_values (HH2_test.cpp:244-260)
+    int bad = 0;
+    for (int i = 0; i < ctx->n_ra; ++i) {
+        if ((vs[i] > 100) || (vs[i] < -200) || std::isnan(vs[i]) || (vd[i] > 100) || (vd[i] < -200) || std::isnan(vd[i])) {
+            bad = -1;
+            break;
+        }
+    }
+    if (out) {
+        hvc_status r;
+        if ((r = collect_log(ctx, nd.soma_log, logc[0], ctx->n_ra, out->soma_counts, out->soma_times, out->soma_cap,
+                             &out->soma_total)) != HVC_OK) return r;
+        if ((r = collect_log(ctx, nd.dend_log, logc[1], ctx->n_ra, out->dend_counts, out->dend_times, out->dend_cap,
+                             &out->dend_total)) != HVC_OK) return r;
+        if ((r = collect_log(ctx, nd.i_log, logc[2], ctx->n_i, out->i_counts, out->i_times, out->i_cap,
+                             &out->i_total)) != HVC_OK) return r;
+        out->bad_values = bad;
+        out->warnings[0] = warn[0]; out->warnings[1] = warn[1]; out->warnings[2] = warn[2];
+        if (out->ra_state) {
+            std::vector<double> col(ctx->n_ra);
+            for (int k = 0; k < 12; ++k) {
+                HVC_TRY(hipMemcpy(col.data(), nd.ra[k], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost));
+                for (int i = 0; i < ctx->n_ra; ++i) out->ra_state[(size_t)i * 12 + k] = col[i];
+            }
+        }
+        if (out->i_state && ctx->n_i > 0) {
+            std::vector<double> col(ctx->n_i);
+            for (int k = 0; k < 7; ++k) {
+                HVC_TRY(hipMemcpy(col.data(), nd.in[k], sizeof(double) * ctx->n_i, hipMemcpyDeviceToHost));
+                for (int i = 0; i < ctx->n_i; ++i) out->i_state[(size_t)i * 7 + k] = col[i];
+            }
+        }
+    }
+    if (bad) return fail(ctx, HVC_E_NUMERIC, "check_bad_values: HVC-RA voltage out of [-200, 100] or NaN");
+    return HVC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-neuron traces
+// ---------------------------------------------------------------------------
+static hvc_status trace_tables(hvc_ctx* ctx, int n_steps, double** d_clock, void** d_decay)
+{
+    std::vector<double> clk(n_steps + 1, 0.0);
+    double t = 0;
+    for (int s = 1; s <= n_steps; ++s) {
+        t += ctx->prm.dt;
+        clk[s] = t;
+    }
+    HVC_TRY(hipMalloc((void**)d_clock, sizeof(double) * (n_steps + 1)));
+    HVC_TRY(hipMalloc(d_decay, sizeof(DecayRow) * (n_steps + 1)));
+    HVC_TRY(hipMemcpyAsync(*d_clock, clk.data(), sizeof(double) * (n_steps + 1), hipMemcpyHostToDevice, ctx->stream));
+    launch_decay_table(ctx->stream, *d_clock, ctx->d_offsets, n_steps + 1, *d_decay);
+    HVC_TRY(hipGetLastError());
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const double* erest, const double* gca,
+                                   uint32_t trial, const double* kick, const double* exc_w, const double* inh_g,
+                                   double* out_state, uint8_t* out_fired)
+{
+    if (!ctx || n <= 0 || n_steps <= 0 || !erest || !gca || !kick || !exc_w || !inh_g || !out_state || !out_fired)
+        return HVC_E_ARG;
+    HVC_TRY(hipSetDevice(ctx->device));
+    double* d_clock = nullptr;
+    void* d_decay = nullptr;
+    hvc_status r = trace_tables(ctx, n_steps, &d_clock, &d_decay);
+    if (r != HVC_OK) return r;
+    size_t in_sz = (size_t)n_steps * n;
+    double *d_el, *d_gca, *d_kick, *d_exc, *d_inh, *d_out;
+    uint8_t* d_fired;
+    HVC_TRY(hipMalloc(&d_el, sizeof(double) * n));
+    HVC_TRY(hipMalloc(&d_gca, sizeof(double) * n));
+    HVC_TRY(hipMalloc(&d_kick, sizeof(double) * in_sz));
+    HVC_TRY(hipMalloc(&d_exc, sizeof(double) * in_sz));
+    HVC_TRY(hipMalloc(&d_inh, sizeof(double) * in_sz));
+    HVC_TRY(hipMalloc(&d_out, sizeof(double) * (in_sz + n) * 12));
+    HVC_TRY(hipMalloc(&d_fired, in_sz));
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(d_el, erest, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_gca, gca, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_kick, kick, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_exc, exc_w, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_inh, inh_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
+    NetDev nd = ctx->nd;
+    nd.clock = d_clock;
+    nd.decay = d_decay;
+    launch_trace_ra(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
+    HVC_TRY(hipGetLastError());
+    HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 12, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    hipFree(d_el); hipFree(d_gca); hipFree(d_kick); hipFree(d_exc); hipFree(d_inh); hipFree(d_out); hipFree(d_fired);
+    hipFree(d_clock); hipFree(d_decay);
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32_t trial,
+                                  const double* exc_g, const double* inh_g, double* out_state, uint8_t* out_fired)
+{
+    if (!ctx || n <= 0 || n_steps <= 0 || !exc_g || !inh_g || !out_state || !out_fired) return HVC_E_ARG;
+    HVC_TRY(hipSetDevice(ctx->device));
+    double* d_clock = nullptr;
+    void* d_decay = nullptr;
+    hvc_status r = trace_tables(ctx, n_steps, &d_clock, &d_decay);
+    if (r != HVC_OK) return r;
+    size_t in_sz = (size_t)n_steps * n;
+    double *d_exc, *d_inh, *d_out;
+    uint8_t* d_fired;
+    HVC_TRY(hipMalloc(&d_exc, sizeof(double) * in_sz));
+    HVC_TRY(hipMalloc(&d_inh, sizeof(double) * in_sz));
+    HVC_TRY(hipMalloc(&d_out, sizeof(double) * (in_sz + n) * 7));
+    HVC_TRY(hipMalloc(&d_fired, in_sz));
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(d_exc, exc_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_inh, inh_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
+    NetDev nd = ctx->nd;
+    nd.clock = d_clock;
+    nd.decay = d_decay;
+    launch_trace_i(s, nd, n, n_steps, first_trial, trial, d_exc, d_inh, d_out, d_fired);
+    HVC_TRY(hipGetLastError());
+    HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 7, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    hipFree(d_exc); hipFree(d_inh); hipFree(d_out); hipFree(d_fired);
+    hipFree(d_clock); hipFree(d_decay);
+    return HVC_OK;
+}

@@ -1,0 +1,130 @@
+// hvc_internal.h -- device data layout shared by the engine's kernels and host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hvc_engine.h"
+
+namespace hvc {
+
+constexpr int kPostRing = 8;     // post-event times kept per HVC-RA neuron
+constexpr int kPreRing = 4;      // delivered pre-event times kept per candidate edge
+constexpr int kMaxStreams = 16;  // in-flight RA->RA spikes per presynaptic row
+constexpr int kMaxInflight = 16; // in-flight spikes per neuron for fixed-weight pushes
+constexpr int kPostListCap = 4096;
+
+// error flags raised by kernels (bit mask)
+enum ErrFlag : uint32_t {
+    kErrSpikeLog = 1u,
+    kErrPostList = 2u,
+    kErrStreams = 4u,
+    kErrInflight = 8u,
+    kErrInbox = 16u,
+    kErrPostRing = 32u,
+};
+
+struct Record {
+    uint64_t key;   // (arrival step << 32) | presynaptic id
+    double w;
+};
+
+// Fixed-weight delivery edges sorted by delay within each presynaptic row.
+struct FixedEdges {
+    const int* ptr;
+    const int* tgt;
+    const double* w;
+    const double* d;
+};
+
+struct Inbox {
+    Record* rec;    // [n_targets][cap]
+    int* cnt;       // [n_targets]
+    int cap;
+};
+
+struct InFlight {
+    int* step;      // [n][kMaxInflight]
+    int* cursor;    // [n][kMaxInflight]
+    int* head;      // [n]
+    int* count;     // [n]
+};
+
+struct NetDev {
+    int n_ra, n_i;
+    int ra_model, i_model;
+    double dt;
+    int num_steps;
+    const double* clock;         // [num_steps], clock[t] = internal_time after step t
+    const void* decay;           // DecayRow [num_steps] (row t used by step t)
+    double sq_sig_s, sq_sig_d, mu_term_s, mu_term_d;
+    int ra_poisson, i_poisson;
+    uint32_t seed_lo, seed_hi;
+    hvc_synaptic_params syn;
+
+    // RA state (SoA)
+    double* ra[12];              // Vs Vd n l h r c Ca Ge_s Gi_s Ge_d Gi_d
+    int* ra_flags;               // bit0 flag_soma, bit1 flag_dend
+    double* ra_noise;            // [4][n_ra]: exc_soma inh_soma exc_dend inh_dend
+    uint32_t* ra_draw;
+    const double* EL;
+    const double* GCa;
+    const int* kick_step;        // step of the training kick, -1 none
+    // I state
+    double* in[7];               // V n m h w Ge Gi
+    int* i_flag;
+    double* i_noise;             // [2][n_i]: exc inh
+    uint32_t* i_draw;
+
+    // per-window outputs
+    uint64_t* soma_mask;
+    uint64_t* dend_mask;
+    uint64_t* i_mask;
+    int2* soma_log; int2* dend_log; int2* i_log;
+    int* log_cnt;                // [3]
+    int log_cap;
+
+    // post events (growth)
+    double* post_ring;           // [n_ra][kPostRing]
+    int* post_cnt;
+    int* post_pend;
+    uint64_t* post_list;         // sorted (step << 32 | id)
+    int* post_list_cnt;
+    unsigned int* blocks_done;
+
+    // RA->RA candidates
+    const int* rr_ptr;
+    const int* rr_tgt;
+    const double* rr_d;
+    const int* rr_perm;          // per row: candidate edges sorted by (delay, target)
+    double* w;
+    uint8_t* act;
+    uint8_t* sup;
+    int* act_list;               // [E] row-segmented
+    int* act_cnt;
+    int* sup_list;               // [n_ra][Nss]
+    int* sup_cnt;
+    int* remodeled;
+    double* pre_ring;            // [E][kPreRing]
+    int* pre_cnt;                // [E]
+    int dense;
+
+    // RA->RA in-flight streams (growth)
+    int* st_step;                // [n_ra][kMaxStreams]
+    int* st_mode;
+    int* st_cursor;
+    int* st_n;
+    int* st_count;               // [n_ra]
+    int* st_items;               // [n_ra][kMaxStreams][Nss] edge indices, sorted by (T, target)
+
+    // fixed-weight pushes
+    FixedEdges ra2i, i2ra;
+    InFlight fl_ra2i, fl_i2ra, fl_rr;
+    int* chain_e;                // chain test: row i's active edges sorted by (delay, target) at rr_ptr[i]
+
+    Inbox in_gi_ra, in_ge_ra, in_ge_i;
+
+    unsigned int* err;
+    int* warn;                   // [3]
+};
+
+}  // namespace hvc

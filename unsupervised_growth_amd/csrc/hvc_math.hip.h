@@ -1,0 +1,134 @@
+// hvc_math.hip.h -- device exp/log and counter-based noise for the HVC engine.
+//
+// The reference evaluates libm exp/log (HH2_test.h:265-276, HHI_buffer.h:173-180,
+// poisson_noise.cpp:14).  The engine defines its transcendental functions with
+// IEEE-754 +, -, *, /, fma and rint only, so every result is reproducible bit for
+// bit on any IEEE machine:
+//   exp: x = k*ln2/64 + r, exp(x) = 2^(k/64) (1 + p(r)), |r| <= ln2/128, with a
+//        double-double table of 2^(j/64) and p the degree-6 Taylor polynomial
+//        (Tang's table method, ACM TOMS 15(2), 1989); error ~0.505 ulp.
+//   log: log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))), s = f/(2+f) (the fdlibm
+//        reduction), R the Taylor series of 2 atanh(s)/s - 2 to degree 11.
+// Noise: Philox4x32-10 (Salmon et al., SC'11).  One call per (RA neuron, step)
+// gives the four Debraband point-distribution indices (HH2_test.cpp:640-643);
+// Poisson draws take one 32-bit word each (poisson_noise.cpp:9-21).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hvc_exp2_table.h"
+
+namespace hvc {
+
+__constant__ double kExp2Table[128] = HVC_EXP2_TABLE_INIT;
+
+constexpr uint32_t kStreamRAWhite = 0x52410000u;
+constexpr uint32_t kStreamIPoisson = 0x49000000u;
+constexpr uint32_t kStreamRAPoisson = 0x52500000u;
+
+__device__ __forceinline__ double u2d(uint64_t u) { return __longlong_as_double((long long)u); }
+__device__ __forceinline__ uint64_t d2u(double x) { return (uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ double pow2i(int e) { return u2d((uint64_t)(e + 1023) << 52); }
+
+__device__ __forceinline__ double dexp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return u2d(0x7ff0000000000000ull);
+    if (x < -745.1332191019412) return 0.0;
+    double kd = __builtin_rint(x * HVC_INV_LN2_64);
+    int k = (int)kd;
+    double r = __builtin_fma(-kd, HVC_LN2_64_HI, x);
+    r = __builtin_fma(-kd, HVC_LN2_64_LO, r);
+    double z = r * r;
+    double p = __builtin_fma(r, 1.0 / 720.0, 1.0 / 120.0);
+    p = __builtin_fma(r, p, 1.0 / 24.0);
+    p = __builtin_fma(r, p, 1.0 / 6.0);
+    p = __builtin_fma(r, p, 0.5);
+    p = __builtin_fma(z, p, r);
+    int j = k & 63;
+    int e = (k - j) / 64;
+    double th = kExp2Table[2 * j];
+    double tl = kExp2Table[2 * j + 1];
+    double y = th + __builtin_fma(th, p, tl);
+    if (e >= -1021 && e <= 1022) return y * pow2i(e);
+    int e1 = e / 2;
+    int e2 = e - e1;
+    return (y * pow2i(e1)) * pow2i(e2);
+}
+
+__device__ __forceinline__ double dlog(double x)
+{
+    if (x != x) return x;
+    if (x < 0.0) return u2d(0x7ff8000000000000ull);
+    if (x == 0.0) return u2d(0xfff0000000000000ull);
+    if (x == u2d(0x7ff0000000000000ull)) return x;
+    int e = 0;
+    uint64_t u = d2u(x);
+    if ((u >> 52) == 0) {
+        x = x * 18014398509481984.0;
+        u = d2u(x);
+        e = -54;
+    }
+    e += (int)(u >> 52) - 1023;
+    double m = u2d((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = 2.0 / 23.0;
+    R = __builtin_fma(z, R, 2.0 / 21.0);
+    R = __builtin_fma(z, R, 2.0 / 19.0);
+    R = __builtin_fma(z, R, 2.0 / 17.0);
+    R = __builtin_fma(z, R, 2.0 / 15.0);
+    R = __builtin_fma(z, R, 2.0 / 13.0);
+    R = __builtin_fma(z, R, 2.0 / 11.0);
+    R = __builtin_fma(z, R, 2.0 / 9.0);
+    R = __builtin_fma(z, R, 2.0 / 7.0);
+    R = __builtin_fma(z, R, 2.0 / 5.0);
+    R = __builtin_fma(z, R, 2.0 / 3.0);
+    R = z * R;
+    double hfsq = 0.5 * f * f;
+    double ed = (double)e;
+    double lo = __builtin_fma(s, hfsq + R, ed * HVC_LN2_LO);
+    return __builtin_fma(ed, HVC_LN2_HI, f - (hfsq - lo));
+}
+
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                         uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+__device__ __forceinline__ int point_index(uint32_t w) { return (int)(((uint64_t)w * 30u) >> 32); }
+
+__device__ __forceinline__ uint32_t poisson_u32(uint32_t seed_lo, uint32_t seed_hi, uint32_t trial,
+                                                uint32_t stream, uint32_t neuron, uint32_t draw)
+{
+    U4 o = philox4x32(draw >> 2, neuron, trial, stream, seed_lo, seed_hi);
+    uint32_t q = draw & 3u;
+    return q == 0 ? o.x : (q == 1 ? o.y : (q == 2 ? o.z : o.w));
+}
+
+}  // namespace hvc

@@ -1,0 +1,216 @@
+"""Python handle on the HIP engine (libhvc_engine.so, C ABI in include/hvc_engine.h).
+
+``Engine`` owns one ``hvc_ctx``: one network resident in device memory, one HIP
+stream.  Every call goes through the C ABI; there is no CPU fallback.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from .network import NETWORK_UPDATE_FREQUENCY, TIMESTEP, TRIAL_DURATION, Network, NoiseParams, SynapticParams
+
+RA_MODEL_DEBRABAND, RA_MODEL_RK4 = 0, 1
+I_MODEL_DP8, I_MODEL_RK4 = 0, 1
+RA_STATE_NAMES = ("Vs", "Vd", "n", "l", "h", "r", "c", "Ca", "Ge_s", "Gi_s", "Ge_d", "Gi_d")
+I_STATE_NAMES = ("V", "n", "m", "h", "w", "Ge", "Gi")
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (_abi.STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+@dataclass
+class TrialResult:
+    soma: list          # per HVC-RA neuron: array of somatic spike times (ms)
+    dend: list          # per HVC-RA neuron: dendritic spike times
+    interneuron: list   # per HVC-I neuron: spike times
+    bad_values: int
+    warnings: tuple
+    ra_state: np.ndarray = None
+    i_state: np.ndarray = None
+
+
+@dataclass
+class SynapseState:
+    w: np.ndarray
+    act_cnt: np.ndarray
+    act_ids: np.ndarray
+    sup_cnt: np.ndarray
+    sup_ids: np.ndarray
+    remodeled: np.ndarray
+
+    def active_lists(self):
+        return np.split(self.act_ids, np.cumsum(self.act_cnt)[:-1])
+
+    def super_lists(self):
+        return np.split(self.sup_ids, np.cumsum(self.sup_cnt)[:-1])
+
+
+def make_params(*, ra_model=RA_MODEL_DEBRABAND, i_model=I_MODEL_DP8, precision=0, dt=TIMESTEP,
+                trial_ms=TRIAL_DURATION, sync_ms=NETWORK_UPDATE_FREQUENCY, noise=None, ra_poisson=0,
+                i_poisson=1, seed=(1, 0), syn=None, spike_capacity=0):
+    noise = noise or NoiseParams()
+    syn = syn or SynapticParams()
+    p = _abi.Params()
+    p.ra_model, p.i_model, p.precision = ra_model, i_model, precision
+    p.dt, p.trial_ms, p.sync_ms = dt, trial_ms, sync_ms
+    p.mu_s, p.sigma_s = noise.white_noise_mean_soma, noise.white_noise_std_soma
+    p.mu_d, p.sigma_d = noise.white_noise_mean_dend, noise.white_noise_std_dend
+    p.ra_poisson, p.i_poisson = int(ra_poisson), int(i_poisson)
+    p.seed_lo, p.seed_hi = int(seed[0]) & 0xffffffff, int(seed[1]) & 0xffffffff
+    for k in SynapticParams.__dataclass_fields__:
+        setattr(p.syn, k, getattr(syn, k))
+    p.spike_capacity = spike_capacity
+    return p
+
+
+class Engine:
+    """One network on one device."""
+
+    def __init__(self, params=None, device=0, **kw):
+        self.lib = _abi.load()
+        self.params = params if params is not None else make_params(**kw)
+        self.ctx = ctypes.c_void_p()
+        st = self.lib.hvc_create(ctypes.byref(self.params), device, ctypes.byref(self.ctx))
+        if st != _abi.HVC_OK:
+            msg = self.lib.hvc_last_error(self.ctx).decode() if self.ctx else ""
+            if self.ctx:
+                self.lib.hvc_destroy(self.ctx)
+            self.ctx = None
+            raise EngineError(st, msg)
+        self.net = None
+
+    def _check(self, st):
+        if st != _abi.HVC_OK:
+            raise EngineError(st, self.lib.hvc_last_error(self.ctx).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.hvc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------
+    def upload(self, net: Network):
+        keep = {k: (_i32(v) if v.dtype.kind in "iu" else _f64(v)) for k, v in net.__dict__.items()
+                if isinstance(v, np.ndarray)}
+        d = _abi.NetworkDesc()
+        d.n_ra, d.n_i = net.n_ra, net.n_i
+        ip, dp = _abi.c_int_p, _abi.c_double_p
+        for name in ("ra2i_ptr", "ra2i_tgt", "i2ra_ptr", "i2ra_tgt", "rr_ptr", "rr_tgt", "act_cnt",
+                     "act_ids", "sup_cnt", "sup_ids", "remodeled", "train_ids"):
+            setattr(d, name, _p(keep[name], ip))
+        for name in ("ra2i_w", "ra2i_d", "i2ra_w", "i2ra_d", "rr_d", "rr_w", "erest", "gca", "spread"):
+            setattr(d, name, _p(keep[name], dp))
+        d.n_train = len(net.train_ids)
+        self._check(self.lib.hvc_upload_network(self.ctx, ctypes.byref(d)))
+        self.net = net
+        return self
+
+    def run_trial(self, mode=0, training=True, trial_index=0, first_trial=False, mean_injection_time=100.0,
+                  want_state=False):
+        """mode 0: growth trial (STDP); mode 1: chain-test trial (no STDP)."""
+        if self.net is None:
+            raise EngineError(_abi.HVC_E_STATE, "upload a network first")
+        t = _abi.Trial(mode, int(bool(training)), int(trial_index) & 0xffffffff, int(bool(first_trial)),
+                       float(mean_injection_time))
+        n_ra, n_i = self.net.n_ra, self.net.n_i
+        counts = [np.zeros(n_ra, np.int32), np.zeros(n_ra, np.int32), np.zeros(max(n_i, 1), np.int32)]
+        ras = np.zeros((n_ra, 12)) if want_state else None
+        ins = np.zeros((max(n_i, 1), 7)) if want_state else None
+        o = _abi.TrialOut()
+        ip, dp = _abi.c_int_p, _abi.c_double_p
+        o.soma_counts, o.dend_counts, o.i_counts = (_p(c, ip) for c in counts)
+        o.ra_state, o.i_state = _p(ras, dp), _p(ins, dp)
+        st = self.lib.hvc_run_trial(self.ctx, ctypes.byref(t), ctypes.byref(o))
+        if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
+            self._check(st)
+        lists = []
+        for kind, total in enumerate((o.soma_total, o.dend_total, o.i_total)):
+            c = np.zeros(n_i if kind == 2 else n_ra, np.int32)
+            times = np.zeros(max(total, 1))
+            tot = ctypes.c_int()
+            self._check(self.lib.hvc_trial_spikes(self.ctx, kind, _p(c, ip) if len(c) else None,
+                                                  _p(times, dp), len(times), ctypes.byref(tot)))
+            lists.append(np.split(times[:tot.value], np.cumsum(c)[:-1]) if len(c) else [])
+        return TrialResult(lists[0], lists[1], lists[2], o.bad_values, tuple(o.warnings), ras,
+                           ins[:n_i] if ins is not None else None)
+
+    def download_synapses(self) -> SynapseState:
+        E, n = self.net.n_candidates, self.net.n_ra
+        w = np.zeros(E)
+        ac, sc, rem = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        ai, si = np.zeros(max(E, 1), np.int32), np.zeros(max(n * max(1, self.params.syn.Nss), 1), np.int32)
+        s = _abi.Synapses(_p(w, _abi.c_double_p), _p(ac, _abi.c_int_p), _p(ai, _abi.c_int_p),
+                          _p(sc, _abi.c_int_p), _p(si, _abi.c_int_p), _p(rem, _abi.c_int_p))
+        self._check(self.lib.hvc_download_synapses(self.ctx, ctypes.byref(s)))
+        return SynapseState(w, ac, ai[:int(ac.sum())].copy(), sc, si[:int(sc.sum())].copy(), rem)
+
+    def upload_synapses(self, syn: SynapseState):
+        w, ac, ai = _f64(syn.w), _i32(syn.act_cnt), _i32(syn.act_ids)
+        sc, si, rem = _i32(syn.sup_cnt), _i32(syn.sup_ids), _i32(syn.remodeled)
+        s = _abi.Synapses(_p(w, _abi.c_double_p), _p(ac, _abi.c_int_p), _p(ai, _abi.c_int_p),
+                          _p(sc, _abi.c_int_p), _p(si, _abi.c_int_p), _p(rem, _abi.c_int_p))
+        self._check(self.lib.hvc_upload_synapses(self.ctx, ctypes.byref(s)))
+
+    def patch_neurons(self, ids, erest, gca):
+        ids, erest, gca = _i32(ids), _f64(erest), _f64(gca)
+        self._check(self.lib.hvc_patch_neurons(self.ctx, _p(ids, _abi.c_int_p), len(ids),
+                                               _p(erest, _abi.c_double_p), _p(gca, _abi.c_double_p)))
+
+    # ------------------------------------------------------------------
+    def trace_ra(self, erest, gca, kick, exc_w, inh_g, trial=0):
+        n_steps, n = kick.shape
+        out = np.zeros((n_steps + 1, n, 12))
+        fired = np.zeros((n_steps, n), np.uint8)
+        a = [_f64(x) for x in (erest, gca, kick, exc_w, inh_g)]
+        dp = _abi.c_double_p
+        self._check(self.lib.hvc_trace_ra(self.ctx, n, n_steps, _p(a[0], dp), _p(a[1], dp), trial,
+                                          _p(a[2], dp), _p(a[3], dp), _p(a[4], dp), _p(out, dp),
+                                          _p(fired, _abi.c_u8_p)))
+        return out, fired
+
+    def trace_i(self, exc_g, inh_g, trial=0, first_trial=False):
+        n_steps, n = exc_g.shape
+        out = np.zeros((n_steps + 1, n, 7))
+        fired = np.zeros((n_steps, n), np.uint8)
+        a = [_f64(x) for x in (exc_g, inh_g)]
+        dp = _abi.c_double_p
+        self._check(self.lib.hvc_trace_i(self.ctx, n, n_steps, int(bool(first_trial)), trial, _p(a[0], dp),
+                                         _p(a[1], dp), _p(out, dp), _p(fired, _abi.c_u8_p)))
+        return out, fired
+
+    def set_timing(self, on=True):
+        self._check(self.lib.hvc_set_timing(self.ctx, int(bool(on))))
+
+    def kernel_timing(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self.lib.hvc_kernel_timing(self.ctx, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
