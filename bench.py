@@ -1,0 +1,141 @@
+#!/usr/bin/env python3
+"""Benchmark of the HVC growth-trial hot path on one MI355X.
+
+A step is one growth trial (500 ms, 24 999 neuron steps, STDP on) of the
+20 000 HVC-RA + 5 500 HVC-I network (configs.c3): 100 RA->RA candidates per
+RA by distance, delays U(0, 10) ms, RK4 form (HH2_buffer::Runge4_step for
+HVC-RA; HVC-I use HHI_buffer::DP8_step because HHI_buffer::Runge4_step
+diverges at the reference's dt = 0.02 ms).
+
+    python bench.py [--steps K] [--warmup W] [--precision fp64|fp32] [--config c3]
+
+Prints one JSON line: neuron-updates/s = (N_RA + N_I) * 24 999 * K / wall time,
+with inputs resident on the device, plus the roofline of the dominant kernel
+(HIP-event durations on the engine stream) and a CPU baseline (the oracle's
+restatement of the same trial, one host thread, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--ra-model", type=int, default=1, help="0 Debraband (drop-in), 1 RK4 (HH2_buffer)")
+    ap.add_argument("--i-model", type=int, default=0, help="0 DP8, 1 RK4")
+    ap.add_argument("--cpu-sample-ms", type=float, default=5.0,
+                    help="trial length of the CPU-baseline sample (same network)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(net, kw, args):
+    """The oracle (C restatement, -O2, one thread) on the same network, shortened trial."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleSim
+    sample_ms = args.cpu_sample_ms
+    sim = OracleSim(net, ra_model=args.ra_model, i_model=args.i_model, trial_ms=sample_ms,
+                    seed=kw.get("seed", (1, 0)))
+    steps = int(sample_ms / 0.02) - 1
+    t = time.perf_counter()
+    sim.run_trial(mode=0, training=True, trial_index=0, first_trial=True, mean_injection_time=100.0)
+    el = time.perf_counter() - t
+    sim.close()
+    return {"value": (net.n_ra + net.n_i) * steps / el, "unit": "neuron-updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle growth trial of %g ms (%d steps) on the same %d RA + %d I network, %.1f s wall"
+                      % (sample_ms, steps, net.n_ra, net.n_i, el)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build measures the single-GPU hot path "
+              "(multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
+        sys.exit(2)
+    from unsupervised_growth_amd import configs, opcount
+    from unsupervised_growth_amd.engine import Engine
+
+    t_build = time.perf_counter()
+    net, kw = configs.CONFIGS[args.config]()
+    t_build = time.perf_counter() - t_build
+    precision = 0 if args.precision == "fp64" else 1
+    eng = Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
+    eng.upload(net)
+    steps_per_trial = int(kw["trial_ms"] / 0.02) - 1
+
+    trial = 0
+    for _ in range(args.warmup):
+        eng.run_trial(mode=0, training=True, trial_index=trial, first_trial=(trial == 0),
+                      mean_injection_time=100.0)
+        trial += 1
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    spikes = 0
+    for _ in range(args.steps):
+        r = eng.run_trial(mode=0, training=True, trial_index=trial, first_trial=(trial == 0),
+                          mean_injection_time=100.0)
+        spikes += sum(map(len, r.soma)) + sum(map(len, r.interneuron))
+        trial += 1
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+    value = (net.n_ra + net.n_i) * steps_per_trial * args.steps / wall
+
+    # roofline of the dominant kernel (HIP events on the engine stream)
+    windows = 2000 if kw["trial_ms"] == 500.0 else int(kw["trial_ms"] / 0.25)
+    kern = {}
+    for name in ("integrate_ra", "integrate_i", "deliver", "sync", "decay"):
+        avg, n = eng.kernel_timing(name)
+        kern[name] = {"avg_ms": avg, "samples": n}
+    per_trial = {k: v["avg_ms"] * (1 if k == "decay" else windows) for k, v in kern.items()}
+    dom = max(per_trial, key=per_trial.get)
+    ra_flops = opcount.flops_per_step("ra", args.ra_model)
+    i_flops = opcount.flops_per_step("i", args.i_model)
+    steps_per_window = steps_per_trial / windows
+    if dom == "integrate_i":
+        flops_launch = i_flops * net.n_i * steps_per_window
+    else:
+        flops_launch = ra_flops * net.n_ra * steps_per_window
+    avg_s = kern[dom]["avg_ms"] / 1000.0
+    peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
+    achieved = flops_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None,
+                "note": "VALU-bound (no dense contraction); peak = MI355X %s vector/matrix rate; achieved = "
+                        "algorithmic flops per launch (opcount) / HIP-event launch duration"
+                        % ("FP64" if precision == 0 else "FP32"),
+                "flops_per_launch": flops_launch, "avg_launch_ms": kern[dom]["avg_ms"],
+                "kernels_ms_per_trial": per_trial}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(net, kw, args)
+    line = {
+        "metric": "RK4 neuron-updates/s at 20k RA + 5.5k I; p50/p99 served trial latency",
+        "value": value, "unit": "neuron-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "f32",
+        "data": "synthetic network (distance rules, seeded), counter-based noise streams",
+        "config": {"workload": "growth trial, %d HVC-RA + %d HVC-I, %d RA->RA candidates, %g ms trials, "
+                               "RA %s + I %s, STDP on" % (net.n_ra, net.n_i, net.n_candidates, kw["trial_ms"],
+                                                          "RK4-3/8" if args.ra_model else "Debraband",
+                                                          "DP8" if args.i_model == 0 else "RK4"),
+                   "config": args.config, "network_build_s": round(t_build, 1), "spikes_per_trial": spikes / args.steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

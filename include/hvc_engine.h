@@ -148,9 +148,12 @@ hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const double* erest, c
 hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32_t trial,
                        const double* exc_g, const double* inh_g, double* out_state, uint8_t* out_fired);
 
-/* Kernel timing of the last run_trial: average device duration (ms) per launch
- * of each kernel class, measured with HIP events on the engine's stream.
- * names: "integrate_ra", "integrate_i", "deliver", "sync", "decay". */
+/* Kernel timing: with timing enabled, HIP events on the engine's stream bracket
+ * every 16th window's launches (and the end-of-trial decay); hvc_kernel_timing
+ * returns the average device duration (ms) per launch of a kernel class over
+ * all trials since hvc_set_timing(ctx, 1) (which clears the averages).
+ * names: "integrate_ra", "integrate_i", "deliver", "sync", "decay".
+ * No counterpart in the reference (it has no profiler). */
 hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches);
 hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled);
 

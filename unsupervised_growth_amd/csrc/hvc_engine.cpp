@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -75,9 +76,12 @@ struct hvc_ctx {
     bool trial_done = false;
     int log_counts[3] = {0, 0, 0};
 
-    // timing
+    // timing: HIP events recorded on the engine stream around sampled launches,
+    // read after the trial's final synchronisation
     bool timing = false;
-    hipEvent_t ev[2 * kNumTimers] = {};
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (class, index of start event)
     double t_sum[kNumTimers] = {};
     int t_cnt[kNumTimers] = {};
 };
@@ -150,7 +154,6 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     *out = ctx;
     HVC_TRY(hipSetDevice(device));
     HVC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    for (int k = 0; k < 2 * kNumTimers; ++k) HVC_TRY(hipEventCreate(&ctx->ev[k]));
 
     // trial clock and synchronisation points (NGS:8906-8925, 9291, 9563)
     const double dt = prm->dt;
@@ -223,8 +226,7 @@ extern "C" void hvc_destroy(hvc_ctx* ctx)
     if (ctx->stream) {
         hipStreamSynchronize(ctx->stream);
         free_bufs(ctx);
-        for (int k = 0; k < 2 * kNumTimers; ++k)
-            if (ctx->ev[k]) hipEventDestroy(ctx->ev[k]);
+        for (auto e : ctx->ev_pool) hipEventDestroy(e);
         hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -340,7 +342,8 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
 
     // per row: candidates sorted by (delay, target).  Two different delays whose
     // sums with a spike time could round to the same delivery time would make
-    // that order differ from the reference's (time, target) order; reject them.
+    // that order differ from the reference's (time, target) order; reject them
+    // (|d1 - d0| >= 4 ulp of the largest clock value keeps the sums distinct).
     std::vector<int> perm(ctx->E);
     for (int i = 0; i < nra; ++i) {
         int b = net->rr_ptr[i], e = net->rr_ptr[i + 1];
@@ -351,8 +354,9 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
         });
         for (int k = b + 1; k < e; ++k) {
             double d0 = net->rr_d[perm[k - 1]], d1 = net->rr_d[perm[k]];
-            if (d0 != d1 && d1 - d0 < 1e-9 * std::max(1.0, ctx->prm.trial_ms))
-                return fail(ctx, HVC_E_ARG, "RA->RA delays of one row differ by less than 1e-9 * trial length");
+            double tmax = ctx->prm.trial_ms + d1;
+            if (d0 != d1 && d1 - d0 < 4 * DBL_EPSILON * tmax)
+                return fail(ctx, HVC_E_ARG, "two RA->RA delays of one row differ by less than 4 ulp of the trial clock");
         }
     }
     int dense = 1;
@@ -544,19 +548,41 @@ extern "C" hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, con
 // ---------------------------------------------------------------------------
 // trials
 // ---------------------------------------------------------------------------
+static hipEvent_t pool_event(hvc_ctx* ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
 static void tstart(hvc_ctx* ctx, int k, bool on)
 {
-    if (on) hipEventRecord(ctx->ev[2 * k], ctx->stream);
+    if (!on) return;
+    size_t idx = ctx->ev_used;
+    hipEvent_t a = pool_event(ctx);
+    hipEvent_t b = pool_event(ctx);
+    if (!a || !b) return;
+    hipEventRecord(a, ctx->stream);
+    ctx->ev_pending.emplace_back(k, idx);
 }
 static void tstop(hvc_ctx* ctx, int k, bool on)
 {
-    if (!on) return;
-    hipEventRecord(ctx->ev[2 * k + 1], ctx->stream);
-    hipEventSynchronize(ctx->ev[2 * k + 1]);
-    float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]);
-    ctx->t_sum[k] += ms;
-    ctx->t_cnt[k] += 1;
+    if (!on || ctx->ev_pending.empty() || ctx->ev_pending.back().first != k) return;
+    hipEventRecord(ctx->ev_pool[ctx->ev_pending.back().second + 1], ctx->stream);
+}
+static void tcollect(hvc_ctx* ctx)
+{
+    for (auto& p : ctx->ev_pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ctx->ev_pool[p.second], ctx->ev_pool[p.second + 1]) == hipSuccess) {
+            ctx->t_sum[p.first] += ms;
+            ctx->t_cnt[p.first] += 1;
+        }
+    }
+    ctx->ev_pending.clear();
+    ctx->ev_used = 0;
 }
 
 static hvc_status collect_log(hvc_ctx* ctx, const int2* d_log, int count, int n, int* counts, double* times, int cap,
@@ -580,6 +606,10 @@ extern "C" hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled)
 {
     if (!ctx) return HVC_E_ARG;
     ctx->timing = enabled != 0;
+    for (int k = 0; k < kNumTimers; ++k) {
+        ctx->t_sum[k] = 0;
+        ctx->t_cnt[k] = 0;
+    }
     return HVC_OK;
 }
 
@@ -630,8 +660,6 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
         }
     }
     HVC_TRY(hipMemcpyAsync(ctx->d_kick, ctx->h_kick.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
-    for (int k = 0; k < kNumTimers; ++k) { ctx->t_sum[k] = 0; ctx->t_cnt[k] = 0; }
-
     launch_reset(s, nd, tr->trial_index, tr->first_trial, ctx->E);
     if (!growth) launch_chain_edges(s, nd);
     const int nw = (int)ctx->windows.size();
@@ -666,6 +694,7 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
     HVC_TRY(hipMemcpyAsync(vs.data(), nd.ra[0], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(vd.data(), nd.ra[1], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
+    tcollect(ctx);
     ctx->trial_done = true;
     for (int k = 0; k < 3; ++k) ctx->log_counts[k] = logc[k];
     if (err) {
