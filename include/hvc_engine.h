@@ -152,10 +152,17 @@ hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32
  * every 16th window's launches (and the end-of-trial decay); hvc_kernel_timing
  * returns the average device duration (ms) per launch of a kernel class over
  * all trials since hvc_set_timing(ctx, 1) (which clears the averages).
- * names: "integrate_ra", "integrate_i", "deliver", "sync", "decay".
+ * names: "integrate" (HVC-RA and HVC-I neurons, one launch per window),
+ * "deliver", "sync", "decay".
  * No counterpart in the reference (it has no profiler). */
 hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches);
 hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled);
+
+/* Device evaluation of the engine's own math on n host values: fn 0 exp(a),
+ * 1 log(a), 2 a / b through the constant-divisor path.  No counterpart in the
+ * reference: it lets the CPU check the device's exp / log / division bit for
+ * bit (the reference calls libm exp / log, HH2_test.h:265-276). */
+hvc_status hvc_eval_math(hvc_ctx* ctx, int fn, int n, const double* a, const double* b, double* out);
 
 void hvc_destroy(hvc_ctx* ctx);
 const char* hvc_last_error(const hvc_ctx* ctx);

@@ -11,7 +11,9 @@
  *
  * exp: table method (Tang, ACM TOMS 15(2), 1989): x = k*ln2/64 + r,
  *      exp(x) = 2^(k/64) * (1 + p(r)), |r| <= ln2/128, 2^(j/64) as a
- *      double-double table, p the degree-6 Taylor polynomial.  Error ~0.505 ulp.
+ *      double-double table, p the degree-6 Taylor polynomial, scaling by ldexp
+ *      (IEEE scaleB: a single rounding, also into the subnormal range).
+ *      Error ~0.505 ulp.
  * log: log(1+f) = f - (hfsq - s*(hfsq + R(s^2))), s = f/(2+f), the reduction of
  *      the fdlibm log (Sun Microsystems, 1993); R is the Taylor series of
  *      2*atanh(s)/s - 2 in z = s^2 to degree 11.
@@ -28,14 +30,11 @@
 static inline uint64_t d2u(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 static inline double u2d(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
 
-/* 2^e as a double, e in [-1022, 1023] */
-static inline double pow2i(int e) { return u2d((uint64_t)(e + 1023) << 52); }
-
 double ora_exp(double x)
 {
     if (x != x) return x;                         /* NaN */
-    if (x > 709.782712893384) return u2d(0x7ff0000000000000ull);
-    if (x < -745.1332191019412) return 0.0;
+    if (x > 710.0) x = 710.0;                     /* beyond: overflow to inf / underflow to 0 */
+    if (x < -746.0) x = -746.0;
     double kd = rint(x * ORA_INV_LN2_64);
     int k = (int)kd;
     double r = fma(-kd, ORA_LN2_64_HI, x);
@@ -52,10 +51,9 @@ double ora_exp(double x)
     double th = ORA_EXP2_T[2 * j];
     double tl = ORA_EXP2_T[2 * j + 1];
     double y = th + fma(th, p, tl);               /* y in [1/e^(1/128), 2*e^(1/128)) */
-    if (e >= -1021 && e <= 1022) return y * pow2i(e);
-    int e1 = e / 2;
-    int e2 = e - e1;
-    return (y * pow2i(e1)) * pow2i(e2);
+    if (e >= -1021 && e <= 1023)                   /* normal or overflowing result: y * 2^e exact */
+        return y * u2d((uint64_t)(e + 1023) << 52);
+    return ldexp(y, e);                            /* one rounding, also for subnormal results */
 }
 
 double ora_log(double x)

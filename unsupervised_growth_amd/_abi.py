@@ -8,7 +8,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhvc_engine.so")
+LIB_PATH = os.environ.get("HVC_ENGINE_LIB", os.path.join(HERE, "libhvc_engine.so"))
 
 c_int_p = ctypes.POINTER(ctypes.c_int)
 c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -21,7 +21,7 @@ STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: 
 # every entry point declared in include/hvc_engine.h
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
            "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
-           "hvc_kernel_timing", "hvc_set_timing", "hvc_destroy", "hvc_last_error"]
+           "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_destroy", "hvc_last_error"]
 
 
 class EngineNotBuilt(RuntimeError):
@@ -105,6 +105,7 @@ def load(path=LIB_PATH):
     lib.hvc_trial_spikes.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
     lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
     lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
+    lib.hvc_eval_math.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, c_double_p]
     lib.hvc_destroy.argtypes = [vp]
     lib.hvc_destroy.restype = None
     lib.hvc_last_error.argtypes = [vp]

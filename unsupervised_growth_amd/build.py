@@ -25,25 +25,26 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
+def build(verbose=False, lib=LIB, defines=()):
     objs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, "_obj_" + src.replace(".", "_") + ".o")
         cmd = [hipcc(), "-x", "hip", "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
                "-ffp-contract=off", "-fno-fast-math",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-Wno-unused-result", "-Wno-unused-value", "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+               "-Wno-unused-result", "-Wno-unused-value", "-I", CSRC] + ["-D" + d for d in defines] + [
+               "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB] + objs
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", lib] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     for o in objs:
         os.remove(o)
-    return LIB
+    return lib
 
 
 if __name__ == "__main__":

@@ -27,20 +27,20 @@ void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint3
 void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
 void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_trial, int E);
-void launch_integrate_ra(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
-void launch_integrate_i(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial);
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_sync(hipStream_t st, const NetDev& nd);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
+void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out);
 }  // namespace hvc
 
 using namespace hvc;
 
 namespace {
 
-enum TimerClass { kTInt = 0, kTIntI, kTDeliver, kTSync, kTDecay, kNumTimers };
-const char* kTimerNames[kNumTimers] = {"integrate_ra", "integrate_i", "deliver", "sync", "decay"};
+enum TimerClass { kTInt = 0, kTDeliver, kTSync, kTDecay, kNumTimers };
+const char* kTimerNames[kNumTimers] = {"integrate", "deliver", "sync", "decay"};
 
 struct DevBuf {
     void* p = nullptr;
@@ -667,11 +667,8 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
-        launch_integrate_ra(s, nd, t0, t1, tr->trial_index, growth);
+        launch_integrate(s, nd, t0, t1, tr->trial_index, growth);
         tstop(ctx, kTInt, tm);
-        tstart(ctx, kTIntI, tm);
-        launch_integrate_i(s, nd, t0, t1, tr->trial_index);
-        tstop(ctx, kTIntI, tm);
         tstart(ctx, kTDeliver, tm);
         launch_deliver(s, nd, t0, t1, growth);
         tstop(ctx, kTDeliver, tm);
@@ -827,5 +824,27 @@ extern "C" hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_tr
     HVC_TRY(hipStreamSynchronize(s));
     hipFree(d_exc); hipFree(d_inh); hipFree(d_out); hipFree(d_fired);
     hipFree(d_clock); hipFree(d_decay);
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_eval_math(hvc_ctx* ctx, int fn, int n, const double* a, const double* b, double* out)
+{
+    if (!ctx || n < 0 || fn < 0 || fn > 2 || (n > 0 && (!a || !out || (fn == 2 && !b)))) return HVC_E_ARG;
+    if (n == 0) return HVC_OK;
+    HVC_TRY(hipSetDevice(ctx->device));
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    HVC_TRY(hipMalloc(&da, sizeof(double) * n));
+    HVC_TRY(hipMalloc(&db, sizeof(double) * n));
+    HVC_TRY(hipMalloc(&dout, sizeof(double) * n));
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (fn == 2) HVC_TRY(hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    launch_eval_math(s, fn, n, da, fn == 2 ? db : da, dout);
+    HVC_TRY(hipGetLastError());
+    HVC_TRY(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    hipFree(da);
+    hipFree(db);
+    hipFree(dout);
     return HVC_OK;
 }

@@ -43,6 +43,56 @@ __global__ void k_decay_table(const double* clock, const double* offsets, int nu
     rows[t] = r;
 }
 
+// One neuron step with the branch-free constant divisions; if any lane of the
+// wave met a quotient outside the proven range, the wave repeats the step from
+// the saved state with IEEE division (EXACT), which gives the same bits.
+__device__ __forceinline__ void ra_step_debraband(RAState& s, double el, double gca, const RAUniform& u, double J1s,
+                                                  double J2s, double J1d, double J2d, double E_dt, double E_half)
+{
+    RAState s0 = s;
+    DivGuard bad;
+    ra_debraband<false>(bad, s, el, gca, u, J1s, J2s, J1d, J2d, E_dt, E_half);
+    if (__any(bad.failed())) {
+        s = s0;
+        ra_debraband<true>(bad, s, el, gca, u, J1s, J2s, J1d, J2d, E_dt, E_half);
+    }
+}
+
+__device__ __forceinline__ void ra_step_runge4(RAState& s, double el, double gca, const RAUniform& u, double E_third,
+                                               double E_twothird, double E_dt)
+{
+    RAState s0 = s;
+    DivGuard bad;
+    ra_runge4<false>(bad, s, el, gca, u, E_third, E_twothird, E_dt);
+    if (__any(bad.failed())) {
+        s = s0;
+        ra_runge4<true>(bad, s, el, gca, u, E_third, E_twothird, E_dt);
+    }
+}
+
+template <int MODEL>
+__device__ __forceinline__ void i_step(IState& s, double dt, const DecayRow& row)
+{
+    IState s0 = s;
+    DivGuard bad;
+    if (MODEL == 0) i_dp8<false>(bad, s, dt, row); else i_runge4<false>(bad, s, dt, row);
+    if (__any(bad.failed())) {
+        s = s0;
+        if (MODEL == 0) i_dp8<true>(bad, s, dt, row); else i_runge4<true>(bad, s, dt, row);
+    }
+}
+
+__device__ __forceinline__ void i_step_lanes(IState& s, int role, int base, double dt, const DecayRow& row)
+{
+    IState s0 = s;
+    DivGuard bad;
+    i_dp8_lanes<false>(bad, s, role, base, dt, row);
+    if (__any(bad.failed())) {
+        s = s0;
+        i_dp8_lanes<true>(bad, s, role, base, dt, row);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // single-neuron traces
 // ---------------------------------------------------------------------------
@@ -79,7 +129,7 @@ __global__ void k_trace_ra(NetDev nd, int n, int n_steps, uint32_t trial, const 
         const DecayRow& row = rows[t];
         if (nd.ra_model == 0) {
             U4 o = philox4x32((uint32_t)t, (uint32_t)k, trial, kStreamRAWhite, nd.seed_lo, nd.seed_hi);
-            ra_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
+            ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
                          kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
                          row.e5[kOffDt], row.e5[kOffHalf]);
         } else {
@@ -90,7 +140,7 @@ __global__ void k_trace_ra(NetDev nd, int n, int n_steps, uint32_t trial, const 
                 ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
                 ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
             }
-            ra_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
+            ra_step_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
         }
         fired_out[idx] = (uint8_t)f;
         double ig = inh_g[idx];
@@ -130,9 +180,9 @@ __global__ void k_trace_i(NetDev nd, int n, int n_steps, int first_trial, uint32
             i_noise_check(ps, time, nd.dt, s.Gi, ni);
         }
         if (nd.i_model == 0)
-            i_dp8(s, nd.dt, rows[t]);
+            i_step<0>(s, nd.dt, rows[t]);
         else
-            i_runge4(s, nd.dt, rows[t]);
+            i_step<1>(s, nd.dt, rows[t]);
         fired_out[idx] = (uint8_t)f;
         double ig = inh_g[idx];
         if (ig != 0.0) s.Gi += ig;
@@ -218,12 +268,14 @@ __device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool p
 }
 
 // ---------------------------------------------------------------------------
-// integrate_ra: HVC-RA neurons through steps t0..t1 (NGS:8931-9216 minus delivery)
+// integrate: HVC-RA neurons (NGS:8931-9216 minus delivery) and HVC-I neurons
+// (NGS:9221-9286 minus delivery) through the steps t0..t1 of one window, in one
+// launch.  HVC-I blocks come first: with five lanes per neuron they carry the
+// longer per-step work, and the RA blocks run beside them.
 // ---------------------------------------------------------------------------
 template <int MODEL>
-__global__ __launch_bounds__(256) void k_integrate_ra(NetDev nd, int t0, int t1, uint32_t trial, int growth)
+__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial, int growth)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < nd.n_ra;
     const int ii = live ? i : 0;
     const DecayRow* rows = (const DecayRow*)nd.decay;
@@ -270,7 +322,7 @@ __global__ __launch_bounds__(256) void k_integrate_ra(NetDev nd, int t0, int t1,
         const DecayRow& row = rows[t];
         if (MODEL == 0) {
             U4 o = philox4x32((uint32_t)t, (uint32_t)ii, trial, kStreamRAWhite, nd.seed_lo, nd.seed_hi);
-            ra_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
+            ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
                          kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
                          row.e5[kOffDt], row.e5[kOffHalf]);
         } else {
@@ -281,7 +333,7 @@ __global__ __launch_bounds__(256) void k_integrate_ra(NetDev nd, int t0, int t1,
                 ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
                 ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
             }
-            ra_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
+            ra_step_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
         }
         bool soma = live && (f & 1);
         bool dend = live && (f & 2);
@@ -316,9 +368,89 @@ __global__ __launch_bounds__(256) void k_integrate_ra(NetDev nd, int t0, int t1,
             nd.ra_draw[i] = ps.draw;
         }
     }
+}
 
-    if (!growth) return;
-    // the last block to finish sorts the window's post-event list by (step, id)
+// HVC-I, one lane per neuron (RK4 form)
+__device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial)
+{
+    const bool live = i < nd.n_i;
+    const int ii = live ? i : 0;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    IState s{nd.in[0][ii], nd.in[1][ii], nd.in[2][ii], nd.in[3][ii], nd.in[4][ii], nd.in[5][ii], nd.in[6][ii]};
+    int flag = nd.i_flag[ii];
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = nd.clock[t - 1];
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        i_step<1>(s, nd.dt, rows[t]);
+        bool fired = live && f;
+        if (fired) mask |= 1ull << (t - t0);
+        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+    }
+    if (live) {
+        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
+        nd.i_draw[i] = ps.draw;
+        nd.i_mask[i] = mask;
+    }
+}
+
+constexpr int kILanes = 5;                    // lanes per HVC-I neuron (DP8)
+constexpr int kINeuronsPerWave = 64 / kILanes; // 12 neurons, 4 idle lanes
+
+// HVC-I, five lanes per neuron (DP8 form); one wave holds 12 neurons
+__device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, uint32_t trial)
+{
+    // the four spare lanes 60..63 shadow lanes 55..58 of the last group: same
+    // neuron, same roles, results discarded -- their quotients then stay in the
+    // range of the real ones and never force the wave to repeat a step
+    const int lane = __lane_id();
+    const bool spare = lane >= kINeuronsPerWave * kILanes;
+    const int role = spare ? lane - kINeuronsPerWave * kILanes : lane % kILanes;
+    const int group = spare ? kINeuronsPerWave - 1 : lane / kILanes;
+    const int base = group * kILanes;
+    const int i = wave * kINeuronsPerWave + group;
+    const bool live = (i < nd.n_i) && !spare;
+    const int ii = (i < nd.n_i) ? i : 0;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    IState s{nd.in[0][ii], nd.in[1][ii], nd.in[2][ii], nd.in[3][ii], nd.in[4][ii], nd.in[5][ii], nd.in[6][ii]};
+    int flag = nd.i_flag[ii];
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = nd.clock[t - 1];
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        i_step_lanes(s, role, base, nd.dt, rows[t]);
+        bool fired = live && f && role == 0;
+        if (fired) mask |= 1ull << (t - t0);
+        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+    }
+    if (live && role == 0) {
+        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
+        nd.i_draw[i] = ps.draw;
+        nd.i_mask[i] = mask;
+    }
+}
+
+// the last block to finish sorts the window's post-event list by (step, id)
+__device__ void sort_post_list_if_last(const NetDev& nd)
+{
     __shared__ bool last;
     __shared__ uint64_t keys[kPostListCap];
     __threadfence();
@@ -350,44 +482,19 @@ __global__ __launch_bounds__(256) void k_integrate_ra(NetDev nd, int t0, int t1,
     if (threadIdx.x == 0) *nd.blocks_done = 0;
 }
 
-// ---------------------------------------------------------------------------
-// integrate_i: HVC-I neurons through steps t0..t1 (NGS:9221-9286 minus delivery)
-// ---------------------------------------------------------------------------
-template <int MODEL>
-__global__ __launch_bounds__(256) void k_integrate_i(NetDev nd, int t0, int t1, uint32_t trial)
+// one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
+template <int RA_MODEL, int I_MODEL>
+__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, uint32_t trial, int growth, int iblocks)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = i < nd.n_i;
-    const int ii = live ? i : 0;
-    const DecayRow* rows = (const DecayRow*)nd.decay;
-    IState s{nd.in[0][ii], nd.in[1][ii], nd.in[2][ii], nd.in[3][ii], nd.in[4][ii], nd.in[5][ii], nd.in[6][ii]};
-    int flag = nd.i_flag[ii];
-    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
-    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
-    uint64_t mask = 0;
-    for (int t = t0; t <= t1; ++t) {
-        int f = i_state_check(s.V, flag);
-        if (nd.i_poisson) {
-            double time = nd.clock[t - 1];
-            i_noise_check(ps, time, nd.dt, s.Ge, ne);
-            i_noise_check(ps, time, nd.dt, s.Gi, ni);
-        }
-        if (MODEL == 0)
-            i_dp8(s, nd.dt, rows[t]);
+    if ((int)blockIdx.x < iblocks) {
+        if (I_MODEL == 0)
+            integrate_i_lanes(nd, blockIdx.x, t0, t1, trial);
         else
-            i_runge4(s, nd.dt, rows[t]);
-        bool fired = live && f;
-        if (fired) mask |= 1ull << (t - t0);
-        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+            integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1, trial);
+    } else {
+        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, trial, growth);
     }
-    if (live) {
-        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
-        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
-        nd.i_flag[i] = flag;
-        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
-        nd.i_draw[i] = ps.draw;
-        nd.i_mask[i] = mask;
-    }
+    if (growth) sort_post_list_if_last(nd);
 }
 
 // ---------------------------------------------------------------------------
@@ -819,7 +926,7 @@ __global__ __launch_bounds__(256) void k_sync(NetDev nd)
         n = nd.in_ge_ra.cnt[i];
         if (n > 0) {
             if (n > nd.in_ge_ra.cap) { atomicOr(nd.err, (unsigned)kErrInbox); n = nd.in_ge_ra.cap; }
-            nd.ra[10][i] += inbox_sum(nd.in_ge_ra, i, n) / 10000.0;  // raiseExcWeight
+            nd.ra[10][i] += HVC_CDIV(inbox_sum(nd.in_ge_ra, i, n), 10000);  // raiseExcWeight: w / Ad
             nd.in_ge_ra.cnt[i] = 0;
         }
     }
@@ -876,6 +983,23 @@ __global__ void k_chain_edges(NetDev nd)
     }
 }
 
+// device evaluation of the engine's math for the CPU cross-check:
+// fn 0 exp(a), 1 log(a), 2 a / b through the constant-divisor path (Y = RN(1/b))
+__global__ void k_eval_math(int fn, int n, const double* a, const double* b, double* out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double x = a[k];
+    double r;
+    if (fn == 0)
+        r = dexp(x);
+    else if (fn == 1)
+        r = dlog(x);
+    else
+        r = cdiv_checked(x, b[k]);
+    out[k] = r;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -906,21 +1030,20 @@ void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_tr
     if (E > 0) k_clear_pre<<<blocks_for(E, 256), 256, 0, st>>>(nd.pre_cnt, E);
 }
 
-void launch_integrate_ra(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth)
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth)
 {
-    if (nd.ra_model == 0)
-        k_integrate_ra<0><<<blocks_for(nd.n_ra, 256), 256, 0, st>>>(nd, t0, t1, trial, growth);
+    int iblocks = nd.n_i == 0 ? 0 : (nd.i_model == 0 ? (nd.n_i + kINeuronsPerWave - 1) / kINeuronsPerWave
+                                                     : blocks_for(nd.n_i, 64));
+    int rblocks = blocks_for(nd.n_ra, 64);
+    dim3 grid(iblocks + rblocks);
+    if (nd.ra_model == 0 && nd.i_model == 0)
+        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+    else if (nd.ra_model == 0)
+        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+    else if (nd.i_model == 0)
+        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
     else
-        k_integrate_ra<1><<<blocks_for(nd.n_ra, 256), 256, 0, st>>>(nd, t0, t1, trial, growth);
-}
-
-void launch_integrate_i(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial)
-{
-    if (nd.n_i == 0) return;
-    if (nd.i_model == 0)
-        k_integrate_i<0><<<blocks_for(nd.n_i, 64), 64, 0, st>>>(nd, t0, t1, trial);
-    else
-        k_integrate_i<1><<<blocks_for(nd.n_i, 64), 64, 0, st>>>(nd, t0, t1, trial);
+        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
 }
 
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth)
@@ -939,6 +1062,11 @@ void launch_sync(hipStream_t st, const NetDev& nd)
 void launch_decay(hipStream_t st, const NetDev& nd)
 {
     k_decay<<<blocks_for(nd.n_ra, 64), 64, 0, st>>>(nd);
+}
+
+void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out)
+{
+    k_eval_math<<<blocks_for(n, 256), 256, 0, st>>>(fn, n, a, b, out);
 }
 
 void launch_chain_edges(hipStream_t st, const NetDev& nd)

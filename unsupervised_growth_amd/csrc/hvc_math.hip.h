@@ -6,7 +6,8 @@
 // bit on any IEEE machine:
 //   exp: x = k*ln2/64 + r, exp(x) = 2^(k/64) (1 + p(r)), |r| <= ln2/128, with a
 //        double-double table of 2^(j/64) and p the degree-6 Taylor polynomial
-//        (Tang's table method, ACM TOMS 15(2), 1989); error ~0.505 ulp.
+//        (Tang's table method, ACM TOMS 15(2), 1989), scaled with ldexp;
+//        error ~0.505 ulp.
 //   log: log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))), s = f/(2+f) (the fdlibm
 //        reduction), R the Taylor series of 2 atanh(s)/s - 2 to degree 11.
 // Noise: Philox4x32-10 (Salmon et al., SC'11).  One call per (RA neuron, step)
@@ -28,13 +29,12 @@ constexpr uint32_t kStreamRAPoisson = 0x52500000u;
 
 __device__ __forceinline__ double u2d(uint64_t u) { return __longlong_as_double((long long)u); }
 __device__ __forceinline__ uint64_t d2u(double x) { return (uint64_t)__double_as_longlong(x); }
-__device__ __forceinline__ double pow2i(int e) { return u2d((uint64_t)(e + 1023) << 52); }
 
 __device__ __forceinline__ double dexp(double x)
 {
-    if (x != x) return x;
-    if (x > 709.782712893384) return u2d(0x7ff0000000000000ull);
-    if (x < -745.1332191019412) return 0.0;
+    // NaN passes through the comparisons, the reduction and the polynomial
+    x = x > 710.0 ? 710.0 : x;
+    x = x < -746.0 ? -746.0 : x;
     double kd = __builtin_rint(x * HVC_INV_LN2_64);
     int k = (int)kd;
     double r = __builtin_fma(-kd, HVC_LN2_64_HI, x);
@@ -46,15 +46,52 @@ __device__ __forceinline__ double dexp(double x)
     p = __builtin_fma(r, p, 0.5);
     p = __builtin_fma(z, p, r);
     int j = k & 63;
-    int e = (k - j) / 64;
+    int e = k >> 6;  // floor(k / 64)
     double th = kExp2Table[2 * j];
     double tl = kExp2Table[2 * j + 1];
     double y = th + __builtin_fma(th, p, tl);
-    if (e >= -1021 && e <= 1022) return y * pow2i(e);
-    int e1 = e / 2;
-    int e2 = e - e1;
-    return (y * pow2i(e1)) * pow2i(e2);
+    return __builtin_ldexp(y, e);  // v_ldexp_f64: one rounding, also for subnormal results
 }
+
+// a / B for a divisor B > 0 known at compile time, with Y = RN(1/B): q = RN(a Y),
+// r = a - B q (exact, fma), result RN(q + r Y) -- Markstein's correction
+// (Markstein 1990, IBM J. Res. Dev. 34(1)).  For 2^-900 <= |q| < 2^1000:
+// |q - a/B| < 1.5 ulp, r is exact, and the fma result differs from a/B by at
+// most 3 * 2^-54 ulp while a/B lies at least ulp / (4 B_odd) from every
+// rounding boundary (B_odd = odd part of B's significand); for B_odd < 2^51 --
+// every divisor used here -- the result is the correctly rounded quotient, bit
+// for bit a / B.  For a = +-0 the result is a zero with a's sign, as a / B is.
+// Any other quotient (tiny, huge, infinite or NaN) is recorded in the guard and
+// the caller repeats the step with IEEE division.  No branch, so the compiler
+// can interleave the many independent quotients of a stage.
+struct DivGuard {
+    int emin = 0;      // least binary exponent of a quotient (frexp convention, 0 for zero)
+    double big = 0.0;  // sum of |q| 2^-1000: >= 1 or NaN once a quotient is huge or not finite
+    __device__ __forceinline__ void note(double q)
+    {
+        emin = min(emin, __builtin_amdgcn_frexp_exp(q));
+        big = __builtin_fma(__builtin_fabs(q), 0x1p-1000, big);
+    }
+    __device__ __forceinline__ bool failed() const { return emin < -899 || !(big < 1.0); }
+};
+
+__device__ __forceinline__ double cdiv_fast(double a, double B, double Y, DivGuard& g)
+{
+    double q = a * Y;
+    double r = __builtin_fma(-q, B, a);
+    double q1 = __builtin_copysign(__builtin_fma(r, Y, q), a);
+    g.note(q);
+    return q1;
+}
+
+// the same quotient with the IEEE fallback applied per value
+__device__ __forceinline__ double cdiv_checked(double a, double B)
+{
+    DivGuard g;
+    double q = cdiv_fast(a, B, 1.0 / B, g);
+    return g.failed() ? a / B : q;
+}
+#define HVC_CDIV(a, B) ::hvc::cdiv_checked((a), (double)(B))
 
 __device__ __forceinline__ double dlog(double x)
 {

@@ -207,6 +207,15 @@ class Engine:
                                          _p(a[1], dp), _p(out, dp), _p(fired, _abi.c_u8_p)))
         return out, fired
 
+    def eval_math(self, fn, a, b=None):
+        """Device exp (fn 0), log (fn 1) or constant-divisor division a / b (fn 2)."""
+        a = _f64(a)
+        b = _f64(b) if b is not None else None
+        out = np.zeros_like(a)
+        dp = _abi.c_double_p
+        self._check(self.lib.hvc_eval_math(self.ctx, fn, len(a), _p(a, dp), _p(b, dp), _p(out, dp)))
+        return out
+
     def set_timing(self, on=True):
         self._check(self.lib.hvc_set_timing(self.ctx, int(bool(on))))
 
