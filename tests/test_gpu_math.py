@@ -21,8 +21,9 @@ def _divisors():
     src = ""
     for f in ("hvc_neuron.hip.h", "hvc_kernels.hip"):
         src += open(os.path.join(ROOT, "unsupervised_growth_amd", "csrc", f)).read()
-    vals = set(float(x) for x in re.findall(r"\bCD\([^;]*?,\s*([0-9.]+)\)", src))
+    vals = set(float(x) for x in re.findall(r"\bCD\([^;]*?,\s*(?:HR\()?([0-9.]+)\)", src))
     vals |= set(float(x) for x in re.findall(r"HVC_CDIV\([^;]*?,\s*([0-9.]+)\)", src))
+    vals |= set(float(x) for x in re.findall(r"\bCK\(-?\d+,\s*[\w.]+,\s*(\d+)\)", src))
     return sorted(vals)
 
 

@@ -28,6 +28,12 @@ void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int fi
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
 void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_trial, int E);
 void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
+void launch_trace_ra_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
+                         const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
+                         uint8_t* fired);
+void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
+                        const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
+void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_sync(hipStream_t st, const NetDev& nd);
 void launch_decay(hipStream_t st, const NetDev& nd);
@@ -137,7 +143,7 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     };
     if (!(prm->dt > 0) || !(prm->trial_ms > 0) || !(prm->sync_ms > 0)) { hvc_status s = bad("dt, trial_ms, sync_ms must be > 0"); *out = ctx; return s; }
     if (prm->ra_model < 0 || prm->ra_model > 1 || prm->i_model < 0 || prm->i_model > 1) { hvc_status s = bad("model must be 0 or 1"); *out = ctx; return s; }
-    if (prm->precision != 0) { hvc_status s = bad("only precision 0 (FP64) is available in this build"); *out = ctx; return s; }
+    if (prm->precision != 0 && prm->precision != 1) { hvc_status s = bad("precision must be 0 (FP64) or 1 (FP32)"); *out = ctx; return s; }
     if (prm->ra_poisson && prm->ra_model != 1) { hvc_status s = bad("RA Poisson noise is part of the RK4 (HH2_buffer) form only"); *out = ctx; return s; }
     if (!(prm->syn.ACTIVATION_THRESHOLD > 0) || !(prm->syn.SUPERSYNAPSE_THRESHOLD > 0) || prm->syn.Nss < 0) {
         hvc_status s = bad("synaptic thresholds must be > 0 and Nss >= 0");
@@ -667,7 +673,10 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
-        launch_integrate(s, nd, t0, t1, tr->trial_index, growth);
+        if (ctx->prm.precision == 1)
+            launch_integrate_f32(s, nd, t0, t1, tr->trial_index, growth);
+        else
+            launch_integrate(s, nd, t0, t1, tr->trial_index, growth);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
         launch_deliver(s, nd, t0, t1, growth);
@@ -785,7 +794,10 @@ extern "C" hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const doubl
     NetDev nd = ctx->nd;
     nd.clock = d_clock;
     nd.decay = d_decay;
-    launch_trace_ra(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
+    if (ctx->prm.precision == 1)
+        launch_trace_ra_f32(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
+    else
+        launch_trace_ra(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
     HVC_TRY(hipGetLastError());
     HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 12, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
@@ -817,7 +829,10 @@ extern "C" hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_tr
     NetDev nd = ctx->nd;
     nd.clock = d_clock;
     nd.decay = d_decay;
-    launch_trace_i(s, nd, n, n_steps, first_trial, trial, d_exc, d_inh, d_out, d_fired);
+    if (ctx->prm.precision == 1)
+        launch_trace_i_f32(s, nd, n, n_steps, first_trial, trial, d_exc, d_inh, d_out, d_fired);
+    else
+        launch_trace_i(s, nd, n, n_steps, first_trial, trial, d_exc, d_inh, d_out, d_fired);
     HVC_TRY(hipGetLastError());
     HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 7, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));

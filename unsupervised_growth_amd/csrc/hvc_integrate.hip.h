@@ -1,0 +1,465 @@
+// hvc_integrate.hip.h -- neuron integration kernels of one precision form.
+//
+// Included once by hvc_kernels.hip (FP64 form, bit-exact) and once by
+// hvc_kernels_f32.hip (FP32 form, HVC_FP32 defined); the kernels land in
+// hvc::p64 or hvc::p32, the launchers are named launch_* and launch_*_f32.
+// Neuron state is stored in HBM as FP64 structure-of-arrays in both forms; the
+// FP32 form rounds it to float when a window starts and widens it when the
+// window ends.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hvc_internal.h"
+#include "hvc_neuron.hip.h"
+
+namespace hvc {
+
+static inline int blocks_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 1; }
+
+HVC_PREC_BEGIN
+
+// One neuron step.  FP64: with the branch-free constant divisions; if any lane
+// of the wave met a quotient outside the proven range, the wave repeats the
+// step from the saved state with IEEE division (EXACT), which gives the same
+// bits.  FP32: the step as is.
+#define HVC_GUARDED_STEP(State, call_fast, call_exact) \
+    Guard bad;                                         \
+    if constexpr (Guard::kActive) {                    \
+        State s0 = s;                                  \
+        call_fast;                                     \
+        if (__any(bad.failed())) {                     \
+            s = s0;                                    \
+            call_exact;                                \
+        }                                              \
+    } else {                                           \
+        call_fast;                                     \
+    }
+
+__device__ __forceinline__ void ra_step_debraband(RAState& s, real el, real gca, const RAUniform& u, real J1s,
+                                                  real J2s, real J1d, real J2d, real E_dt, real E_half)
+{
+    HVC_GUARDED_STEP(RAState, (ra_debraband<false>(bad, s, el, gca, u, J1s, J2s, J1d, J2d, E_dt, E_half)),
+                     (ra_debraband<true>(bad, s, el, gca, u, J1s, J2s, J1d, J2d, E_dt, E_half)))
+}
+
+__device__ __forceinline__ void ra_step_runge4(RAState& s, real el, real gca, const RAUniform& u, real E_third,
+                                               real E_twothird, real E_dt)
+{
+    HVC_GUARDED_STEP(RAState, (ra_runge4<false>(bad, s, el, gca, u, E_third, E_twothird, E_dt)),
+                     (ra_runge4<true>(bad, s, el, gca, u, E_third, E_twothird, E_dt)))
+}
+
+template <int MODEL>
+__device__ __forceinline__ void i_step(IState& s, real dt, const DecayRow& row)
+{
+    if (MODEL == 0) {
+        HVC_GUARDED_STEP(IState, (i_dp8<false>(bad, s, dt, row)), (i_dp8<true>(bad, s, dt, row)))
+    } else {
+        HVC_GUARDED_STEP(IState, (i_runge4<false>(bad, s, dt, row)), (i_runge4<true>(bad, s, dt, row)))
+    }
+}
+
+__device__ __forceinline__ void i_step_lanes(IState& s, int role, int base, real dt, const DecayRow& row)
+{
+    HVC_GUARDED_STEP(IState, (i_dp8_lanes<false>(bad, s, role, base, dt, row)),
+                     (i_dp8_lanes<true>(bad, s, role, base, dt, row)))
+}
+#undef HVC_GUARDED_STEP
+
+// per-network stepper constants in the form's type
+__device__ __forceinline__ RAUniform ra_uniform(const NetDev& nd)
+{
+    return RAUniform{(real)nd.sq_sig_s, (real)nd.sq_sig_d, (real)nd.mu_term_s, (real)nd.mu_term_d, (real)nd.dt};
+}
+
+// ---------------------------------------------------------------------------
+// single-neuron traces
+// ---------------------------------------------------------------------------
+__global__ void k_trace_ra(NetDev nd, int n, int n_steps, uint32_t trial, const double* EL, const double* GCa,
+                           const double* kick, const double* exc_w, const double* inh_g, double* out, uint8_t* fired_out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    const RAUniform u = ra_uniform(nd);
+    RAState s{HR(-79.97619025), HR(-79.97268759), HR(0.01101284), HR(0.0), HR(0.9932845), HR(0.00055429),
+              HR(0.00000261762353), HR(0.01689572), HR(0.0), HR(0.0), HR(0.0), HR(0.0)};
+    int fs = 0, fd = 0;
+    const real el = (real)EL[k], gca = (real)GCa[k];
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamRAPoisson, (uint32_t)k, 0u};
+    double nes = 0, nis = 0, ned = 0, nid = 0;
+    if (nd.ra_poisson) {
+        nes = 1000.0 * ps.spike_time(100.0);
+        nis = 1000.0 * ps.spike_time(100.0);
+        ned = 1000.0 * ps.spike_time(100.0);
+        nid = 1000.0 * ps.spike_time(100.0);
+    }
+    auto dump = [&](int row) {
+        double* o = out + ((size_t)row * n + k) * 12;
+        o[0] = s.Vs; o[1] = s.Vd; o[2] = s.n; o[3] = s.l; o[4] = s.h; o[5] = s.r;
+        o[6] = s.c; o[7] = s.Ca; o[8] = s.Ge_s; o[9] = s.Gi_s; o[10] = s.Ge_d; o[11] = s.Gi_d;
+    };
+    dump(0);
+    for (int t = 1; t <= n_steps; ++t) {
+        size_t idx = (size_t)(t - 1) * n + k;
+        double kk = kick[idx];
+        if (kk != 0.0) s.Ge_d += kk;
+        int f = ra_state_check(s.Vs, s.Vd, fs, fd);
+        const DecayRow& row = rows[t];
+        if (nd.ra_model == 0) {
+            U4 o = philox4x32((uint32_t)t, (uint32_t)k, trial, kStreamRAWhite, nd.seed_lo, nd.seed_hi);
+            ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
+                         kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
+                         row.e5[kOffDt], row.e5[kOffHalf]);
+        } else {
+            if (nd.ra_poisson) {
+                double time = nd.clock[t - 1];
+                ra_noise_check(ps, time, nd.dt, s.Gi_s, 0.035, nis);
+                ra_noise_check(ps, time, nd.dt, s.Gi_d, 0.045, nid);
+                ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
+                ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
+            }
+            ra_step_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
+        }
+        fired_out[idx] = (uint8_t)f;
+        double ig = inh_g[idx];
+        if (ig != 0.0) s.Gi_d += ig;
+        double ew = exc_w[idx];
+        if (ew != 0.0) s.Ge_d += ew / 10000.0;
+        dump(t);
+    }
+}
+
+__global__ void k_trace_i(NetDev nd, int n, int n_steps, int first_trial, uint32_t trial, const double* exc_g,
+                          const double* inh_g, double* out, uint8_t* fired_out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    IState s{HR(-66.0), HR(0.125), HR(0.0), HR(0.99), HR(0.0), HR(0.0), HR(0.0)};
+    int flag = 0;
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)k, 0u};
+    double ne = 0, ni = 0;
+    if (nd.i_poisson) {
+        double init_dt = first_trial ? 0.0 : nd.dt;
+        ne = i_init_noise(ps, init_dt);
+        ni = i_init_noise(ps, init_dt);
+    }
+    auto dump = [&](int row) {
+        double* o = out + ((size_t)row * n + k) * 7;
+        o[0] = s.V; o[1] = s.n; o[2] = s.m; o[3] = s.h; o[4] = s.w; o[5] = s.Ge; o[6] = s.Gi;
+    };
+    dump(0);
+    for (int t = 1; t <= n_steps; ++t) {
+        size_t idx = (size_t)(t - 1) * n + k;
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = nd.clock[t - 1];
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        if (nd.i_model == 0)
+            i_step<0>(s, nd.dt, rows[t]);
+        else
+            i_step<1>(s, (real)nd.dt, rows[t]);
+        fired_out[idx] = (uint8_t)f;
+        double ig = inh_g[idx];
+        if (ig != 0.0) s.Gi += ig;
+        double eg = exc_g[idx];
+        if (eg != 0.0) s.Ge += eg;
+        dump(t);
+    }
+}
+
+// append (neuron, step) to a log with one atomic per wave
+__device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool pred, int neuron, int step,
+                                            unsigned int* err)
+{
+    uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(cnt, __popcll(m));
+    base = __shfl(base, leader);
+    if (pred) {
+        int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (pos < cap)
+            log[pos] = make_int2(neuron, step);
+        else
+            atomicOr(err, (unsigned)kErrSpikeLog);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// integrate: HVC-RA neurons (NGS:8931-9216 minus delivery) and HVC-I neurons
+// (NGS:9221-9286 minus delivery) through the steps t0..t1 of one window, in one
+// launch.  HVC-I blocks come first: with five lanes per neuron they carry the
+// longer per-step work, and the RA blocks run beside them.
+// ---------------------------------------------------------------------------
+template <int MODEL>
+__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial, int growth)
+{
+    const bool live = i < nd.n_ra;
+    const int ii = live ? i : 0;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    const RAUniform u = ra_uniform(nd);
+    RAState s;
+    s.Vs = nd.ra[0][ii]; s.Vd = nd.ra[1][ii]; s.n = nd.ra[2][ii]; s.l = nd.ra[3][ii];
+    s.h = nd.ra[4][ii]; s.r = nd.ra[5][ii]; s.c = nd.ra[6][ii]; s.Ca = nd.ra[7][ii];
+    s.Ge_s = nd.ra[8][ii]; s.Gi_s = nd.ra[9][ii]; s.Ge_d = nd.ra[10][ii]; s.Gi_d = nd.ra[11][ii];
+    int flags = nd.ra_flags[ii];
+    int fs = flags & 1, fd = (flags >> 1) & 1;
+    const real el = (real)nd.EL[ii], gca = (real)nd.GCa[ii];
+    const int kick = nd.kick_step[ii];
+
+    // post events detected in the previous window become visible now (NGS:9385-9386)
+    int pcnt = nd.post_cnt[ii];
+    double last_post = 0.0;
+    if (growth && live) {
+        int pend = nd.post_pend[ii];
+        if (pend > 0) {
+            double cs = nd.clock[t0 - 1];
+            for (int q = 0; q < pend; ++q) {
+                nd.post_ring[(size_t)ii * kPostRing + (pcnt % kPostRing)] = cs;
+                ++pcnt;
+            }
+            if (pend > 2) atomicOr(nd.err, (unsigned)kErrPostRing);
+            nd.post_cnt[ii] = pcnt;
+            nd.post_pend[ii] = 0;
+        }
+        if (pcnt > 0) last_post = nd.post_ring[(size_t)ii * kPostRing + ((pcnt - 1) % kPostRing)];
+    }
+    int new_posts = 0;
+
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamRAPoisson, (uint32_t)ii, nd.ra_draw[ii]};
+    double nes = 0, nis = 0, ned = 0, nid = 0;
+    if (MODEL == 1 && nd.ra_poisson) {
+        nes = nd.ra_noise[ii]; nis = nd.ra_noise[(size_t)nd.n_ra + ii];
+        ned = nd.ra_noise[2 * (size_t)nd.n_ra + ii]; nid = nd.ra_noise[3 * (size_t)nd.n_ra + ii];
+    }
+
+    uint64_t smask = 0, dmask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        if (kick == t) s.Ge_d += 3.0;  // raiseE(G_TRAINING_KICK), NGS:8933-8940
+        int f = ra_state_check(s.Vs, s.Vd, fs, fd);
+        const DecayRow& row = rows[t];
+        if (MODEL == 0) {
+            U4 o = philox4x32((uint32_t)t, (uint32_t)ii, trial, kStreamRAWhite, nd.seed_lo, nd.seed_hi);
+            ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
+                         kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
+                         row.e5[kOffDt], row.e5[kOffHalf]);
+        } else {
+            if (nd.ra_poisson) {
+                double time = nd.clock[t - 1];
+                ra_noise_check(ps, time, nd.dt, s.Gi_s, 0.035, nis);
+                ra_noise_check(ps, time, nd.dt, s.Gi_d, 0.045, nid);
+                ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
+                ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
+            }
+            ra_step_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
+        }
+        bool soma = live && (f & 1);
+        bool dend = live && (f & 2);
+        if (soma) smask |= 1ull << (t - t0);
+        if (dend) dmask |= 1ull << (t - t0);
+        wave_append(nd.soma_log, &nd.log_cnt[0], nd.log_cap, soma, i, t, nd.err);
+        wave_append(nd.dend_log, &nd.log_cnt[1], nd.log_cap, dend, i, t, nd.err);
+        if (growth && soma) {
+            // new postsynaptic event (NGS:8956-8967)
+            double it = nd.clock[t];
+            if (pcnt == 0 || it > last_post + 30.0) {
+                int pos = atomicAdd(nd.post_list_cnt, 1);
+                if (pos < kPostListCap)
+                    nd.post_list[pos] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)i;
+                else
+                    atomicOr(nd.err, (unsigned)kErrPostList);
+                ++new_posts;
+            }
+        }
+    }
+    if (live) {
+        nd.ra[0][i] = s.Vs; nd.ra[1][i] = s.Vd; nd.ra[2][i] = s.n; nd.ra[3][i] = s.l;
+        nd.ra[4][i] = s.h; nd.ra[5][i] = s.r; nd.ra[6][i] = s.c; nd.ra[7][i] = s.Ca;
+        nd.ra[8][i] = s.Ge_s; nd.ra[9][i] = s.Gi_s; nd.ra[10][i] = s.Ge_d; nd.ra[11][i] = s.Gi_d;
+        nd.ra_flags[i] = fs | (fd << 1);
+        nd.soma_mask[i] = smask;
+        nd.dend_mask[i] = dmask;
+        if (growth) nd.post_pend[i] = new_posts;
+        if (MODEL == 1 && nd.ra_poisson) {
+            nd.ra_noise[i] = nes; nd.ra_noise[(size_t)nd.n_ra + i] = nis;
+            nd.ra_noise[2 * (size_t)nd.n_ra + i] = ned; nd.ra_noise[3 * (size_t)nd.n_ra + i] = nid;
+            nd.ra_draw[i] = ps.draw;
+        }
+    }
+}
+
+// HVC-I, one lane per neuron (RK4 form)
+__device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial)
+{
+    const bool live = i < nd.n_i;
+    const int ii = live ? i : 0;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
+             (real)nd.in[5][ii], (real)nd.in[6][ii]};
+    int flag = nd.i_flag[ii];
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = nd.clock[t - 1];
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        i_step<1>(s, (real)nd.dt, rows[t]);
+        bool fired = live && f;
+        if (fired) mask |= 1ull << (t - t0);
+        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+    }
+    if (live) {
+        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
+        nd.i_draw[i] = ps.draw;
+        nd.i_mask[i] = mask;
+    }
+}
+
+constexpr int kILanes = 5;                    // lanes per HVC-I neuron (DP8)
+constexpr int kINeuronsPerWave = 64 / kILanes; // 12 neurons, 4 idle lanes
+
+// HVC-I, five lanes per neuron (DP8 form); one wave holds 12 neurons
+__device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, uint32_t trial)
+{
+    // the four spare lanes 60..63 shadow lanes 55..58 of the last group: same
+    // neuron, same roles, results discarded -- their quotients then stay in the
+    // range of the real ones and never force the wave to repeat a step
+    const int lane = __lane_id();
+    const bool spare = lane >= kINeuronsPerWave * kILanes;
+    const int role = spare ? lane - kINeuronsPerWave * kILanes : lane % kILanes;
+    const int group = spare ? kINeuronsPerWave - 1 : lane / kILanes;
+    const int base = group * kILanes;
+    const int i = wave * kINeuronsPerWave + group;
+    const bool live = (i < nd.n_i) && !spare;
+    const int ii = (i < nd.n_i) ? i : 0;
+    const DecayRow* rows = (const DecayRow*)nd.decay;
+    IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
+             (real)nd.in[5][ii], (real)nd.in[6][ii]};
+    int flag = nd.i_flag[ii];
+    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = nd.clock[t - 1];
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        i_step_lanes(s, role, base, (real)nd.dt, rows[t]);
+        bool fired = live && f && role == 0;
+        if (fired) mask |= 1ull << (t - t0);
+        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+    }
+    if (live && role == 0) {
+        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
+        nd.i_draw[i] = ps.draw;
+        nd.i_mask[i] = mask;
+    }
+}
+
+// the last block to finish sorts the window's post-event list by (step, id)
+__device__ void sort_post_list_if_last(const NetDev& nd)
+{
+    __shared__ bool last;
+    __shared__ uint64_t keys[kPostListCap];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(nd.blocks_done, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    int n = atomicAdd(nd.post_list_cnt, 0);
+    if (n > kPostListCap) n = kPostListCap;
+    int np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (int k = threadIdx.x; k < np2; k += blockDim.x) keys[k] = (k < n) ? nd.post_list[k] : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= np2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < np2; k += blockDim.x) {
+                int p = k ^ stride;
+                if (p > k) {
+                    bool up = ((k & size) == 0);
+                    uint64_t a = keys[k], b = keys[p];
+                    if ((a > b) == up) { keys[k] = b; keys[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) nd.post_list[k] = keys[k];
+    if (threadIdx.x == 0) *nd.blocks_done = 0;
+}
+
+// one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
+template <int RA_MODEL, int I_MODEL>
+__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, uint32_t trial, int growth, int iblocks)
+{
+    if ((int)blockIdx.x < iblocks) {
+        if (I_MODEL == 0)
+            integrate_i_lanes(nd, blockIdx.x, t0, t1, trial);
+        else
+            integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1, trial);
+    } else {
+        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, trial, growth);
+    }
+    if (growth) sort_post_list_if_last(nd);
+}
+
+}  // namespace p32 / p64
+
+#ifdef HVC_FP32
+#define HVC_LAUNCHER(name) name##_f32
+#else
+#define HVC_LAUNCHER(name) name
+#endif
+
+void HVC_LAUNCHER(launch_trace_ra)(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
+                     const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
+                     uint8_t* fired)
+{
+    k_trace_ra<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, trial, EL, GCa, kick, exc_w, inh_g, out, fired);
+}
+
+void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
+                    const double* exc_g, const double* inh_g, double* out, uint8_t* fired)
+{
+    k_trace_i<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, first_trial, trial, exc_g, inh_g, out, fired);
+}
+
+void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth)
+{
+    int iblocks = nd.n_i == 0 ? 0 : (nd.i_model == 0 ? (nd.n_i + kINeuronsPerWave - 1) / kINeuronsPerWave
+                                                     : blocks_for(nd.n_i, 64));
+    int rblocks = blocks_for(nd.n_ra, 64);
+    dim3 grid(iblocks + rblocks);
+    if (nd.ra_model == 0 && nd.i_model == 0)
+        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+    else if (nd.ra_model == 0)
+        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+    else if (nd.i_model == 0)
+        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+    else
+        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+}
+
+#undef HVC_LAUNCHER
+
+}  // namespace hvc

@@ -21,7 +21,7 @@
 
 namespace hvc {
 
-__constant__ double kExp2Table[128] = HVC_EXP2_TABLE_INIT;
+static __constant__ double kExp2Table[128] = HVC_EXP2_TABLE_INIT;
 
 constexpr uint32_t kStreamRAWhite = 0x52410000u;
 constexpr uint32_t kStreamIPoisson = 0x49000000u;
