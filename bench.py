@@ -59,6 +59,23 @@ def cpu_baseline(net, kw, args):
                       % (sample_ms, steps, net.n_ra, net.n_i, el)}
 
 
+def fp32_check(net, kw, args, syn0, trial_index, r32):
+    """Spike-time error of the FP32 form: the first timed trial again in the FP64
+    form (bit-exact with the oracle) from the same synapse state."""
+    from unsupervised_growth_amd.engine import Engine
+    from unsupervised_growth_amd.spikes import compare_results
+    with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=0, **kw) as e64:
+        e64.upload(net)
+        e64.upload_synapses(syn0)
+        r64 = e64.run_trial(mode=0, training=True, trial_index=trial_index, first_trial=(trial_index == 0),
+                            mean_injection_time=100.0)
+    c = compare_results(r64, r32, window=1.0)
+    return {"reference": "FP64 form (bit-exact with the oracle), same trial and synapse state",
+            "max_abs_dt_ms": max(v["max_abs_dt_ms"] for v in c.values()),
+            "unpaired_spikes": sum(v["unpaired"] for v in c.values()),
+            "spikes": sum(v["spikes_ref"] for v in c.values()), "dt_ms": 0.02, "per_kind": c}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -82,14 +99,21 @@ def main():
                       mean_injection_time=100.0)
         trial += 1
     eng.set_timing(True)
+    syn0 = eng.download_synapses() if precision == 1 else None
+    first_timed = trial
     t0 = time.perf_counter()
     spikes = 0
+    results = []
     for _ in range(args.steps):
         r = eng.run_trial(mode=0, training=True, trial_index=trial, first_trial=(trial == 0),
                           mean_injection_time=100.0)
         spikes += sum(map(len, r.soma)) + sum(map(len, r.interneuron))
+        results.append(r)
         trial += 1
     wall = time.perf_counter() - t0
+    precision_check = None
+    if precision == 1:
+        precision_check = fp32_check(net, kw, args, syn0, first_timed, results[0])
     ms_per_step = 1000.0 * wall / args.steps
     value = (net.n_ra + net.n_i) * steps_per_trial * args.steps / wall
 
@@ -130,6 +154,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if precision_check is not None:
+        line["precision_check"] = precision_check
     print(json.dumps(line), flush=True)
     eng.close()
 

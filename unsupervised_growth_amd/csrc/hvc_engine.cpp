@@ -36,6 +36,7 @@ void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, in
 void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_sync(hipStream_t st, const NetDev& nd);
+void launch_sort_posts(hipStream_t st, const NetDev& nd);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
 void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out);
@@ -424,7 +425,6 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
     nd.post_pend = dalloc<int>(ctx, nra, &st);
     nd.post_list = dalloc<uint64_t>(ctx, kPostListCap, &st);
     nd.post_list_cnt = dalloc<int>(ctx, 1, &st);
-    nd.blocks_done = dalloc<unsigned int>(ctx, 1, &st);
     int* d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
     int* d_rr_tgt = dalloc<int>(ctx, E, &st);
     double* d_rr_d = dalloc<double>(ctx, E, &st);
@@ -679,6 +679,7 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
             launch_integrate(s, nd, t0, t1, tr->trial_index, growth);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
+        if (growth) launch_sort_posts(s, nd);
         launch_deliver(s, nd, t0, t1, growth);
         tstop(ctx, kTDeliver, tm);
         tstart(ctx, kTSync, tm);

@@ -374,40 +374,6 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, ui
     }
 }
 
-// the last block to finish sorts the window's post-event list by (step, id)
-__device__ void sort_post_list_if_last(const NetDev& nd)
-{
-    __shared__ bool last;
-    __shared__ uint64_t keys[kPostListCap];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(nd.blocks_done, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    int n = atomicAdd(nd.post_list_cnt, 0);
-    if (n > kPostListCap) n = kPostListCap;
-    int np2 = 1;
-    while (np2 < n) np2 <<= 1;
-    for (int k = threadIdx.x; k < np2; k += blockDim.x) keys[k] = (k < n) ? nd.post_list[k] : ~0ull;
-    __syncthreads();
-    for (int size = 2; size <= np2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < np2; k += blockDim.x) {
-                int p = k ^ stride;
-                if (p > k) {
-                    bool up = ((k & size) == 0);
-                    uint64_t a = keys[k], b = keys[p];
-                    if ((a > b) == up) { keys[k] = b; keys[p] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) nd.post_list[k] = keys[k];
-    if (threadIdx.x == 0) *nd.blocks_done = 0;
-}
-
 // one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
 template <int RA_MODEL, int I_MODEL>
 __global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, uint32_t trial, int growth, int iblocks)
@@ -420,7 +386,6 @@ __global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, uin
     } else {
         integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, trial, growth);
     }
-    if (growth) sort_post_list_if_last(nd);
 }
 
 }  // namespace p32 / p64

@@ -89,7 +89,6 @@ struct NetDev {
     int* post_pend;
     uint64_t* post_list;         // sorted (step << 32 | id)
     int* post_list_cnt;
-    unsigned int* blocks_done;
 
     // RA->RA candidates
     const int* rr_ptr;

@@ -86,7 +86,6 @@ __global__ void k_reset(NetDev nd, uint32_t trial, int first_trial)
     if (i == 0) {
         nd.log_cnt[0] = 0; nd.log_cnt[1] = 0; nd.log_cnt[2] = 0;
         *nd.post_list_cnt = 0;
-        *nd.blocks_done = 0;
         *nd.err = 0;
         nd.warn[0] = 0; nd.warn[1] = 0; nd.warn[2] = 0;
     }
@@ -96,6 +95,37 @@ __global__ void k_clear_pre(int* pre_cnt, int E)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < E) pre_cnt[e] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// the window's post-event list, appended in any order by k_integrate, sorted by
+// (step, id): the order in which the sync block meets new post events
+// (NGS:9375-9386).  One block, bitonic network in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_sort_posts(NetDev nd)
+{
+    __shared__ uint64_t keys[kPostListCap];
+    int n = *nd.post_list_cnt;
+    if (n > kPostListCap) n = kPostListCap;
+    if (n <= 1) return;
+    int np2 = 1;
+    while (np2 < n) np2 <<= 1;
+    for (int k = threadIdx.x; k < np2; k += blockDim.x) keys[k] = (k < n) ? nd.post_list[k] : ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= np2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int k = threadIdx.x; k < np2; k += blockDim.x) {
+                int p = k ^ stride;
+                if (p > k) {
+                    bool up = ((k & size) == 0);
+                    uint64_t a = keys[k], b = keys[p];
+                    if ((a > b) == up) { keys[k] = b; keys[p] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) nd.post_list[k] = keys[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -621,6 +651,11 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth
     int nmax = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
     int blocks_ra = blocks_for(nmax, 64);
     k_deliver<<<3 * blocks_ra, 64, 0, st>>>(nd, t0, t1, growth, blocks_ra);
+}
+
+void launch_sort_posts(hipStream_t st, const NetDev& nd)
+{
+    k_sort_posts<<<1, 1024, 0, st>>>(nd);
 }
 
 void launch_sync(hipStream_t st, const NetDev& nd)
