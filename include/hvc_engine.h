@@ -136,6 +136,28 @@ hvc_status hvc_upload_synapses(hvc_ctx* ctx, const hvc_synapses* syn);
  * update_neuron_properties_sameDendrite_diffMaturationRate, NGS:2136-2183). */
 hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, const double* erest, const double* gca);
 
+/* Batched form: n_nets independent networks resident in one context, laid
+ * side by side in device memory and advanced together by one launch per kernel
+ * and coupling window.  keys holds 2 words per network (its noise-stream key;
+ * hvc_params.seed_* is not used).  Network b's results equal what a context
+ * holding network b alone (seed_lo/hi = keys[2b], keys[2b+1]) produces.  All
+ * networks share hvc_params (models, precision, dt, trial length, synaptic
+ * parameters).  Neuron ids in every per-network call are local to the network.
+ * No counterpart in the reference, which runs one network per simulator
+ * process (NetworkGrowthSimulator, NGS.h:30-40). */
+hvc_status hvc_upload_batch(hvc_ctx* ctx, int n_nets, const hvc_network* nets, const uint32_t* keys);
+
+/* One trial of every network: trials[b] and outs[b] (outs may be NULL) belong to
+ * network b; all trials must have the same mode.  Returns HVC_E_NUMERIC when
+ * any network fails check_bad_values (outs[b].bad_values tells which). */
+hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs);
+
+/* Per-network views of the last trial and of the synapse state (as
+ * hvc_trial_spikes, hvc_download_synapses, hvc_upload_synapses). */
+hvc_status hvc_batch_spikes(hvc_ctx* ctx, int net, int kind, int* counts, double* times, int cap, int* total);
+hvc_status hvc_batch_download_synapses(hvc_ctx* ctx, int net, hvc_synapses* syn);
+hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* syn);
+
 /* Single-neuron traces on the device: the neuron seam of the reference
  * (HH2_test::Debraband_step_no_target_update HH2_test.cpp:535-545,
  * HH2_buffer::R4_step_no_target_update HH2_buffer.cpp:521-528,

@@ -21,7 +21,9 @@ STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: 
 # every entry point declared in include/hvc_engine.h
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
            "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
-           "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_destroy", "hvc_last_error"]
+           "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_upload_batch", "hvc_run_batch",
+           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses", "hvc_destroy",
+           "hvc_last_error"]
 
 
 class EngineNotBuilt(RuntimeError):
@@ -103,6 +105,11 @@ def load(path=LIB_PATH):
     lib.hvc_trace_i.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                 c_double_p, c_double_p, c_double_p, c_u8_p]
     lib.hvc_trial_spikes.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
+    lib.hvc_upload_batch.argtypes = [vp, ctypes.c_int, P(NetworkDesc), P(ctypes.c_uint32)]
+    lib.hvc_run_batch.argtypes = [vp, P(Trial), P(TrialOut)]
+    lib.hvc_batch_spikes.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
+    lib.hvc_batch_download_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
+    lib.hvc_batch_upload_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
     lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
     lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
     lib.hvc_eval_math.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, c_double_p]

@@ -26,14 +26,14 @@ void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint3
                      uint8_t* fired);
 void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
-void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_trial, int E);
-void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
+void launch_reset(hipStream_t st, const NetDev& nd, int E);
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_trace_ra_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
                          const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
                          uint8_t* fired);
 void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                         const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
-void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth);
+void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
 void launch_sync(hipStream_t st, const NetDev& nd);
 void launch_sort_posts(hipStream_t st, const NetDev& nd);
@@ -75,13 +75,24 @@ struct hvc_ctx {
     NetDev nd{};
     int n_ra = 0, n_i = 0, E = 0;
     std::vector<int> rr_ptr, rr_tgt;
-    std::vector<int> train_ids;
+    std::vector<int> train_ids, train_net;
     std::vector<double> spread;
     int* d_kick = nullptr;
     std::vector<int> h_kick;
 
+    // networks resident in the context (one for hvc_upload_network)
+    int n_nets = 0;
+    std::vector<int> ra_off, i_off;
+    std::vector<uint32_t> keys;
+    uint32_t* d_net_trial = nullptr;
+    int* d_net_first = nullptr;
+    std::vector<uint32_t> h_net_trial;
+    std::vector<int> h_net_first;
+
     bool trial_done = false;
     int log_counts[3] = {0, 0, 0};
+    std::vector<int2> h_log[3];
+    bool h_log_valid[3] = {false, false, false};
 
     // timing: HIP events recorded on the engine stream around sampled launches,
     // read after the trial's final synchronisation
@@ -280,71 +291,172 @@ static void sort_fixed(const int* ptr, const int* tgt, const double* w, const do
     }
 }
 
-static hvc_status set_synapse_state(hvc_ctx* ctx, const double* w, const int* act_cnt, const int* act_ids,
-                                    const int* sup_cnt, const int* sup_ids, const int* remodeled)
+// synapse state of the rows [r0, r1) (edges [E0, E1)) from lists of local ids
+// (network-local ids are offset by r0)
+static hvc_status set_synapse_state(hvc_ctx* ctx, int r0, int r1, const double* w, const int* act_cnt,
+                                    const int* act_ids, const int* sup_cnt, const int* sup_ids, const int* remodeled)
 {
-    const int n = ctx->n_ra, E = ctx->E, Nss = ctx->prm.syn.Nss;
-    std::vector<uint8_t> act(E + 1, 0), sup(E + 1, 0);
-    std::vector<int> alist(E + 1, 0), slist((size_t)n * std::max(Nss, 1) + 1, 0);
-    std::vector<int> acnt(n), scnt(n), rem(n);
+    const int n = r1 - r0, Nss = ctx->prm.syn.Nss, NssA = std::max(Nss, 1);
+    const int E0 = ctx->rr_ptr[r0], E1 = ctx->rr_ptr[r1], En = E1 - E0;
+    std::vector<uint8_t> act(En + 1, 0), sup(En + 1, 0);
+    std::vector<int> alist(En + 1, 0), slist((size_t)n * NssA + 1, 0);
+    std::vector<int> acnt(n + 1), scnt(n + 1), rem(n + 1);
     auto edge_of = [&](int i, int j) {
         auto b = ctx->rr_tgt.begin() + ctx->rr_ptr[i], e = ctx->rr_tgt.begin() + ctx->rr_ptr[i + 1];
         auto it = std::lower_bound(b, e, j);
         return (it != e && *it == j) ? (int)(it - ctx->rr_tgt.begin()) : -1;
     };
     int oa = 0, os = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int r = 0; r < n; ++r) {
+        const int i = r0 + r;
         int len = ctx->rr_ptr[i + 1] - ctx->rr_ptr[i];
-        int a = act_cnt ? act_cnt[i] : 0, s = sup_cnt ? sup_cnt[i] : 0;
+        int a = act_cnt ? act_cnt[r] : 0, s = sup_cnt ? sup_cnt[r] : 0;
         if (a < 0 || a > len || s < 0 || s > len || s > Nss) return fail(ctx, HVC_E_ARG, "active/super list size out of range");
-        acnt[i] = a; scnt[i] = s;
+        acnt[r] = a; scnt[r] = s;
         for (int k = 0; k < a; ++k) {
-            int e = edge_of(i, act_ids[oa++]);
-            if (e < 0 || act[e]) return fail(ctx, HVC_E_ARG, "active synapse is not a candidate or repeated");
-            act[e] = 1;
-            alist[ctx->rr_ptr[i] + k] = ctx->rr_tgt[e];
+            int e = edge_of(i, act_ids[oa++] + r0);
+            if (e < 0 || act[e - E0]) return fail(ctx, HVC_E_ARG, "active synapse is not a candidate or repeated");
+            act[e - E0] = 1;
+            alist[ctx->rr_ptr[i] - E0 + k] = ctx->rr_tgt[e];
         }
         for (int k = 0; k < s; ++k) {
-            int e = edge_of(i, sup_ids[os++]);
-            if (e < 0 || sup[e]) return fail(ctx, HVC_E_ARG, "supersynapse is not a candidate or repeated");
-            sup[e] = 1;
-            slist[(size_t)i * Nss + k] = ctx->rr_tgt[e];
+            int e = edge_of(i, sup_ids[os++] + r0);
+            if (e < 0 || sup[e - E0]) return fail(ctx, HVC_E_ARG, "supersynapse is not a candidate or repeated");
+            sup[e - E0] = 1;
+            slist[(size_t)r * NssA + k] = ctx->rr_tgt[e];
         }
-        rem[i] = remodeled ? remodeled[i] : 0;
+        rem[r] = remodeled ? remodeled[r] : 0;
     }
     NetDev& nd = ctx->nd;
     hipStream_t st = ctx->stream;
-    HVC_TRY(hipMemcpyAsync(nd.w, w, sizeof(double) * E, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.act, act.data(), E, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.sup, sup.data(), E, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.act_list, alist.data(), sizeof(int) * E, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.sup_list, slist.data(), sizeof(int) * (size_t)n * Nss, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.act_cnt, acnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.sup_cnt, scnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    HVC_TRY(hipMemcpyAsync(nd.remodeled, rem.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    if (En > 0) {
+        if (w) {
+            HVC_TRY(hipMemcpyAsync(nd.w + E0, w, sizeof(double) * En, hipMemcpyHostToDevice, st));
+        } else {
+            std::vector<double> z(En, 0.0);
+            HVC_TRY(hipMemcpy(nd.w + E0, z.data(), sizeof(double) * En, hipMemcpyHostToDevice));
+        }
+        HVC_TRY(hipMemcpyAsync(nd.act + E0, act.data(), En, hipMemcpyHostToDevice, st));
+        HVC_TRY(hipMemcpyAsync(nd.sup + E0, sup.data(), En, hipMemcpyHostToDevice, st));
+        HVC_TRY(hipMemcpyAsync(nd.act_list + E0, alist.data(), sizeof(int) * En, hipMemcpyHostToDevice, st));
+    }
+    HVC_TRY(hipMemcpyAsync(nd.sup_list + (size_t)r0 * NssA, slist.data(), sizeof(int) * (size_t)n * NssA,
+                           hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.act_cnt + r0, acnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.sup_cnt + r0, scnt.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    HVC_TRY(hipMemcpyAsync(nd.remodeled + r0, rem.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     HVC_TRY(hipStreamSynchronize(st));
     return HVC_OK;
 }
 
-extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
-{
-    if (!ctx || !net) return HVC_E_ARG;
-    if (ctx->uploaded) return fail(ctx, HVC_E_STATE, "network already uploaded to this context");
-    const int nra = net->n_ra, ni = net->n_i;
-    if (nra <= 0 || ni < 0) return fail(ctx, HVC_E_ARG, "n_ra must be > 0 and n_i >= 0");
-    std::string why;
-    if (!check_csr(net->ra2i_ptr, net->ra2i_tgt, net->ra2i_d, nra, ni, false, why)) return fail(ctx, HVC_E_ARG, "RA->I: " + why);
-    if (!check_csr(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_d, ni, nra, false, why)) return fail(ctx, HVC_E_ARG, "I->RA: " + why);
-    if (!check_csr(net->rr_ptr, net->rr_tgt, net->rr_d, nra, nra, true, why)) return fail(ctx, HVC_E_ARG, "RA->RA: " + why);
-    for (int k = 0; k < net->n_train; ++k)
-        if (net->train_ids[k] < 0 || net->train_ids[k] >= nra) return fail(ctx, HVC_E_ARG, "training id out of range");
+// Networks laid side by side: network b's HVC-RA ids are offset by ra_off[b],
+// its HVC-I ids by i_off[b], its candidate edges by e_off[b].  The merged
+// network is block-diagonal, so one launch per kernel advances all of them.
+struct Merged {
+    std::vector<int> ra2i_ptr, ra2i_tgt, i2ra_ptr, i2ra_tgt, rr_ptr, rr_tgt, act_cnt, act_ids, sup_cnt, sup_ids;
+    std::vector<int> remodeled, train_ids, train_net;
+    std::vector<double> ra2i_w, ra2i_d, i2ra_w, i2ra_d, rr_d, rr_w, erest, gca, spread;
+    std::vector<int> ra_off, i_off;
+    hvc_network view{};
+};
 
+static void merge_networks(int n, const hvc_network* nets, Merged& m)
+{
+    m.ra_off.assign(1, 0);
+    m.i_off.assign(1, 0);
+    m.ra2i_ptr.assign(1, 0);
+    m.i2ra_ptr.assign(1, 0);
+    m.rr_ptr.assign(1, 0);
+    for (int b = 0; b < n; ++b) {
+        const hvc_network& x = nets[b];
+        const int ro = m.ra_off.back(), io = m.i_off.back();
+        for (int i = 0; i < x.n_ra; ++i) {
+            for (int e = x.ra2i_ptr[i]; e < x.ra2i_ptr[i + 1]; ++e) {
+                m.ra2i_tgt.push_back(x.ra2i_tgt[e] + io);
+                m.ra2i_w.push_back(x.ra2i_w[e]);
+                m.ra2i_d.push_back(x.ra2i_d[e]);
+            }
+            m.ra2i_ptr.push_back((int)m.ra2i_tgt.size());
+            for (int e = x.rr_ptr[i]; e < x.rr_ptr[i + 1]; ++e) {
+                m.rr_tgt.push_back(x.rr_tgt[e] + ro);
+                m.rr_d.push_back(x.rr_d[e]);
+                m.rr_w.push_back(x.rr_w ? x.rr_w[e] : 0.0);
+            }
+            m.rr_ptr.push_back((int)m.rr_tgt.size());
+            m.act_cnt.push_back(x.act_cnt ? x.act_cnt[i] : 0);
+            m.sup_cnt.push_back(x.sup_cnt ? x.sup_cnt[i] : 0);
+            m.remodeled.push_back(x.remodeled ? x.remodeled[i] : 0);
+            m.erest.push_back(x.erest[i]);
+            m.gca.push_back(x.gca[i]);
+        }
+        int na = 0, ns = 0;
+        for (int i = 0; i < x.n_ra; ++i) {
+            na += x.act_cnt ? x.act_cnt[i] : 0;
+            ns += x.sup_cnt ? x.sup_cnt[i] : 0;
+        }
+        for (int k = 0; k < na; ++k) m.act_ids.push_back(x.act_ids[k] + ro);
+        for (int k = 0; k < ns; ++k) m.sup_ids.push_back(x.sup_ids[k] + ro);
+        for (int i = 0; i < x.n_i; ++i) {
+            for (int e = x.i2ra_ptr[i]; e < x.i2ra_ptr[i + 1]; ++e) {
+                m.i2ra_tgt.push_back(x.i2ra_tgt[e] + ro);
+                m.i2ra_w.push_back(x.i2ra_w[e]);
+                m.i2ra_d.push_back(x.i2ra_d[e]);
+            }
+            m.i2ra_ptr.push_back((int)m.i2ra_tgt.size());
+        }
+        for (int k = 0; k < x.n_train; ++k) {
+            m.train_ids.push_back(x.train_ids[k] + ro);
+            m.train_net.push_back(b);
+            m.spread.push_back(x.spread[k]);
+        }
+        m.ra_off.push_back(ro + x.n_ra);
+        m.i_off.push_back(io + x.n_i);
+    }
+    hvc_network& v = m.view;
+    v.n_ra = m.ra_off.back();
+    v.n_i = m.i_off.back();
+    auto ip = [](std::vector<int>& a) { return a.empty() ? nullptr : a.data(); };
+    auto dp = [](std::vector<double>& a) { return a.empty() ? nullptr : a.data(); };
+    v.ra2i_ptr = m.ra2i_ptr.data(); v.ra2i_tgt = ip(m.ra2i_tgt); v.ra2i_w = dp(m.ra2i_w); v.ra2i_d = dp(m.ra2i_d);
+    v.i2ra_ptr = m.i2ra_ptr.data(); v.i2ra_tgt = ip(m.i2ra_tgt); v.i2ra_w = dp(m.i2ra_w); v.i2ra_d = dp(m.i2ra_d);
+    v.rr_ptr = m.rr_ptr.data(); v.rr_tgt = ip(m.rr_tgt); v.rr_d = dp(m.rr_d); v.rr_w = dp(m.rr_w);
+    v.act_cnt = ip(m.act_cnt); v.act_ids = ip(m.act_ids); v.sup_cnt = ip(m.sup_cnt); v.sup_ids = ip(m.sup_ids);
+    v.remodeled = ip(m.remodeled); v.erest = dp(m.erest); v.gca = dp(m.gca);
+    v.n_train = (int)m.train_ids.size(); v.train_ids = ip(m.train_ids); v.spread = dp(m.spread);
+}
+
+static hvc_status check_network(hvc_ctx* ctx, const hvc_network* net, const char* what)
+{
+    const int nra = net->n_ra, ni = net->n_i;
+    if (nra <= 0 || ni < 0) return fail(ctx, HVC_E_ARG, std::string(what) + ": n_ra must be > 0 and n_i >= 0");
+    std::string why;
+    if (!check_csr(net->ra2i_ptr, net->ra2i_tgt, net->ra2i_d, nra, ni, false, why)) return fail(ctx, HVC_E_ARG, std::string(what) + " RA->I: " + why);
+    if (!check_csr(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_d, ni, nra, false, why)) return fail(ctx, HVC_E_ARG, std::string(what) + " I->RA: " + why);
+    if (!check_csr(net->rr_ptr, net->rr_tgt, net->rr_d, nra, nra, true, why)) return fail(ctx, HVC_E_ARG, std::string(what) + " RA->RA: " + why);
+    if (!net->erest || !net->gca) return fail(ctx, HVC_E_ARG, std::string(what) + ": erest and gca are required");
+    for (int k = 0; k < net->n_train; ++k)
+        if (net->train_ids[k] < 0 || net->train_ids[k] >= nra) return fail(ctx, HVC_E_ARG, std::string(what) + ": training id out of range");
+    return HVC_OK;
+}
+
+// upload of the (merged) network; ra_off / i_off / keys describe the networks in it
+static hvc_status upload_impl(hvc_ctx* ctx, const hvc_network* net, int n_nets, const std::vector<int>& ra_off,
+                              const std::vector<int>& i_off, const uint32_t* keys, const std::vector<int>& train_net)
+{
+    const int nra = net->n_ra, ni = net->n_i;
+    if (nra >= kMaxKeyNeurons) return fail(ctx, HVC_E_ARG, "too many HVC-RA neurons in one context");
+    if (ctx->num_steps >= kMaxKeySteps) return fail(ctx, HVC_E_ARG, "trial has too many steps");
     ctx->n_ra = nra;
     ctx->n_i = ni;
     ctx->E = net->rr_ptr[nra];
+    ctx->n_nets = n_nets;
+    ctx->ra_off = ra_off;
+    ctx->i_off = i_off;
+    ctx->keys.assign(keys, keys + 2 * n_nets);
     ctx->rr_ptr.assign(net->rr_ptr, net->rr_ptr + nra + 1);
     ctx->rr_tgt.assign(net->rr_tgt, net->rr_tgt + ctx->E);
     ctx->train_ids.assign(net->train_ids, net->train_ids + net->n_train);
+    ctx->train_net = train_net;
     ctx->spread.assign(net->spread, net->spread + net->n_train);
 
     // per row: candidates sorted by (delay, target).  Two different delays whose
@@ -366,7 +478,8 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
                 return fail(ctx, HVC_E_ARG, "two RA->RA delays of one row differ by less than 4 ulp of the trial clock");
         }
     }
-    int dense = 1;
+    // a single network whose rows list every target: edge index = row start + target
+    int dense = n_nets == 1 ? 1 : 0;
     for (int i = 0; i < nra && dense; ++i) {
         if (net->rr_ptr[i + 1] - net->rr_ptr[i] != nra) { dense = 0; break; }
         for (int j = 0; j < nra; ++j)
@@ -390,13 +503,30 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
     int cap_ge_ra = max_indeg(net->rr_ptr, net->rr_tgt, nra, nra, 1) + 8;
     int cap_ge_i = max_indeg(net->ra2i_ptr, net->ra2i_tgt, nra, ni, 0) + 8;
 
+    // network of every neuron
+    std::vector<int> ra_net(nra), i_net(std::max(ni, 1), 0);
+    for (int b = 0; b < n_nets; ++b) {
+        for (int i = ra_off[b]; i < ra_off[b + 1]; ++i) ra_net[i] = b;
+        for (int i = i_off[b]; i < i_off[b + 1]; ++i) i_net[i] = b;
+    }
+
     hvc_status st = HVC_OK;
     NetDev& nd = ctx->nd;
     nd.n_ra = nra;
     nd.n_i = ni;
     nd.dense = dense;
+    nd.n_nets = n_nets;
     const int E = ctx->E;
     const int Nss = ctx->prm.syn.Nss;
+    int* d_ra_net = dalloc<int>(ctx, nra, &st);
+    int* d_i_net = dalloc<int>(ctx, std::max(ni, 1), &st);
+    int* d_ra_off = dalloc<int>(ctx, n_nets + 1, &st);
+    int* d_i_off = dalloc<int>(ctx, n_nets + 1, &st);
+    uint32_t* d_key = dalloc<uint32_t>(ctx, 2 * (size_t)n_nets, &st);
+    ctx->d_net_trial = dalloc<uint32_t>(ctx, n_nets, &st);
+    ctx->d_net_first = dalloc<int>(ctx, n_nets, &st);
+    nd.ra_net = d_ra_net; nd.i_net = d_i_net; nd.net_ra_off = d_ra_off; nd.net_i_off = d_i_off;
+    nd.net_key = d_key; nd.net_trial = ctx->d_net_trial; nd.net_first = ctx->d_net_first;
     for (int k = 0; k < 12; ++k) nd.ra[k] = dalloc<double>(ctx, nra, &st);
     nd.ra_flags = dalloc<int>(ctx, nra, &st);
     nd.ra_noise = dalloc<double>(ctx, 4 * (size_t)nra, &st);
@@ -479,63 +609,131 @@ extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
     nd.in_ge_ra = mk_inbox(nra, cap_ge_ra);
     nd.in_ge_i = mk_inbox(ni, cap_ge_i);
     nd.err = dalloc<unsigned int>(ctx, 1, &st);
-    nd.warn = dalloc<int>(ctx, 4, &st);
+    nd.warn = dalloc<int>(ctx, 3 * (size_t)n_nets + 1, &st);
     if (st != HVC_OK) {
         free_bufs(ctx);
         return st;
     }
 
     hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(d_ra_net, ra_net.data(), sizeof(int) * nra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_i_net, i_net.data(), sizeof(int) * std::max(ni, 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_ra_off, ra_off.data(), sizeof(int) * (n_nets + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_i_off, i_off.data(), sizeof(int) * (n_nets + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(d_key, keys, sizeof(uint32_t) * 2 * n_nets, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_EL, net->erest, sizeof(double) * nra, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_GCa, net->gca, sizeof(double) * nra, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_rr_ptr, net->rr_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+    if (E > 0) {
+        HVC_TRY(hipMemcpyAsync(d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+    }
     HVC_TRY(hipMemcpyAsync(d_a_ptr, net->ra2i_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_a_tgt, a_tgt.data(), sizeof(int) * a_tgt.size(), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_a_w, a_w.data(), sizeof(double) * a_w.size(), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_a_d, a_d.data(), sizeof(double) * a_d.size(), hipMemcpyHostToDevice, s));
+    if (!a_tgt.empty()) {
+        HVC_TRY(hipMemcpyAsync(d_a_tgt, a_tgt.data(), sizeof(int) * a_tgt.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_a_w, a_w.data(), sizeof(double) * a_w.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_a_d, a_d.data(), sizeof(double) * a_d.size(), hipMemcpyHostToDevice, s));
+    }
     HVC_TRY(hipMemcpyAsync(d_b_ptr, net->i2ra_ptr, sizeof(int) * (ni + 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_b_tgt, b_tgt.data(), sizeof(int) * b_tgt.size(), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_b_w, b_w.data(), sizeof(double) * b_w.size(), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_b_d, b_d.data(), sizeof(double) * b_d.size(), hipMemcpyHostToDevice, s));
+    if (!b_tgt.empty()) {
+        HVC_TRY(hipMemcpyAsync(d_b_tgt, b_tgt.data(), sizeof(int) * b_tgt.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_b_w, b_w.data(), sizeof(double) * b_w.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(d_b_d, b_d.data(), sizeof(double) * b_d.size(), hipMemcpyHostToDevice, s));
+    }
     HVC_TRY(hipStreamSynchronize(s));
     ctx->uploaded = true;
-    return set_synapse_state(ctx, net->rr_w, net->act_cnt, net->act_ids, net->sup_cnt, net->sup_ids, net->remodeled);
+    return set_synapse_state(ctx, 0, nra, net->rr_w, net->act_cnt, net->act_ids, net->sup_cnt, net->sup_ids,
+                             net->remodeled);
+}
+
+extern "C" hvc_status hvc_upload_network(hvc_ctx* ctx, const hvc_network* net)
+{
+    if (!ctx || !net) return HVC_E_ARG;
+    if (ctx->uploaded) return fail(ctx, HVC_E_STATE, "network already uploaded to this context");
+    hvc_status r = check_network(ctx, net, "network");
+    if (r != HVC_OK) return r;
+    uint32_t key[2] = {ctx->prm.seed_lo, ctx->prm.seed_hi};
+    return upload_impl(ctx, net, 1, {0, net->n_ra}, {0, net->n_i}, key, std::vector<int>(net->n_train, 0));
+}
+
+extern "C" hvc_status hvc_upload_batch(hvc_ctx* ctx, int n_nets, const hvc_network* nets, const uint32_t* keys)
+{
+    if (!ctx || !nets || !keys || n_nets <= 0) return HVC_E_ARG;
+    if (ctx->uploaded) return fail(ctx, HVC_E_STATE, "networks already uploaded to this context");
+    if (n_nets >= kMaxNets) return fail(ctx, HVC_E_ARG, "too many networks in one context");
+    for (int b = 0; b < n_nets; ++b) {
+        char what[32];
+        std::snprintf(what, sizeof what, "network %d", b);
+        hvc_status r = check_network(ctx, &nets[b], what);
+        if (r != HVC_OK) return r;
+    }
+    Merged m;
+    merge_networks(n_nets, nets, m);
+    return upload_impl(ctx, &m.view, n_nets, m.ra_off, m.i_off, keys, m.train_net);
+}
+
+static hvc_status check_net_index(hvc_ctx* ctx, int net)
+{
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    if (net < 0 || net >= ctx->n_nets) return fail(ctx, HVC_E_ARG, "network index out of range");
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* syn)
+{
+    if (!ctx || !syn) return HVC_E_ARG;
+    hvc_status r = check_net_index(ctx, net);
+    if (r != HVC_OK) return r;
+    return set_synapse_state(ctx, ctx->ra_off[net], ctx->ra_off[net + 1], syn->w, syn->act_cnt, syn->act_ids,
+                             syn->sup_cnt, syn->sup_ids, syn->remodeled);
 }
 
 extern "C" hvc_status hvc_upload_synapses(hvc_ctx* ctx, const hvc_synapses* syn)
 {
     if (!ctx || !syn) return HVC_E_ARG;
-    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
-    return set_synapse_state(ctx, syn->w, syn->act_cnt, syn->act_ids, syn->sup_cnt, syn->sup_ids, syn->remodeled);
+    if (ctx->uploaded && ctx->n_nets != 1) return fail(ctx, HVC_E_STATE, "several networks: use hvc_batch_upload_synapses");
+    return hvc_batch_upload_synapses(ctx, 0, syn);
+}
+
+extern "C" hvc_status hvc_batch_download_synapses(hvc_ctx* ctx, int net, hvc_synapses* syn)
+{
+    if (!ctx || !syn) return HVC_E_ARG;
+    hvc_status r = check_net_index(ctx, net);
+    if (r != HVC_OK) return r;
+    const int r0 = ctx->ra_off[net], r1 = ctx->ra_off[net + 1], n = r1 - r0;
+    const int E0 = ctx->rr_ptr[r0], E1 = ctx->rr_ptr[r1], En = E1 - E0, Nss = ctx->prm.syn.Nss;
+    const int NssA = std::max(Nss, 1);
+    NetDev& nd = ctx->nd;
+    std::vector<int> alist(En + 1), slist((size_t)n * NssA + 1), acnt(n + 1), scnt(n + 1), rem(n + 1);
+    hipStream_t s = ctx->stream;
+    if (En > 0) {
+        if (syn->w) HVC_TRY(hipMemcpyAsync(syn->w, nd.w + E0, sizeof(double) * En, hipMemcpyDeviceToHost, s));
+        HVC_TRY(hipMemcpyAsync(alist.data(), nd.act_list + E0, sizeof(int) * En, hipMemcpyDeviceToHost, s));
+    }
+    HVC_TRY(hipMemcpyAsync(slist.data(), nd.sup_list + (size_t)r0 * NssA, sizeof(int) * (size_t)n * NssA,
+                           hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(acnt.data(), nd.act_cnt + r0, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(scnt.data(), nd.sup_cnt + r0, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(rem.data(), nd.remodeled + r0, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    int oa = 0, os = 0;
+    for (int r = 0; r < n; ++r) {
+        const int i = r0 + r;
+        if (syn->act_cnt) syn->act_cnt[r] = acnt[r];
+        if (syn->sup_cnt) syn->sup_cnt[r] = scnt[r];
+        if (syn->remodeled) syn->remodeled[r] = rem[r];
+        if (syn->act_ids) for (int k = 0; k < acnt[r]; ++k) syn->act_ids[oa++] = alist[ctx->rr_ptr[i] - E0 + k] - r0;
+        if (syn->sup_ids) for (int k = 0; k < scnt[r]; ++k) syn->sup_ids[os++] = slist[(size_t)r * NssA + k] - r0;
+    }
+    return HVC_OK;
 }
 
 extern "C" hvc_status hvc_download_synapses(hvc_ctx* ctx, hvc_synapses* syn)
 {
     if (!ctx || !syn) return HVC_E_ARG;
-    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
-    const int n = ctx->n_ra, E = ctx->E, Nss = ctx->prm.syn.Nss;
-    NetDev& nd = ctx->nd;
-    std::vector<int> alist(E), slist((size_t)n * std::max(Nss, 1)), acnt(n), scnt(n), rem(n);
-    hipStream_t s = ctx->stream;
-    if (syn->w) HVC_TRY(hipMemcpyAsync(syn->w, nd.w, sizeof(double) * E, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(alist.data(), nd.act_list, sizeof(int) * E, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(slist.data(), nd.sup_list, sizeof(int) * (size_t)n * Nss, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(acnt.data(), nd.act_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(scnt.data(), nd.sup_cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(rem.data(), nd.remodeled, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipStreamSynchronize(s));
-    int oa = 0, os = 0;
-    for (int i = 0; i < n; ++i) {
-        if (syn->act_cnt) syn->act_cnt[i] = acnt[i];
-        if (syn->sup_cnt) syn->sup_cnt[i] = scnt[i];
-        if (syn->remodeled) syn->remodeled[i] = rem[i];
-        if (syn->act_ids) for (int k = 0; k < acnt[i]; ++k) syn->act_ids[oa++] = alist[ctx->rr_ptr[i] + k];
-        if (syn->sup_ids) for (int k = 0; k < scnt[i]; ++k) syn->sup_ids[os++] = slist[(size_t)i * Nss + k];
-    }
-    return HVC_OK;
+    if (ctx->uploaded && ctx->n_nets != 1) return fail(ctx, HVC_E_STATE, "several networks: use hvc_batch_download_synapses");
+    return hvc_batch_download_synapses(ctx, 0, syn);
 }
 
 extern "C" hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, const double* erest, const double* gca)
@@ -591,19 +789,39 @@ static void tcollect(hvc_ctx* ctx)
     ctx->ev_used = 0;
 }
 
-static hvc_status collect_log(hvc_ctx* ctx, const int2* d_log, int count, int n, int* counts, double* times, int cap,
-                              int* total)
+// the last trial's spike log of one kind, sorted by (neuron, step), fetched once
+static hvc_status sorted_log(hvc_ctx* ctx, int kind, const std::vector<int2>** out)
 {
+    std::vector<int2>& log = ctx->h_log[kind];
+    if (!ctx->h_log_valid[kind]) {
+        const int2* d_log = kind == 0 ? ctx->nd.soma_log : (kind == 1 ? ctx->nd.dend_log : ctx->nd.i_log);
+        int count = ctx->log_counts[kind];
+        log.resize(count);
+        if (count > 0) HVC_TRY(hipMemcpy(log.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
+        std::sort(log.begin(), log.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+        ctx->h_log_valid[kind] = true;
+    }
+    *out = &log;
+    return HVC_OK;
+}
+
+// spikes of neurons [n0, n1) of one kind: counts per neuron (local index),
+// times neuron by neuron
+static hvc_status spikes_range(hvc_ctx* ctx, int kind, int n0, int n1, int* counts, double* times, int cap, int* total)
+{
+    const std::vector<int2>* log = nullptr;
+    hvc_status r = sorted_log(ctx, kind, &log);
+    if (r != HVC_OK) return r;
+    auto lo = std::lower_bound(log->begin(), log->end(), n0, [](const int2& a, int v) { return a.x < v; });
+    auto hi = std::lower_bound(log->begin(), log->end(), n1, [](const int2& a, int v) { return a.x < v; });
+    const int count = (int)(hi - lo);
     *total = count;
-    if (!counts) return HVC_OK;
     if (times && count > cap) return fail(ctx, HVC_E_ARG, "spike-time buffer too small");
-    std::vector<int2> log(count);
-    if (count > 0) HVC_TRY(hipMemcpy(log.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
-    std::sort(log.begin(), log.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-    std::fill(counts, counts + n, 0);
-    for (int k = 0; k < count; ++k) {
-        counts[log[k].x]++;
-        if (times) times[k] = ctx->clock[log[k].y];
+    if (counts) std::fill(counts, counts + (n1 - n0), 0);
+    int k = 0;
+    for (auto it = lo; it != hi; ++it, ++k) {
+        if (counts) counts[it->x - n0]++;
+        if (times) times[k] = ctx->clock[it->y];
     }
     return HVC_OK;
 }
@@ -632,41 +850,60 @@ extern "C" hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* 
     return fail(ctx, HVC_E_ARG, std::string("unknown kernel class ") + name);
 }
 
+extern "C" hvc_status hvc_batch_spikes(hvc_ctx* ctx, int net, int kind, int* counts, double* times, int cap, int* total)
+{
+    if (!ctx || kind < 0 || kind > 2 || !total) return HVC_E_ARG;
+    hvc_status r = check_net_index(ctx, net);
+    if (r != HVC_OK) return r;
+    if (!ctx->trial_done) return fail(ctx, HVC_E_STATE, "no trial has run");
+    const std::vector<int>& off = kind == 2 ? ctx->i_off : ctx->ra_off;
+    return spikes_range(ctx, kind, off[net], off[net + 1], counts, times, cap, total);
+}
+
 extern "C" hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, double* times, int cap, int* total)
 {
     if (!ctx || kind < 0 || kind > 2 || !total) return HVC_E_ARG;
     if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
     if (!ctx->trial_done) return fail(ctx, HVC_E_STATE, "no trial has run");
-    const int2* log = kind == 0 ? ctx->nd.soma_log : (kind == 1 ? ctx->nd.dend_log : ctx->nd.i_log);
-    int n = kind == 2 ? ctx->n_i : ctx->n_ra;
-    return collect_log(ctx, log, ctx->log_counts[kind], n, counts, times, cap, total);
+    return spikes_range(ctx, kind, 0, kind == 2 ? ctx->n_i : ctx->n_ra, counts, times, cap, total);
 }
 
-extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial_out* out)
+// One trial of every network in the context (trials[b] for network b); the
+// networks advance together, one launch per kernel and coupling window.
+static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs)
 {
-    if (!ctx || !tr) return HVC_E_ARG;
-    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
-    if (tr->mode != 0 && tr->mode != 1) return fail(ctx, HVC_E_ARG, "mode must be 0 (growth) or 1 (chain test)");
+    const int B = ctx->n_nets;
+    const int mode = trials[0].mode;
+    if (mode != 0 && mode != 1) return fail(ctx, HVC_E_ARG, "mode must be 0 (growth) or 1 (chain test)");
+    for (int b = 1; b < B; ++b)
+        if (trials[b].mode != mode) return fail(ctx, HVC_E_ARG, "all trials of a batch must have the same mode");
     HVC_TRY(hipSetDevice(ctx->device));
-    const int growth = tr->mode == 0;
-    const int training = growth ? tr->training : 1;
+    const int growth = mode == 0;
     NetDev& nd = ctx->nd;
     hipStream_t s = ctx->stream;
 
     // training kick: first step with internal_time > spread + mean_injection_time (NGS:8933-8940)
     ctx->h_kick.assign(ctx->n_ra, -1);
-    if (training) {
-        for (size_t k = 0; k < ctx->train_ids.size(); ++k) {
-            double thr = ctx->spread[k] + tr->mean_injection_time;
-            for (int t = 1; t < ctx->num_steps; ++t)
-                if (ctx->clock[t] > thr) {
-                    ctx->h_kick[ctx->train_ids[k]] = t;
-                    break;
-                }
-        }
+    for (size_t k = 0; k < ctx->train_ids.size(); ++k) {
+        const hvc_trial& tr = trials[ctx->train_net[k]];
+        if (growth && !tr.training) continue;
+        double thr = ctx->spread[k] + tr.mean_injection_time;
+        for (int t = 1; t < ctx->num_steps; ++t)
+            if (ctx->clock[t] > thr) {
+                ctx->h_kick[ctx->train_ids[k]] = t;
+                break;
+            }
+    }
+    ctx->h_net_trial.resize(B);
+    ctx->h_net_first.resize(B);
+    for (int b = 0; b < B; ++b) {
+        ctx->h_net_trial[b] = trials[b].trial_index;
+        ctx->h_net_first[b] = trials[b].first_trial ? 1 : 0;
     }
     HVC_TRY(hipMemcpyAsync(ctx->d_kick, ctx->h_kick.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
-    launch_reset(s, nd, tr->trial_index, tr->first_trial, ctx->E);
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, ctx->h_net_trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_first, ctx->h_net_first.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    launch_reset(s, nd, ctx->E);
     if (!growth) launch_chain_edges(s, nd);
     const int nw = (int)ctx->windows.size();
     for (int w = 0; w < nw; ++w) {
@@ -674,9 +911,9 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1, tr->trial_index, growth);
+            launch_integrate_f32(s, nd, t0, t1, growth);
         else
-            launch_integrate(s, nd, t0, t1, tr->trial_index, growth);
+            launch_integrate(s, nd, t0, t1, growth);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
         if (growth) launch_sort_posts(s, nd);
@@ -693,57 +930,82 @@ extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial
     }
     HVC_TRY(hipGetLastError());
     unsigned int err = 0;
-    int logc[4] = {0, 0, 0, 0}, warn[4] = {0, 0, 0, 0};
+    int logc[4] = {0, 0, 0, 0};
+    std::vector<int> warn(3 * (size_t)B);
     HVC_TRY(hipMemcpyAsync(&err, nd.err, sizeof(err), hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(logc, nd.log_cnt, sizeof(logc), hipMemcpyDeviceToHost, s));
-    HVC_TRY(hipMemcpyAsync(warn, nd.warn, sizeof(warn), hipMemcpyDeviceToHost, s));
+    HVC_TRY(hipMemcpyAsync(warn.data(), nd.warn, sizeof(int) * 3 * B, hipMemcpyDeviceToHost, s));
     std::vector<double> vs(ctx->n_ra), vd(ctx->n_ra);
     HVC_TRY(hipMemcpyAsync(vs.data(), nd.ra[0], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(vd.data(), nd.ra[1], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
     tcollect(ctx);
     ctx->trial_done = true;
-    for (int k = 0; k < 3; ++k) ctx->log_counts[k] = logc[k];
+    for (int k = 0; k < 3; ++k) {
+        ctx->log_counts[k] = logc[k];
+        ctx->h_log_valid[k] = false;
+    }
     if (err) {
         char buf[160];
         std::snprintf(buf, sizeof buf, "engine capacity exceeded (flags 0x%x); trial result discarded", err);
         return fail(ctx, HVC_E_CAPACITY, buf);
     }
-    // check_bad_values (HH2_test.cpp:244-260)
-    int bad = 0;
-    for (int i = 0; i < ctx->n_ra; ++i) {
-        if ((vs[i] > 100) || (vs[i] < -200) || std::isnan(vs[i]) || (vd[i] > 100) || (vd[i] < -200) || std::isnan(vd[i])) {
-            bad = -1;
-            break;
+    // check_bad_values (HH2_test.cpp:244-260), per network
+    int any_bad = 0;
+    for (int b = 0; b < B; ++b) {
+        int bad = 0;
+        for (int i = ctx->ra_off[b]; i < ctx->ra_off[b + 1]; ++i) {
+            if ((vs[i] > 100) || (vs[i] < -200) || std::isnan(vs[i]) || (vd[i] > 100) || (vd[i] < -200) ||
+                std::isnan(vd[i])) {
+                bad = -1;
+                break;
+            }
         }
-    }
-    if (out) {
+        any_bad |= bad;
+        if (!outs) continue;
+        hvc_trial_out* out = &outs[b];
+        const int r0 = ctx->ra_off[b], r1 = ctx->ra_off[b + 1], i0 = ctx->i_off[b], i1 = ctx->i_off[b + 1];
         hvc_status r;
-        if ((r = collect_log(ctx, nd.soma_log, logc[0], ctx->n_ra, out->soma_counts, out->soma_times, out->soma_cap,
-                             &out->soma_total)) != HVC_OK) return r;
-        if ((r = collect_log(ctx, nd.dend_log, logc[1], ctx->n_ra, out->dend_counts, out->dend_times, out->dend_cap,
-                             &out->dend_total)) != HVC_OK) return r;
-        if ((r = collect_log(ctx, nd.i_log, logc[2], ctx->n_i, out->i_counts, out->i_times, out->i_cap,
-                             &out->i_total)) != HVC_OK) return r;
+        if ((r = spikes_range(ctx, 0, r0, r1, out->soma_counts, out->soma_times, out->soma_cap, &out->soma_total)) != HVC_OK)
+            return r;
+        if ((r = spikes_range(ctx, 1, r0, r1, out->dend_counts, out->dend_times, out->dend_cap, &out->dend_total)) != HVC_OK)
+            return r;
+        if ((r = spikes_range(ctx, 2, i0, i1, out->i_counts, out->i_times, out->i_cap, &out->i_total)) != HVC_OK)
+            return r;
         out->bad_values = bad;
-        out->warnings[0] = warn[0]; out->warnings[1] = warn[1]; out->warnings[2] = warn[2];
-        if (out->ra_state) {
-            std::vector<double> col(ctx->n_ra);
+        for (int k = 0; k < 3; ++k) out->warnings[k] = warn[3 * b + k];
+        if (out->ra_state && r1 > r0) {
+            std::vector<double> col(r1 - r0);
             for (int k = 0; k < 12; ++k) {
-                HVC_TRY(hipMemcpy(col.data(), nd.ra[k], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost));
-                for (int i = 0; i < ctx->n_ra; ++i) out->ra_state[(size_t)i * 12 + k] = col[i];
+                HVC_TRY(hipMemcpy(col.data(), nd.ra[k] + r0, sizeof(double) * (r1 - r0), hipMemcpyDeviceToHost));
+                for (int i = 0; i < r1 - r0; ++i) out->ra_state[(size_t)i * 12 + k] = col[i];
             }
         }
-        if (out->i_state && ctx->n_i > 0) {
-            std::vector<double> col(ctx->n_i);
+        if (out->i_state && i1 > i0) {
+            std::vector<double> col(i1 - i0);
             for (int k = 0; k < 7; ++k) {
-                HVC_TRY(hipMemcpy(col.data(), nd.in[k], sizeof(double) * ctx->n_i, hipMemcpyDeviceToHost));
-                for (int i = 0; i < ctx->n_i; ++i) out->i_state[(size_t)i * 7 + k] = col[i];
+                HVC_TRY(hipMemcpy(col.data(), nd.in[k] + i0, sizeof(double) * (i1 - i0), hipMemcpyDeviceToHost));
+                for (int i = 0; i < i1 - i0; ++i) out->i_state[(size_t)i * 7 + k] = col[i];
             }
         }
     }
-    if (bad) return fail(ctx, HVC_E_NUMERIC, "check_bad_values: HVC-RA voltage out of [-200, 100] or NaN");
+    if (any_bad) return fail(ctx, HVC_E_NUMERIC, "check_bad_values: HVC-RA voltage out of [-200, 100] or NaN");
     return HVC_OK;
+}
+
+extern "C" hvc_status hvc_run_trial(hvc_ctx* ctx, const hvc_trial* tr, hvc_trial_out* out)
+{
+    if (!ctx || !tr) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    if (ctx->n_nets != 1) return fail(ctx, HVC_E_STATE, "several networks: use hvc_run_batch");
+    return run_impl(ctx, tr, out);
+}
+
+extern "C" hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs)
+{
+    if (!ctx || !trials) return HVC_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload networks first");
+    return run_impl(ctx, trials, outs);
 }
 
 // ---------------------------------------------------------------------------

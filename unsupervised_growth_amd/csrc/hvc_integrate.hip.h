@@ -198,7 +198,7 @@ __device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool p
 // longer per-step work, and the RA blocks run beside them.
 // ---------------------------------------------------------------------------
 template <int MODEL>
-__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial, int growth)
+__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, int growth)
 {
     const bool live = i < nd.n_ra;
     const int ii = live ? i : 0;
@@ -232,7 +232,8 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uin
     }
     int new_posts = 0;
 
-    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamRAPoisson, (uint32_t)ii, nd.ra_draw[ii]};
+    const NoiseId sid = ra_noise_id(nd, ii);
+    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamRAPoisson, sid.local, nd.ra_draw[ii]};
     double nes = 0, nis = 0, ned = 0, nid = 0;
     if (MODEL == 1 && nd.ra_poisson) {
         nes = nd.ra_noise[ii]; nis = nd.ra_noise[(size_t)nd.n_ra + ii];
@@ -245,7 +246,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uin
         int f = ra_state_check(s.Vs, s.Vd, fs, fd);
         const DecayRow& row = rows[t];
         if (MODEL == 0) {
-            U4 o = philox4x32((uint32_t)t, (uint32_t)ii, trial, kStreamRAWhite, nd.seed_lo, nd.seed_hi);
+            U4 o = philox4x32((uint32_t)t, sid.local, sid.trial, kStreamRAWhite, sid.k0, sid.k1);
             ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
                          kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
                          row.e5[kOffDt], row.e5[kOffHalf]);
@@ -271,7 +272,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uin
             if (pcnt == 0 || it > last_post + 30.0) {
                 int pos = atomicAdd(nd.post_list_cnt, 1);
                 if (pos < kPostListCap)
-                    nd.post_list[pos] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)i;
+                    nd.post_list[pos] = post_key(nd.ra_net[i], t, i);
                 else
                     atomicOr(nd.err, (unsigned)kErrPostList);
                 ++new_posts;
@@ -295,7 +296,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, uin
 }
 
 // HVC-I, one lane per neuron (RK4 form)
-__device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1, uint32_t trial)
+__device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
 {
     const bool live = i < nd.n_i;
     const int ii = live ? i : 0;
@@ -303,7 +304,8 @@ __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1, uint
     IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
              (real)nd.in[5][ii], (real)nd.in[6][ii]};
     int flag = nd.i_flag[ii];
-    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    const NoiseId sid = i_noise_id(nd, ii);
+    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
     double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
@@ -332,7 +334,7 @@ constexpr int kILanes = 5;                    // lanes per HVC-I neuron (DP8)
 constexpr int kINeuronsPerWave = 64 / kILanes; // 12 neurons, 4 idle lanes
 
 // HVC-I, five lanes per neuron (DP8 form); one wave holds 12 neurons
-__device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, uint32_t trial)
+__device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
 {
     // the four spare lanes 60..63 shadow lanes 55..58 of the last group: same
     // neuron, same roles, results discarded -- their quotients then stay in the
@@ -349,7 +351,8 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, ui
     IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
              (real)nd.in[5][ii], (real)nd.in[6][ii]};
     int flag = nd.i_flag[ii];
-    PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)ii, nd.i_draw[ii]};
+    const NoiseId sid = i_noise_id(nd, ii);
+    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
     double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
@@ -376,15 +379,15 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1, ui
 
 // one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
 template <int RA_MODEL, int I_MODEL>
-__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, uint32_t trial, int growth, int iblocks)
+__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, int growth, int iblocks)
 {
     if ((int)blockIdx.x < iblocks) {
         if (I_MODEL == 0)
-            integrate_i_lanes(nd, blockIdx.x, t0, t1, trial);
+            integrate_i_lanes(nd, blockIdx.x, t0, t1);
         else
-            integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1, trial);
+            integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1);
     } else {
-        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, trial, growth);
+        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, growth);
     }
 }
 
@@ -409,20 +412,20 @@ void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n
     k_trace_i<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, first_trial, trial, exc_g, inh_g, out, fired);
 }
 
-void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, uint32_t trial, int growth)
+void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, int growth)
 {
     int iblocks = nd.n_i == 0 ? 0 : (nd.i_model == 0 ? (nd.n_i + kINeuronsPerWave - 1) / kINeuronsPerWave
                                                      : blocks_for(nd.n_i, 64));
     int rblocks = blocks_for(nd.n_ra, 64);
     dim3 grid(iblocks + rblocks);
     if (nd.ra_model == 0 && nd.i_model == 0)
-        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
     else if (nd.ra_model == 0)
-        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
     else if (nd.i_model == 0)
-        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
     else
-        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, trial, growth, iblocks);
+        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
 }
 
 #undef HVC_LAUNCHER

@@ -11,7 +11,8 @@ constexpr int kPostRing = 8;     // post-event times kept per HVC-RA neuron
 constexpr int kPreRing = 4;      // delivered pre-event times kept per candidate edge
 constexpr int kMaxStreams = 16;  // in-flight RA->RA spikes per presynaptic row
 constexpr int kMaxInflight = 16; // in-flight spikes per neuron for fixed-weight pushes
-constexpr int kPostListCap = 4096;
+constexpr int kPostListCap = 8192;  // new post events per window, all networks together
+constexpr int kMaxNets = 4096;      // networks per context (12 bits of a post-event key)
 
 // error flags raised by kernels (bit mask)
 enum ErrFlag : uint32_t {
@@ -21,6 +22,22 @@ enum ErrFlag : uint32_t {
     kErrInflight = 8u,
     kErrInbox = 16u,
     kErrPostRing = 32u,
+};
+
+// post-event key: network (12 bits), step (24 bits), neuron (28 bits); the
+// sorted keys list each network's events by (step, neuron)
+__host__ __device__ __forceinline__ uint64_t post_key(int net, int step, int id)
+{
+    return ((uint64_t)(uint32_t)net << 52) | ((uint64_t)(uint32_t)step << 28) | (uint64_t)(uint32_t)id;
+}
+__host__ __device__ __forceinline__ int post_key_id(uint64_t k) { return (int)(k & 0xfffffffull); }
+constexpr int kMaxKeySteps = 1 << 24;
+constexpr int kMaxKeyNeurons = 1 << 28;
+
+// identity of one neuron's noise streams: key of its network, trial index of
+// its network, neuron id within its network
+struct NoiseId {
+    uint32_t k0, k1, trial, local;
 };
 
 struct Record {
@@ -58,8 +75,19 @@ struct NetDev {
     const void* decay;           // DecayRow [num_steps] (row t used by step t)
     double sq_sig_s, sq_sig_d, mu_term_s, mu_term_d;
     int ra_poisson, i_poisson;
-    uint32_t seed_lo, seed_hi;
+    uint32_t seed_lo, seed_hi;   // noise key of the single-neuron traces
     hvc_synaptic_params syn;
+
+    // networks resident in this context: neurons of network b are the RA ids
+    // [net_ra_off[b], net_ra_off[b+1]) and the I ids [net_i_off[b], net_i_off[b+1])
+    int n_nets;
+    const int* ra_net;           // [n_ra] network of each HVC-RA
+    const int* i_net;            // [n_i]
+    const int* net_ra_off;       // [n_nets + 1]
+    const int* net_i_off;        // [n_nets + 1]
+    const uint32_t* net_key;     // [2 n_nets] noise-stream key
+    const uint32_t* net_trial;   // [n_nets] trial index of the running trial
+    const int* net_first;        // [n_nets] first trial after initialisation (NGS:849-850)
 
     // RA state (SoA)
     double* ra[12];              // Vs Vd n l h r c Ca Ge_s Gi_s Ge_d Gi_d
@@ -87,7 +115,7 @@ struct NetDev {
     double* post_ring;           // [n_ra][kPostRing]
     int* post_cnt;
     int* post_pend;
-    uint64_t* post_list;         // sorted (step << 32 | id)
+    uint64_t* post_list;         // post_key(network, step, neuron), sorted
     int* post_list_cnt;
 
     // RA->RA candidates
@@ -123,7 +151,19 @@ struct NetDev {
     Inbox in_gi_ra, in_ge_ra, in_ge_i;
 
     unsigned int* err;
-    int* warn;                   // [3]
+    int* warn;                   // [3 n_nets]
 };
+
+__device__ __forceinline__ NoiseId ra_noise_id(const NetDev& nd, int i)
+{
+    int b = nd.ra_net[i];
+    return NoiseId{nd.net_key[2 * b], nd.net_key[2 * b + 1], nd.net_trial[b], (uint32_t)(i - nd.net_ra_off[b])};
+}
+
+__device__ __forceinline__ NoiseId i_noise_id(const NetDev& nd, int i)
+{
+    int b = nd.i_net[i];
+    return NoiseId{nd.net_key[2 * b], nd.net_key[2 * b + 1], nd.net_trial[b], (uint32_t)(i - nd.net_i_off[b])};
+}
 
 }  // namespace hvc

@@ -47,7 +47,7 @@ __global__ void k_decay_table(const double* clock, const double* offsets, int nu
 // trial start: setToRest_afterEpoch (NGS:4591-4621), HH2_test::set_to_rest
 // (HH2_test.cpp:191-242), HHI_buffer::set_to_rest (HHI_buffer.cpp:178-200)
 // ---------------------------------------------------------------------------
-__global__ void k_reset(NetDev nd, uint32_t trial, int first_trial)
+__global__ void k_reset(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nd.n_ra) {
@@ -57,7 +57,8 @@ __global__ void k_reset(NetDev nd, uint32_t trial, int first_trial)
         nd.ra_flags[i] = 0;
         nd.ra_draw[i] = 0;
         if (nd.ra_poisson) {
-            PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamRAPoisson, (uint32_t)i, 0u};
+            const NoiseId sid = ra_noise_id(nd, i);
+            PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamRAPoisson, sid.local, 0u};
             for (int q = 0; q < 4; ++q) nd.ra_noise[(size_t)q * nd.n_ra + i] = 1000.0 * ps.spike_time(100.0);
             nd.ra_draw[i] = ps.draw;
         }
@@ -73,9 +74,10 @@ __global__ void k_reset(NetDev nd, uint32_t trial, int first_trial)
         nd.in[0][i] = -66.0; nd.in[1][i] = 0.125; nd.in[2][i] = 0.0; nd.in[3][i] = 0.99; nd.in[4][i] = 0.0;
         nd.in[5][i] = 0.0; nd.in[6][i] = 0.0;
         nd.i_flag[i] = 0;
-        PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)i, 0u};
+        const NoiseId sid = i_noise_id(nd, i);
+        PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, 0u};
         if (nd.i_poisson) {
-            double init_dt = first_trial ? 0.0 : nd.dt;
+            double init_dt = nd.net_first[nd.i_net[i]] ? 0.0 : nd.dt;
             nd.i_noise[i] = i_init_noise(ps, init_dt);
             nd.i_noise[(size_t)nd.n_i + i] = i_init_noise(ps, init_dt);
         }
@@ -87,8 +89,8 @@ __global__ void k_reset(NetDev nd, uint32_t trial, int first_trial)
         nd.log_cnt[0] = 0; nd.log_cnt[1] = 0; nd.log_cnt[2] = 0;
         *nd.post_list_cnt = 0;
         *nd.err = 0;
-        nd.warn[0] = 0; nd.warn[1] = 0; nd.warn[2] = 0;
     }
+    if (i < 3 * nd.n_nets) nd.warn[i] = 0;
 }
 
 __global__ void k_clear_pre(int* pre_cnt, int E)
@@ -256,6 +258,20 @@ __device__ __forceinline__ bool list_erase(int* list, int* cnt, int j)
     return false;
 }
 
+// warning counters of the network that owns HVC-RA row i
+__device__ __forceinline__ int* row_warn(const NetDev& nd, int i) { return nd.warn + 3 * nd.ra_net[i]; }
+
+// first index of the sorted post-event keys that is >= key
+__device__ __forceinline__ int post_lower_bound(const uint64_t* keys, int n, uint64_t key)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 // NGS:16546-16592
 __device__ void update_synapse(const NetDev& nd, int i, int e)
 {
@@ -275,11 +291,11 @@ __device__ void update_synapse(const NetDev& nd, int i, int e)
     if ((w < p.SUPERSYNAPSE_THRESHOLD) && (nd.sup[e] == 1)) {
         nd.sup[e] = 0;
         nd.remodeled[i] = 0;
-        if (!list_erase(slist, &nd.sup_cnt[i], j)) atomicAdd(&nd.warn[2], 1);
+        if (!list_erase(slist, &nd.sup_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
     }
     if ((w < p.ACTIVATION_THRESHOLD) && (nd.act[e] == 1)) {
         nd.act[e] = 0;
-        if (!list_erase(alist, &nd.act_cnt[i], j)) atomicAdd(&nd.warn[2], 1);
+        if (!list_erase(alist, &nd.act_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
     }
 }
 
@@ -312,7 +328,7 @@ __device__ void ltd_on_delivery(const NetDev& nd, int i, int e, int tgt, double 
     while (k < pc && ring[k % kPostRing] < lim) ++k;
     for (; k < pc; ++k) {
         double dt = ring[k % kPostRing] - T;
-        nd.w[e] = ltd(nd.syn, nd.w[e], dt, nd.warn);
+        nd.w[e] = ltd(nd.syn, nd.w[e], dt, row_warn(nd, i));
         update_synapse(nd, i, e);
     }
 }
@@ -458,15 +474,20 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
         nd.st_count[i] = sc;
     }
 
-    // sync at step t1: LTP / LTD against this window's new post events (NGS:9375-9542)
+    // sync at step t1: LTP / LTD against this window's new post events of the
+    // row's own network (NGS:9375-9542)
     int np = *nd.post_list_cnt;
     if (np > kPostListCap) np = kPostListCap;
     if (np == 0) return;
+    const int net = nd.ra_net[i];
+    const int k_lo = post_lower_bound(nd.post_list, np, post_key(net, 0, 0));
+    const int k_hi = post_lower_bound(nd.post_list, np, post_key(net + 1, 0, 0));
+    if (k_lo == k_hi) return;
     const double now = nd.clock[t1];
     const bool saturated = (nd.sup_cnt[i] == Nss);
     const int* slist = nd.sup_list + (size_t)i * Nss;
-    for (int k = 0; k < np; ++k) {
-        int post = (int)(uint32_t)(nd.post_list[k] & 0xffffffffull);
+    for (int k = k_lo; k < k_hi; ++k) {
+        int post = post_key_id(nd.post_list[k]);
         if (post == i) continue;
         if (saturated) {
             bool found = false;
@@ -486,9 +507,9 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
         for (; q < pc; ++q) {
             double dt = now - ring[q % kPreRing];
             if (dt <= nd.syn.T_0)
-                nd.w[e] = ltd(nd.syn, nd.w[e], dt, nd.warn);
+                nd.w[e] = ltd(nd.syn, nd.w[e], dt, row_warn(nd, i));
             else
-                nd.w[e] = ltp(nd.syn, nd.w[e], dt, nd.warn);
+                nd.w[e] = ltp(nd.syn, nd.w[e], dt, row_warn(nd, i));
             update_synapse(nd, i, e);
         }
     }
@@ -639,10 +660,11 @@ void launch_decay_table(hipStream_t st, const double* clock, const double* offse
     k_decay_table<<<blocks_for(num_steps, 128), 128, 0, st>>>(clock, offsets, num_steps, (DecayRow*)rows);
 }
 
-void launch_reset(hipStream_t st, const NetDev& nd, uint32_t trial, int first_trial, int E)
+void launch_reset(hipStream_t st, const NetDev& nd, int E)
 {
     int n = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
-    k_reset<<<blocks_for(n, 256), 256, 0, st>>>(nd, trial, first_trial);
+    if (3 * nd.n_nets > n) n = 3 * nd.n_nets;
+    k_reset<<<blocks_for(n, 256), 256, 0, st>>>(nd);
     if (E > 0) k_clear_pre<<<blocks_for(E, 256), 256, 0, st>>>(nd.pre_cnt, E);
 }
 

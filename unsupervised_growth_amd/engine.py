@@ -95,6 +95,7 @@ class Engine:
             self.ctx = None
             raise EngineError(st, msg)
         self.net = None
+        self.nets = []
 
     def _check(self, st):
         if st != _abi.HVC_OK:
@@ -118,7 +119,8 @@ class Engine:
         self.close()
 
     # ------------------------------------------------------------------
-    def upload(self, net: Network):
+    @staticmethod
+    def _desc(net: Network):
         keep = {k: (_i32(v) if v.dtype.kind in "iu" else _f64(v)) for k, v in net.__dict__.items()
                 if isinstance(v, np.ndarray)}
         d = _abi.NetworkDesc()
@@ -130,8 +132,29 @@ class Engine:
         for name in ("ra2i_w", "ra2i_d", "i2ra_w", "i2ra_d", "rr_d", "rr_w", "erest", "gca", "spread"):
             setattr(d, name, _p(keep[name], dp))
         d.n_train = len(net.train_ids)
+        return d, keep
+
+    def upload(self, net: Network):
+        d, keep = self._desc(net)
         self._check(self.lib.hvc_upload_network(self.ctx, ctypes.byref(d)))
         self.net = net
+        self.nets = [net]
+        return self
+
+    def upload_batch(self, nets, keys):
+        """Several independent networks in this context (the batched form).
+        keys: one (seed_lo, seed_hi) pair per network."""
+        descs, keeps = [], []
+        for n in nets:
+            d, k = self._desc(n)
+            descs.append(d)
+            keeps.append(k)
+        arr = (_abi.NetworkDesc * len(nets))(*descs)
+        kk = np.ascontiguousarray(np.asarray(keys, np.uint32).reshape(len(nets), 2))
+        self._check(self.lib.hvc_upload_batch(self.ctx, len(nets), arr,
+                                              kk.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        self.nets = list(nets)
+        self.net = self.nets[0] if len(self.nets) == 1 else None
         return self
 
     def run_trial(self, mode=0, training=True, trial_index=0, first_trial=False, mean_injection_time=100.0,
@@ -163,22 +186,63 @@ class Engine:
         return TrialResult(lists[0], lists[1], lists[2], o.bad_values, tuple(o.warnings), ras,
                            ins[:n_i] if ins is not None else None)
 
-    def download_synapses(self) -> SynapseState:
-        E, n = self.net.n_candidates, self.net.n_ra
-        w = np.zeros(E)
+    def _spikes(self, net_index, kind, n):
+        c = np.zeros(n, np.int32)
+        tot = ctypes.c_int()
+        ip, dp = _abi.c_int_p, _abi.c_double_p
+        self._check(self.lib.hvc_batch_spikes(self.ctx, net_index, kind, _p(c, ip) if n else None, None, 0,
+                                              ctypes.byref(tot)))
+        times = np.zeros(max(tot.value, 1))
+        self._check(self.lib.hvc_batch_spikes(self.ctx, net_index, kind, _p(c, ip) if n else None,
+                                              _p(times, dp), len(times), ctypes.byref(tot)))
+        return np.split(times[:tot.value], np.cumsum(c)[:-1]) if n else []
+
+    def run_batch(self, trials):
+        """One trial of every network of a batch; trials: list of dicts with the
+        keyword arguments of run_trial (mode must agree).  Returns TrialResults."""
+        if not getattr(self, "nets", None):
+            raise EngineError(_abi.HVC_E_STATE, "upload networks first")
+        B = len(self.nets)
+        if len(trials) != B:
+            raise ValueError("one trial per network")
+        ts = (_abi.Trial * B)()
+        for b, kw in enumerate(trials):
+            ts[b] = _abi.Trial(int(kw.get("mode", 0)), int(bool(kw.get("training", True))),
+                               int(kw.get("trial_index", 0)) & 0xffffffff, int(bool(kw.get("first_trial", False))),
+                               float(kw.get("mean_injection_time", 100.0)))
+        st = self.lib.hvc_run_batch(self.ctx, ts, None)
+        if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
+            self._check(st)
+        out = []
+        for b, n in enumerate(self.nets):
+            out.append(TrialResult(self._spikes(b, 0, n.n_ra), self._spikes(b, 1, n.n_ra),
+                                   self._spikes(b, 2, n.n_i), 0, (0, 0, 0)))
+        return out
+
+    def download_synapses(self, net_index=None) -> SynapseState:
+        b = 0 if net_index is None else net_index
+        net = self.nets[b]
+        E, n = net.n_candidates, net.n_ra
+        w = np.zeros(max(E, 1))
         ac, sc, rem = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
         ai, si = np.zeros(max(E, 1), np.int32), np.zeros(max(n * max(1, self.params.syn.Nss), 1), np.int32)
         s = _abi.Synapses(_p(w, _abi.c_double_p), _p(ac, _abi.c_int_p), _p(ai, _abi.c_int_p),
                           _p(sc, _abi.c_int_p), _p(si, _abi.c_int_p), _p(rem, _abi.c_int_p))
-        self._check(self.lib.hvc_download_synapses(self.ctx, ctypes.byref(s)))
-        return SynapseState(w, ac, ai[:int(ac.sum())].copy(), sc, si[:int(sc.sum())].copy(), rem)
+        if net_index is None:
+            self._check(self.lib.hvc_download_synapses(self.ctx, ctypes.byref(s)))
+        else:
+            self._check(self.lib.hvc_batch_download_synapses(self.ctx, b, ctypes.byref(s)))
+        return SynapseState(w[:E], ac, ai[:int(ac.sum())].copy(), sc, si[:int(sc.sum())].copy(), rem)
 
-    def upload_synapses(self, syn: SynapseState):
+    def upload_synapses(self, syn: SynapseState, net_index=None):
         w, ac, ai = _f64(syn.w), _i32(syn.act_cnt), _i32(syn.act_ids)
         sc, si, rem = _i32(syn.sup_cnt), _i32(syn.sup_ids), _i32(syn.remodeled)
         s = _abi.Synapses(_p(w, _abi.c_double_p), _p(ac, _abi.c_int_p), _p(ai, _abi.c_int_p),
                           _p(sc, _abi.c_int_p), _p(si, _abi.c_int_p), _p(rem, _abi.c_int_p))
-        self._check(self.lib.hvc_upload_synapses(self.ctx, ctypes.byref(s)))
+        if net_index is None:
+            self._check(self.lib.hvc_upload_synapses(self.ctx, ctypes.byref(s)))
+        else:
+            self._check(self.lib.hvc_batch_upload_synapses(self.ctx, net_index, ctypes.byref(s)))
 
     def patch_neurons(self, ids, erest, gca):
         ids, erest, gca = _i32(ids), _f64(erest), _f64(gca)
