@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--cpu-sample-ms", type=float, default=5.0,
                     help="trial length of the CPU-baseline sample (same network)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nets", type=int, default=64, help="c4: number of networks")
+    ap.add_argument("--check-nets", type=int, default=4, help="c4: networks re-run alone for comparison")
     return ap.parse_args()
 
 
@@ -73,7 +75,105 @@ def fp32_check(net, kw, args, syn0, trial_index, r32):
     return {"reference": "FP64 form (bit-exact with the oracle), same trial and synapse state",
             "max_abs_dt_ms": max(v["max_abs_dt_ms"] for v in c.values()),
             "unpaired_spikes": sum(v["unpaired"] for v in c.values()),
+            "paired_within_one_step": sum(v["paired_within_one_step"] for v in c.values()),
             "spikes": sum(v["spikes_ref"] for v in c.values()), "dt_ms": 0.02, "per_kind": c}
+
+
+def bench_batch(args):
+    """Batched form (configs.c4): all networks advance together in one context."""
+    from unsupervised_growth_amd import configs, opcount
+    from unsupervised_growth_amd.engine import Engine
+    from unsupervised_growth_amd.spikes import compare_results
+
+    t_build = time.perf_counter()
+    nets, keys, kw = configs.c4(n_nets=args.nets)
+    t_build = time.perf_counter() - t_build
+    precision = 0 if args.precision == "fp64" else 1
+    eng = Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
+    eng.upload_batch(nets, keys)
+    steps_per_trial = int(round(kw["trial_ms"] / 0.02)) - 1
+    n_neurons = sum(n.n_ra + n.n_i for n in nets)
+
+    def trials(tr):
+        return [dict(mode=0, training=True, trial_index=tr, first_trial=(tr == 0), mean_injection_time=100.0)
+                for _ in nets]
+
+    trial = 0
+    for _ in range(args.warmup):
+        eng.run_batch(trials(trial))
+        trial += 1
+    syn0 = [eng.download_synapses(b) for b in range(len(nets))]
+    first_timed = trial
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    results = []
+    for _ in range(args.steps):
+        results.append(eng.run_batch(trials(trial)))
+        trial += 1
+    wall = time.perf_counter() - t0
+    kern = {name: eng.kernel_timing(name)[0] for name in ("integrate", "deliver", "sync", "decay")}
+    eng.close()
+
+    # network by network against the unbatched form (same trial, same synapse state)
+    check_nets = list(range(min(args.check_nets, len(nets))))
+    worst_alone = {"max_abs_dt_ms": 0.0, "unpaired_spikes": 0, "spikes": 0, "networks": len(check_nets)}
+    for b in check_nets:
+        with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, seed=keys[b], **kw) as e1:
+            e1.upload(nets[b])
+            e1.upload_synapses(syn0[b])
+            r1 = e1.run_trial(mode=0, training=True, trial_index=first_timed, first_trial=(first_timed == 0),
+                              mean_injection_time=100.0)
+        c = compare_results(r1, results[0][b], window=1.0)
+        worst_alone["max_abs_dt_ms"] = max(worst_alone["max_abs_dt_ms"], max(v["max_abs_dt_ms"] for v in c.values()))
+        worst_alone["unpaired_spikes"] += sum(v["unpaired"] for v in c.values())
+        worst_alone["spikes"] += sum(v["spikes_ref"] for v in c.values())
+    precision_check = None
+    if precision == 1:
+        with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=0, **kw) as e64:
+            e64.upload_batch(nets, keys)
+            for b in range(len(nets)):
+                e64.upload_synapses(syn0[b], b)
+            r64 = e64.run_batch(trials(first_timed))
+        tot = {"max_abs_dt_ms": 0.0, "unpaired_spikes": 0, "paired_within_one_step": 0, "spikes": 0,
+               "networks_exceeding_one_step": 0}
+        for b in range(len(nets)):
+            c = compare_results(r64[b], results[0][b], window=1.0)
+            worst = max(v["max_abs_dt_ms"] for v in c.values())
+            tot["max_abs_dt_ms"] = max(tot["max_abs_dt_ms"], worst)
+            tot["networks_exceeding_one_step"] += int(worst > 0.02 + 1e-9)
+            tot["unpaired_spikes"] += sum(v["unpaired"] for v in c.values())
+            tot["paired_within_one_step"] += sum(v["paired_within_one_step"] for v in c.values())
+            tot["spikes"] += sum(v["spikes_ref"] for v in c.values())
+        precision_check = dict(reference="FP64 batched form (bit-exact with the oracle), same trial", dt_ms=0.02, **tot)
+
+    windows = int(round(kw["trial_ms"] / 0.25))
+    ra_flops = opcount.flops_per_step("ra", args.ra_model)
+    i_flops = opcount.flops_per_step("i", args.i_model)
+    flops_launch = sum(ra_flops * n.n_ra + i_flops * n.n_i for n in nets) * steps_per_trial / windows
+    peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
+    achieved = flops_launch / (kern["integrate"] / 1000.0) / 1e12 if kern["integrate"] > 0 else 0.0
+    line = {
+        "metric": "RK4 neuron-updates/s, batched form (all networks together)",
+        "value": n_neurons * steps_per_trial * args.steps / wall, "unit": "neuron-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "f32",
+        "data": "synthetic networks (distance rules, seeded), counter-based noise streams",
+        "config": {"workload": "batched growth trials: %d networks, %d HVC-RA + %d HVC-I in total, N_RA %d..%d, "
+                               "%g ms trials, RA %s + I DP8, STDP on"
+                               % (len(nets), sum(n.n_ra for n in nets), sum(n.n_i for n in nets),
+                                  min(n.n_ra for n in nets), max(n.n_ra for n in nets), kw["trial_ms"],
+                                  "RK4-3/8" if args.ra_model else "Debraband"),
+                   "config": "c4", "network_build_s": round(t_build, 1)},
+        "roofline": {"bound": "mfma", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None, "flops_per_launch": flops_launch,
+                     "avg_launch_ms": kern["integrate"],
+                     "kernels_ms_per_trial": {k: v * (1 if k == "decay" else windows) for k, v in kern.items()}},
+        "batch_vs_alone": worst_alone,
+    }
+    if precision_check is not None:
+        line["precision_check"] = precision_check
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -82,6 +182,9 @@ def main():
         print("bench.py runs on one GPU: this build measures the single-GPU hot path "
               "(multi-GPU sharding is a design note, not an implemented path)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        bench_batch(args)
+        return
     from unsupervised_growth_amd import configs, opcount
     from unsupervised_growth_amd.engine import Engine
 

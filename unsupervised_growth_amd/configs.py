@@ -9,6 +9,9 @@ C2  2000 RA + 550 I on the unit sphere, Gaussian distance rule (peak 0.3,
     RA->RA candidates, 1000 ms trials, noise seed 7.
 C3  20000 RA + 5500 I on the unit sphere, the same distance rule, 100 RA->RA
     candidates per RA (nearest), delays U(0, 10) ms.
+C4  batched form: 64 independent networks, N_RA = round(exp(U(ln 500, ln 8000))),
+    N_I = round(0.275 N_RA), each built like C3 with delays U(0, 8) ms and seed
+    100 + b (also its noise key), 1000 ms trials.
 Weights follow sample_G: U(0,1) * G_max (NetworkGrowthSimulator.cpp:739-742).
 Each builder returns (network, engine keyword arguments).
 """
@@ -85,6 +88,33 @@ def c2(seed=7, trial_ms=1000.0):
     rr_d = _delays(rng, len(rr_tgt), dl, TIMESTEP)
     net = _finish(rng, n_ra, n_i, ra2i, i2ra, rr_ptr, rr_tgt, rr_d, 10)
     return net, dict(trial_ms=trial_ms, seed=(seed, 0))
+
+
+def sphere_network(rng, n_ra, n_i, delay_max, k_rr=100, n_train=None, spread_max=20.0):
+    """RA and I uniform on the unit sphere, Gaussian distance rule (peak 0.3,
+    width 0.1) for RA->I and I->RA, the k_rr nearest RA as RA->RA candidates,
+    delays U(0, delay_max) ms."""
+    pr, pi = sphere_positions(rng, n_ra), sphere_positions(rng, n_i)
+    dl = ("uniform", 0.0, delay_max)
+    ra2i = _gaussian_csr(rng, pr, pi, 0.3, 0.1, 0.25, dl, TIMESTEP)
+    i2ra = _gaussian_csr(rng, pi, pr, 0.3, 0.1, 0.5, dl, TIMESTEP)
+    rr_ptr, rr_tgt = _nearest_candidates(pr, min(k_rr, n_ra - 1))
+    rr_d = _delays(rng, len(rr_tgt), dl, TIMESTEP)
+    if n_train is None:
+        n_train = max(10, n_ra // 200)
+    return _finish(rng, n_ra, n_i, ra2i, i2ra, rr_ptr, rr_tgt, rr_d, n_train, spread_max=spread_max)
+
+
+def c4(n_nets=64, seed0=100):
+    """The batched workload: (networks, noise keys, engine keyword arguments)."""
+    nets, keys = [], []
+    for b in range(n_nets):
+        rng = np.random.default_rng(seed0 + b)
+        n_ra = int(round(np.exp(rng.uniform(np.log(500.0), np.log(8000.0)))))
+        n_i = int(round(0.275 * n_ra))
+        nets.append(sphere_network(rng, n_ra, n_i, 8.0))
+        keys.append((seed0 + b, 0))
+    return nets, keys, dict(trial_ms=1000.0)
 
 
 def c3(seed=11, k_rr=100):

@@ -59,12 +59,28 @@ __device__ __forceinline__ void i_step(IState& s, real dt, const DecayRow& row)
     }
 }
 
-__device__ __forceinline__ void i_step_lanes(IState& s, int role, int base, real dt, const DecayRow& row)
+__device__ __forceinline__ void i_step_lanes(IState& s, real& xlo, int role, int base, real dt, const DecayRow& row)
 {
-    HVC_GUARDED_STEP(IState, (i_dp8_lanes<false>(bad, s, role, base, dt, row)),
-                     (i_dp8_lanes<true>(bad, s, role, base, dt, row)))
+    HVC_GUARDED_STEP(IState, (i_dp8_lanes<false>(bad, s, xlo, role, base, dt, row)),
+                     (i_dp8_lanes<true>(bad, s, xlo, role, base, dt, row)))
 }
 #undef HVC_GUARDED_STEP
+
+// state variables between HBM (FP64) and registers: the FP32 form keeps a
+// variable as float hi + float lo (compensated accumulation), which the FP64
+// copy in HBM holds exactly
+#ifdef HVC_FP32
+#define HVC_LOAD(st, f, d)                            \
+    {                                                 \
+        double d_ = (d);                              \
+        st.f = (real)d_;                              \
+        st.f##_lo = (real)(d_ - (double)st.f);        \
+    }
+#define HVC_WIDE(st, f) ((double)st.f + (double)st.f##_lo)
+#else
+#define HVC_LOAD(st, f, d) st.f = (d);
+#define HVC_WIDE(st, f) (st.f)
+#endif
 
 // per-network stepper constants in the form's type
 __device__ __forceinline__ RAUniform ra_uniform(const NetDev& nd)
@@ -82,8 +98,12 @@ __global__ void k_trace_ra(NetDev nd, int n, int n_steps, uint32_t trial, const 
     if (k >= n) return;
     const DecayRow* rows = (const DecayRow*)nd.decay;
     const RAUniform u = ra_uniform(nd);
-    RAState s{HR(-79.97619025), HR(-79.97268759), HR(0.01101284), HR(0.0), HR(0.9932845), HR(0.00055429),
-              HR(0.00000261762353), HR(0.01689572), HR(0.0), HR(0.0), HR(0.0), HR(0.0)};
+    // HH2_test::set_to_rest (HH2_test.cpp:196-205)
+    RAState s;
+    HVC_LOAD(s, Vs, -79.97619025) HVC_LOAD(s, Vd, -79.97268759) HVC_LOAD(s, n, 0.01101284)
+    HVC_LOAD(s, l, 0.0) HVC_LOAD(s, h, 0.9932845) HVC_LOAD(s, r, 0.00055429)
+    HVC_LOAD(s, c, 0.00000261762353) HVC_LOAD(s, Ca, 0.01689572)
+    s.Ge_s = 0; s.Gi_s = 0; s.Ge_d = 0; s.Gi_d = 0;
     int fs = 0, fd = 0;
     const real el = (real)EL[k], gca = (real)GCa[k];
     PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamRAPoisson, (uint32_t)k, 0u};
@@ -96,8 +116,9 @@ __global__ void k_trace_ra(NetDev nd, int n, int n_steps, uint32_t trial, const 
     }
     auto dump = [&](int row) {
         double* o = out + ((size_t)row * n + k) * 12;
-        o[0] = s.Vs; o[1] = s.Vd; o[2] = s.n; o[3] = s.l; o[4] = s.h; o[5] = s.r;
-        o[6] = s.c; o[7] = s.Ca; o[8] = s.Ge_s; o[9] = s.Gi_s; o[10] = s.Ge_d; o[11] = s.Gi_d;
+        o[0] = HVC_WIDE(s, Vs); o[1] = HVC_WIDE(s, Vd); o[2] = HVC_WIDE(s, n); o[3] = HVC_WIDE(s, l);
+        o[4] = HVC_WIDE(s, h); o[5] = HVC_WIDE(s, r); o[6] = HVC_WIDE(s, c); o[7] = HVC_WIDE(s, Ca);
+        o[8] = s.Ge_s; o[9] = s.Gi_s; o[10] = s.Ge_d; o[11] = s.Gi_d;
     };
     dump(0);
     for (int t = 1; t <= n_steps; ++t) {
@@ -136,7 +157,10 @@ __global__ void k_trace_i(NetDev nd, int n, int n_steps, int first_trial, uint32
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const DecayRow* rows = (const DecayRow*)nd.decay;
-    IState s{HR(-66.0), HR(0.125), HR(0.0), HR(0.99), HR(0.0), HR(0.0), HR(0.0)};
+    // HHI_buffer::set_to_rest (HHI_buffer.cpp:181-185)
+    IState s;
+    HVC_LOAD(s, V, -66.0) HVC_LOAD(s, n, 0.125) HVC_LOAD(s, m, 0.0) HVC_LOAD(s, h, 0.99) HVC_LOAD(s, w, 0.0)
+    s.Ge = 0; s.Gi = 0;
     int flag = 0;
     PoissonStream ps{nd.seed_lo, nd.seed_hi, trial, kStreamIPoisson, (uint32_t)k, 0u};
     double ne = 0, ni = 0;
@@ -147,7 +171,8 @@ __global__ void k_trace_i(NetDev nd, int n, int n_steps, int first_trial, uint32
     }
     auto dump = [&](int row) {
         double* o = out + ((size_t)row * n + k) * 7;
-        o[0] = s.V; o[1] = s.n; o[2] = s.m; o[3] = s.h; o[4] = s.w; o[5] = s.Ge; o[6] = s.Gi;
+        o[0] = HVC_WIDE(s, V); o[1] = HVC_WIDE(s, n); o[2] = HVC_WIDE(s, m); o[3] = HVC_WIDE(s, h);
+        o[4] = HVC_WIDE(s, w); o[5] = s.Ge; o[6] = s.Gi;
     };
     dump(0);
     for (int t = 1; t <= n_steps; ++t) {
@@ -205,8 +230,9 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, int
     const DecayRow* rows = (const DecayRow*)nd.decay;
     const RAUniform u = ra_uniform(nd);
     RAState s;
-    s.Vs = nd.ra[0][ii]; s.Vd = nd.ra[1][ii]; s.n = nd.ra[2][ii]; s.l = nd.ra[3][ii];
-    s.h = nd.ra[4][ii]; s.r = nd.ra[5][ii]; s.c = nd.ra[6][ii]; s.Ca = nd.ra[7][ii];
+    HVC_LOAD(s, Vs, nd.ra[0][ii]) HVC_LOAD(s, Vd, nd.ra[1][ii]) HVC_LOAD(s, n, nd.ra[2][ii])
+    HVC_LOAD(s, l, nd.ra[3][ii]) HVC_LOAD(s, h, nd.ra[4][ii]) HVC_LOAD(s, r, nd.ra[5][ii])
+    HVC_LOAD(s, c, nd.ra[6][ii]) HVC_LOAD(s, Ca, nd.ra[7][ii])
     s.Ge_s = nd.ra[8][ii]; s.Gi_s = nd.ra[9][ii]; s.Ge_d = nd.ra[10][ii]; s.Gi_d = nd.ra[11][ii];
     int flags = nd.ra_flags[ii];
     int fs = flags & 1, fd = (flags >> 1) & 1;
@@ -280,8 +306,9 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, int
         }
     }
     if (live) {
-        nd.ra[0][i] = s.Vs; nd.ra[1][i] = s.Vd; nd.ra[2][i] = s.n; nd.ra[3][i] = s.l;
-        nd.ra[4][i] = s.h; nd.ra[5][i] = s.r; nd.ra[6][i] = s.c; nd.ra[7][i] = s.Ca;
+        nd.ra[0][i] = HVC_WIDE(s, Vs); nd.ra[1][i] = HVC_WIDE(s, Vd); nd.ra[2][i] = HVC_WIDE(s, n);
+        nd.ra[3][i] = HVC_WIDE(s, l); nd.ra[4][i] = HVC_WIDE(s, h); nd.ra[5][i] = HVC_WIDE(s, r);
+        nd.ra[6][i] = HVC_WIDE(s, c); nd.ra[7][i] = HVC_WIDE(s, Ca);
         nd.ra[8][i] = s.Ge_s; nd.ra[9][i] = s.Gi_s; nd.ra[10][i] = s.Ge_d; nd.ra[11][i] = s.Gi_d;
         nd.ra_flags[i] = fs | (fd << 1);
         nd.soma_mask[i] = smask;
@@ -301,8 +328,10 @@ __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
     const bool live = i < nd.n_i;
     const int ii = live ? i : 0;
     const DecayRow* rows = (const DecayRow*)nd.decay;
-    IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
-             (real)nd.in[5][ii], (real)nd.in[6][ii]};
+    IState s;
+    HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
+    HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     const NoiseId sid = i_noise_id(nd, ii);
     PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
@@ -321,7 +350,8 @@ __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
         wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
     }
     if (live) {
-        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
+        nd.in[0][i] = HVC_WIDE(s, V); nd.in[1][i] = HVC_WIDE(s, n); nd.in[2][i] = HVC_WIDE(s, m);
+        nd.in[3][i] = HVC_WIDE(s, h); nd.in[4][i] = HVC_WIDE(s, w);
         nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
         nd.i_flag[i] = flag;
         nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
@@ -348,12 +378,19 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
     const bool live = (i < nd.n_i) && !spare;
     const int ii = (i < nd.n_i) ? i : 0;
     const DecayRow* rows = (const DecayRow*)nd.decay;
-    IState s{(real)nd.in[0][ii], (real)nd.in[1][ii], (real)nd.in[2][ii], (real)nd.in[3][ii], (real)nd.in[4][ii],
-             (real)nd.in[5][ii], (real)nd.in[6][ii]};
+    IState s;
+    HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
+    HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     const NoiseId sid = i_noise_id(nd, ii);
     PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
     double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    // low part of this lane's own variable (FP32 form; zero in FP64)
+    real xlo = 0;
+#ifdef HVC_FP32
+    xlo = role == 0 ? s.V_lo : (role == 1 ? s.n_lo : (role == 2 ? s.m_lo : (role == 3 ? s.h_lo : s.w_lo)));
+#endif
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
@@ -362,13 +399,21 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
             i_noise_check(ps, time, nd.dt, s.Ge, ne);
             i_noise_check(ps, time, nd.dt, s.Gi, ni);
         }
-        i_step_lanes(s, role, base, (real)nd.dt, rows[t]);
+        i_step_lanes(s, xlo, role, base, (real)nd.dt, rows[t]);
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
         wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
     }
+    if (live) {
+        // each lane writes its own variable, widened with its low part
+        const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
+#ifdef HVC_FP32
+        nd.in[role][i] = (double)xr + (double)xlo;
+#else
+        nd.in[role][i] = xr;
+#endif
+    }
     if (live && role == 0) {
-        nd.in[0][i] = s.V; nd.in[1][i] = s.n; nd.in[2][i] = s.m; nd.in[3][i] = s.h; nd.in[4][i] = s.w;
         nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
         nd.i_flag[i] = flag;
         nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;

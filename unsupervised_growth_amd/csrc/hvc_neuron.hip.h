@@ -44,7 +44,29 @@ struct Guard {
 #define CDV(a, B, invB) ((a) * (invB))
 #define CK(n, k, d) ((k) * (real)((double)(n) / (double)(d)))
 #define DIVR(a, b) ((real)(a) * __builtin_amdgcn_rcpf((real)(b)))
-#define EXPR(x) __expf(x)
+#define EXPR(x) ::hvc::p32::fexp(x)
+// exp(x) = 2^(x log2 e): the product x log2 e is carried as p + e (fma gives
+// the rounding error of x L_hi exactly, x L_lo adds the tail of log2 e), the
+// hardware exp2 evaluates 2^p and 2^e ~ 1 + e ln 2 corrects it; about 2 ulp.
+__device__ __forceinline__ float fexp(float x)
+{
+    const float L_hi = 1.44269502162933349609375f;      // RN(log2 e)
+    const float L_lo = 1.925963033500011079e-8f;        // log2 e - L_hi
+    float p = x * L_hi;
+    float e = __builtin_fmaf(x, L_hi, -p);
+    e = __builtin_fmaf(x, L_lo, e);
+    float r = __builtin_amdgcn_exp2f(p);
+    return __builtin_fmaf(r, e * 0.693147180559945309f, r);
+}
+// state update with compensated (Kahan) summation: st.f + st.f_lo carries the
+// value, so increments far below half an ulp of st.f still accumulate
+#define HVC_ACC(st, f, inc)                  \
+    {                                        \
+        real y_ = (inc) + st.f##_lo;         \
+        real t_ = st.f + y_;                 \
+        st.f##_lo = y_ - (t_ - st.f);        \
+        st.f = t_;                           \
+    }
 #else
 #define HVC_PREC_BEGIN inline namespace p64 {
 HVC_PREC_BEGIN
@@ -61,6 +83,7 @@ struct Guard : DivGuard {
 #define CK(n, k, d) CD((n) * (k), d)
 #define DIVR(a, b) ((a) / (b))
 #define EXPR(x) ::hvc::dexp(x)
+#define HVC_ACC(st, f, inc) st.f = st.f + inc;
 #endif
 
 // offsets c*dt of the stage times used by the steppers, in this order
@@ -83,6 +106,9 @@ struct DecayRow {
 struct RAState {
     real Vs, Vd, n, l, h, r, c, Ca;
     real Ge_s, Gi_s, Ge_d, Gi_d;
+#ifdef HVC_FP32
+    real Vs_lo, Vd_lo, n_lo, l_lo, h_lo, r_lo, c_lo, Ca_lo;
+#endif
 };
 
 // per-network constants derived once on the host with the reference's own expressions
@@ -181,14 +207,14 @@ __device__ __forceinline__ void ra_debraband(Guard& bad, RAState& s, real EL, re
 #undef HVC_H4
 
 #define HVC_FIN(k) (dt * (CD(a1.k, 6) - HR(0.005430430675258792) * a2.k + CD(2 * a3.k, 3) + HR(0.1720970973419255) * a4.k))
-    s.Vs = s.Vs + HVC_FIN(Vs) + CD(u.sq_sig_s * J1s * 100000, 5000);
-    s.Vd = s.Vd + HVC_FIN(Vd) + CD(u.sq_sig_d * J1d * 100000, 10000);
-    s.Ca = s.Ca + HVC_FIN(Ca);
-    s.n = s.n + HVC_FIN(n);
-    s.l = s.l + HVC_FIN(l);
-    s.h = s.h + HVC_FIN(h);
-    s.r = s.r + HVC_FIN(r);
-    s.c = s.c + HVC_FIN(c);
+    HVC_ACC(s, Vs, HVC_FIN(Vs) + CD(u.sq_sig_s * J1s * 100000, 5000))
+    HVC_ACC(s, Vd, HVC_FIN(Vd) + CD(u.sq_sig_d * J1d * 100000, 10000))
+    HVC_ACC(s, Ca, HVC_FIN(Ca))
+    HVC_ACC(s, n, HVC_FIN(n))
+    HVC_ACC(s, l, HVC_FIN(l))
+    HVC_ACC(s, h, HVC_FIN(h))
+    HVC_ACC(s, r, HVC_FIN(r))
+    HVC_ACC(s, c, HVC_FIN(c))
 #undef HVC_FIN
     s.Ge_d = s.Ge_d * E_dt;
     s.Ge_s = s.Ge_s * E_dt;
@@ -219,13 +245,13 @@ __device__ __forceinline__ void ra_runge4(Guard& bad, RAState& s, real EL, real 
     s.Gi_d = s.Gi_d * E_dt;
     s.Gi_s = s.Gi_s * E_dt;
 #define HVC_F(k) (dt * (k1.k + 3 * k2.k + 3 * k3.k + k4.k) * HR(0.125))
-    s.Vs = s.Vs + HVC_F(Vs);
-    s.n = s.n + HVC_F(n);
-    s.h = s.h + HVC_F(h);
-    s.Vd = s.Vd + HVC_F(Vd);
-    s.r = s.r + HVC_F(r);
-    s.c = s.c + HVC_F(c);
-    s.Ca = s.Ca + HVC_F(Ca);
+    HVC_ACC(s, Vs, HVC_F(Vs))
+    HVC_ACC(s, n, HVC_F(n))
+    HVC_ACC(s, h, HVC_F(h))
+    HVC_ACC(s, Vd, HVC_F(Vd))
+    HVC_ACC(s, r, HVC_F(r))
+    HVC_ACC(s, c, HVC_F(c))
+    HVC_ACC(s, Ca, HVC_F(Ca))
 #undef HVC_F
 }
 
@@ -251,6 +277,9 @@ __device__ __forceinline__ int ra_state_check(real Vs, real Vd, int& flag_soma, 
 // ---------------- HVC-I ----------------
 struct IState {
     real V, n, m, h, w, Ge, Gi;
+#ifdef HVC_FP32
+    real V_lo, n_lo, m_lo, h_lo, w_lo;
+#endif
 };
 
 constexpr real kI_gL = HR(0.1), kI_El = -HR(65.0), kI_gNa = HR(100.0), kI_Ena = HR(55.0), kI_gKdr = HR(20.0);
@@ -341,11 +370,11 @@ __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Deca
 #define CF(x) (CK(13451932, k1.x, 455176623) - CK(808719846, k6.x, 976000145) + CK(1757004468, k7.x, 5645159321) + \
                CK(656045339, k8.x, 265891186) - CK(3867574721, k9.x, 1518517206) + CK(465885868, k10.x, 322736535) + \
                CK(53011238, k11.x, 667516719) + CK(2, k12.x, 45))
-    s.V = s.V + dt * CF(V);
-    s.n = s.n + dt * CF(n);
-    s.m = s.m + dt * CF(m);
-    s.h = s.h + dt * CF(h);
-    s.w = s.w + dt * CF(w);
+    HVC_ACC(s, V, dt * CF(V))
+    HVC_ACC(s, n, dt * CF(n))
+    HVC_ACC(s, m, dt * CF(m))
+    HVC_ACC(s, h, dt * CF(h))
+    HVC_ACC(s, w, dt * CF(w))
     s.Ge = s.Ge * row.e2[kOffDt];
     s.Gi = s.Gi * row.e5[kOffDt];
 #undef C5
@@ -375,11 +404,11 @@ __device__ __forceinline__ void i_runge4(Guard& bad, IState& s, real dt, const D
                   row.e2[kOffDt], row.e5[kOffDt]);
 #undef S4
 #define F(x) (dt * (k1.x + 3 * k2.x + 3 * k3.x + k4.x) * HR(0.125))
-    s.V = s.V + F(V);
-    s.n = s.n + F(n);
-    s.m = s.m + F(m);
-    s.h = s.h + F(h);
-    s.w = s.w + F(w);
+    HVC_ACC(s, V, F(V))
+    HVC_ACC(s, n, F(n))
+    HVC_ACC(s, m, F(m))
+    HVC_ACC(s, h, F(h))
+    HVC_ACC(s, w, F(w))
 #undef F
     s.Ge = s.Ge * row.e2[kOffDt];
     s.Gi = s.Gi * row.e5[kOffDt];
@@ -513,7 +542,8 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
 
 // HHI_buffer::DP8_step with five lanes per neuron; s is replicated in the group
 template <bool EXACT>
-__device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, int role, int base, real dt, const DecayRow& row)
+__device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, int role, int base, real dt,
+                                            const DecayRow& row)
 {
     const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
@@ -562,9 +592,15 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, int role, int
                    CK(4093664535, k9, 808688257) + CK(3962137247, k10, 1805957418) + CK(65686358, k11, 487910083));
     HVC_BCAST5(x)
     real k12 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDt], row.e5[kOffDt]);
-    x = x0 + dt * (CK(13451932, k1, 455176623) - CK(808719846, k6, 976000145) + CK(1757004468, k7, 5645159321) +
-                   CK(656045339, k8, 265891186) - CK(3867574721, k9, 1518517206) + CK(465885868, k10, 322736535) +
-                   CK(53011238, k11, 667516719) + CK(2, k12, 45));
+    {
+        struct { real x, x_lo; } acc{x0, xlo};
+        HVC_ACC(acc, x, dt * (CK(13451932, k1, 455176623) - CK(808719846, k6, 976000145) +
+                              CK(1757004468, k7, 5645159321) + CK(656045339, k8, 265891186) -
+                              CK(3867574721, k9, 1518517206) + CK(465885868, k10, 322736535) +
+                              CK(53011238, k11, 667516719) + CK(2, k12, 45)))
+        x = acc.x;
+        xlo = acc.x_lo;
+    }
     HVC_BCAST5(x)
     s.V = v; s.n = n1; s.m = m1; s.h = h1; s.w = w1;
     s.Ge = s.Ge * row.e2[kOffDt];

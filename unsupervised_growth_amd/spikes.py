@@ -27,13 +27,15 @@ def _pair(a, b, window):
     return dts
 
 
-def compare_trains(ref, test, window=1.0):
-    """ref, test: lists (one entry per neuron) of spike-time arrays in ms."""
+def compare_trains(ref, test, window=1.0, step=0.02):
+    """ref, test: lists (one entry per neuron) of spike-time arrays in ms;
+    step: the time step, for the count of paired spikes within one step."""
     if len(ref) != len(test):
         raise ValueError("trains cover different neuron counts: %d vs %d" % (len(ref), len(test)))
     n_ref = n_test = paired = 0
     worst = 0.0
     neurons_differ = 0
+    within = 0
     for a, b in zip(ref, test):
         a = np.asarray(a, float)
         b = np.asarray(b, float)
@@ -42,7 +44,9 @@ def compare_trains(ref, test, window=1.0):
         dts = _pair(a, b, window)
         paired += len(dts)
         if dts:
-            worst = max(worst, float(np.max(np.abs(dts))))
+            ad = np.abs(dts)
+            worst = max(worst, float(np.max(ad)))
+            within += int(np.count_nonzero(ad <= step + 1e-9))
         if len(dts) != len(a) or len(dts) != len(b):
             neurons_differ += 1
     return {
@@ -52,10 +56,11 @@ def compare_trains(ref, test, window=1.0):
         "unpaired": (n_ref - paired) + (n_test - paired),
         "mismatch_frac": ((n_ref - paired) + (n_test - paired)) / max(1, n_ref + n_test),
         "max_abs_dt_ms": worst,
+        "paired_within_one_step": within,
         "neurons_differ": neurons_differ,
     }
 
 
-def compare_results(ref, test, window=1.0):
+def compare_results(ref, test, window=1.0, step=0.02):
     """Per kind (soma, dend, interneuron) comparison of two TrialResults."""
-    return {k: compare_trains(getattr(ref, k), getattr(test, k), window) for k in ("soma", "dend", "interneuron")}
+    return {k: compare_trains(getattr(ref, k), getattr(test, k), window, step) for k in ("soma", "dend", "interneuron")}
