@@ -148,7 +148,7 @@ hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, const double* 
 hvc_status hvc_upload_batch(hvc_ctx* ctx, int n_nets, const hvc_network* nets, const uint32_t* keys);
 
 /* One trial of every network: trials[b] and outs[b] (outs may be NULL) belong to
- * network b; all trials must have the same mode.  Returns HVC_E_NUMERIC when
+ * network b; each network runs its own mode (growth or chain test).  Returns HVC_E_NUMERIC when
  * any network fails check_bad_values (outs[b].bad_values tells which). */
 hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs);
 
@@ -157,6 +157,38 @@ hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* o
 hvc_status hvc_batch_spikes(hvc_ctx* ctx, int net, int kind, int* counts, double* times, int cap, int* total);
 hvc_status hvc_batch_download_synapses(hvc_ctx* ctx, int net, hvc_synapses* syn);
 hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* syn);
+
+/* Serving pool: a context whose networks join and leave between trials.  The
+ * resident networks are packed side by side in device memory (a release moves
+ * the networks behind it down, on the device, with their ids and edge
+ * indices); every hvc_pool_run advances all of them by one trial in one launch
+ * per kernel and coupling window, each with its own trial (growth or chain
+ * test, trial index, kick time).  Network state stays on the device from
+ * admission to release.  Capacities fix the pool's device footprint; a network
+ * that does not fit returns HVC_E_CAPACITY from hvc_pool_admit.  Handles are
+ * stable across admissions and releases.  No counterpart in the reference. */
+typedef struct {
+    int max_nets;                   /* resident networks */
+    int max_ra, max_i;              /* neurons of all resident networks */
+    long long max_rr;               /* RA->RA candidates */
+    long long max_ra2i, max_i2ra;   /* fixed-weight connections */
+    int max_in_degree;              /* per target and connection type */
+    int spikes_per_neuron;          /* spike-log capacity per neuron and trial (0: 256) */
+} hvc_pool_caps;
+
+/* Device bytes one network of the given sizes occupies in a pool of these
+ * capacities (its HBM footprint), and the bytes of the whole pool. */
+hvc_status hvc_pool_footprint(const hvc_params* params, const hvc_pool_caps* caps, int n_ra, int n_i,
+                              long long n_rr, long long n_ra2i, long long n_i2ra, size_t* network_bytes,
+                              size_t* pool_bytes);
+hvc_status hvc_pool_create(const hvc_params* params, int device, const hvc_pool_caps* caps, hvc_ctx** out);
+hvc_status hvc_pool_admit(hvc_ctx* ctx, const hvc_network* net, uint32_t key_lo, uint32_t key_hi, int* handle);
+hvc_status hvc_pool_release(hvc_ctx* ctx, int handle);
+/* One trial of every resident network: n must equal the number of resident
+ * networks; trials[k] and outs[k] (outs may be NULL) belong to handles[k]. */
+hvc_status hvc_pool_run(hvc_ctx* ctx, int n, const int* handles, const hvc_trial* trials, hvc_trial_out* outs);
+hvc_status hvc_pool_spikes(hvc_ctx* ctx, int handle, int kind, int* counts, double* times, int cap, int* total);
+hvc_status hvc_pool_download_synapses(hvc_ctx* ctx, int handle, hvc_synapses* syn);
 
 /* Single-neuron traces on the device: the neuron seam of the reference
  * (HH2_test::Debraband_step_no_target_update HH2_test.cpp:535-545,

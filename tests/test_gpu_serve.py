@@ -1,0 +1,73 @@
+"""Serving pool: networks join and leave a resident batch between trials.
+
+Requests of different sizes arrive at different rounds, wait for room (the pool
+holds at most three networks here), run interleaved with each other, and leave
+when done -- every release moves the networks behind it down in device memory.
+Each request's spikes (and, for a growth request, its final synapse state) must
+equal those of the same network and trials run alone, bit for bit, in FP64 and
+in FP32.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from unsupervised_growth_amd import configs
+    from unsupervised_growth_amd.network import random_network
+    from unsupervised_growth_amd.serve import Request
+    rng = np.random.default_rng(3)
+    nets = [configs.grown_chain_network(np.random.default_rng(40 + k), n, max(4, n // 4), group=10, fan=5)
+            for k, n in enumerate((60, 100, 40, 80))]
+    grow = random_network(50, 12, n_train=6, p_ra2i=0.3, p_i2ra=0.3, gie_max=1.0, gei_max=0.5, spread_max=20.0,
+                          seed=21, rr=10)
+    spec = [(0, nets[0], 3, 1), (0, nets[1], 5, 1), (1, nets[2], 2, 1), (2, grow, 4, 0), (3, nets[3], 3, 1)]
+    reqs = []
+    for k, (arr, net, ntr, mode) in enumerate(spec):
+        reqs.append(Request(rid=k, arrival=arr, net=net, key=(900 + k, 5), onsets=100.0 + rng.normal(0, 1, ntr),
+                            mode=mode, first_trial_index=10 * k))
+    return reqs
+
+
+def _alone(req, precision, trial_ms):
+    from unsupervised_growth_amd.engine import Engine
+    with Engine(precision=precision, trial_ms=trial_ms, seed=req.key) as e:
+        e.upload(req.net)
+        res = [e.run_trial(**req.trial_kw(k)) for k in range(req.n_trials)]
+        syn = e.download_synapses() if req.mode == 0 else None
+    return res, syn
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_served_requests_equal_alone(precision):
+    from unsupervised_growth_amd.engine import Pool, pool_caps
+    from unsupervised_growth_amd.serve import Server
+    trial_ms = 200.0
+    reqs = _requests()
+    caps = pool_caps(max_nets=3, max_ra=400, max_i=120, max_rr=4000, max_ra2i=20000, max_i2ra=20000,
+                     max_in_degree=200, spikes_per_neuron=128)
+    log = []
+    with Pool(caps, precision=precision, trial_ms=trial_ms) as pool:
+        stats = Server(pool).run(reqs, realtime=False, log=log.append)
+    print("precision=%d %s" % (precision, stats), flush=True)
+    for line in log:
+        print(line)
+    assert stats["rounds"] >= 7
+    assert max(int(l.split()[2]) for l in log) == 3       # the pool filled up
+    assert any(int(l.split()[4]) > 0 for l in log)        # and requests had to wait
+    spikes = 0
+    for r in reqs:
+        alone, syn = _alone(r, precision, trial_ms)
+        assert len(r.results) == len(alone)
+        for k, (a, b) in enumerate(zip(alone, r.results)):
+            for kind in ("soma", "dend", "interneuron"):
+                x, y = getattr(a, kind), getattr(b, kind)
+                assert len(x) == len(y)
+                for u, v in zip(x, y):
+                    np.testing.assert_array_equal(u, v, err_msg="request %d trial %d %s" % (r.rid, k, kind))
+            spikes += sum(map(len, a.soma))
+        if syn is not None:
+            for f in ("w", "act_cnt", "act_ids", "sup_cnt", "sup_ids", "remodeled"):
+                np.testing.assert_array_equal(getattr(r.synapses, f), getattr(syn, f), err_msg=f)
+    assert spikes > 0

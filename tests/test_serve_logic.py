@@ -1,0 +1,78 @@
+"""Host logic of the serving loop with a stand-in pool (no device): first come,
+first served admission by capacity, rounds over every resident request,
+release when complete, latency statistics."""
+import numpy as np
+
+from unsupervised_growth_amd.serve import Request, Server
+
+
+class _Params:
+    trial_ms = 10.0
+    dt = 0.02
+
+
+class FakePool:
+    """Capacity in neurons; run() records which handles ran together."""
+
+    def __init__(self, capacity):
+        self.capacity = capacity
+        self.used = 0
+        self.next = 1
+        self.resident = {}
+        self.params = _Params()
+        self.rounds = []
+
+    def admit(self, net, key):
+        n = net.n_ra + net.n_i
+        if self.used + n > self.capacity:
+            return None
+        h = self.next
+        self.next += 1
+        self.resident[h] = n
+        self.used += n
+        return h
+
+    def release(self, h):
+        self.used -= self.resident.pop(h)
+
+    def run(self, trials):
+        self.rounds.append(sorted(trials))
+        return {h: ("result", h, kw["trial_index"]) for h, kw in trials.items()}
+
+    def download_synapses(self, h):
+        return ("syn", h)
+
+
+class _Net:
+    def __init__(self, n_ra, n_i):
+        self.n_ra, self.n_i = n_ra, n_i
+
+
+def _req(rid, arrival, n, trials, mode=1):
+    return Request(rid=rid, arrival=arrival, net=_Net(n, 0), key=(rid, 0), onsets=np.full(trials, 100.0),
+                   mode=mode, first_trial_index=100 * rid)
+
+
+def test_fifo_admission_and_release():
+    pool = FakePool(capacity=10)
+    reqs = [_req(0, 0, 6, 2), _req(1, 0, 6, 1), _req(2, 0, 3, 1), _req(3, 5, 2, 1, mode=0)]
+    stats = Server(pool).run(reqs, realtime=False)
+    # request 1 does not fit beside 0, and request 2 waits behind it (first come, first served)
+    assert pool.rounds[0] == [1] and pool.rounds[1] == [1]
+    assert reqs[1].admitted == 2 and reqs[2].admitted == 2
+    assert pool.rounds[2] == [2, 3]
+    # every request ran its trials with consecutive trial indices
+    for r in reqs:
+        assert [x[2] for x in r.results] == [r.first_trial_index + k for k in range(r.n_trials)]
+    assert reqs[3].synapses == ("syn", reqs[3].handle)
+    assert pool.used == 0 and not pool.resident
+    assert stats["requests"] == 4 and stats["rounds"] == len(pool.rounds) + 2
+
+
+def test_latency_statistics():
+    pool = FakePool(capacity=100)
+    reqs = [_req(k, k, 5, 3) for k in range(5)]
+    stats = Server(pool).run(reqs, realtime=False)
+    lat = [r.finished - r.arrival for r in reqs]
+    assert stats["latency_p50"] == np.percentile(lat, 50)
+    assert stats["latency_max"] == max(lat)

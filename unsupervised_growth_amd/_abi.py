@@ -22,8 +22,9 @@ STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: 
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
            "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
            "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_upload_batch", "hvc_run_batch",
-           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses", "hvc_destroy",
-           "hvc_last_error"]
+           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses", "hvc_pool_footprint",
+           "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
+           "hvc_pool_download_synapses", "hvc_destroy", "hvc_last_error"]
 
 
 class EngineNotBuilt(RuntimeError):
@@ -73,6 +74,12 @@ class TrialOut(ctypes.Structure):
                 ("ra_state", c_double_p), ("i_state", c_double_p)]
 
 
+class PoolCaps(ctypes.Structure):
+    _fields_ = [("max_nets", ctypes.c_int), ("max_ra", ctypes.c_int), ("max_i", ctypes.c_int),
+                ("max_rr", ctypes.c_longlong), ("max_ra2i", ctypes.c_longlong), ("max_i2ra", ctypes.c_longlong),
+                ("max_in_degree", ctypes.c_int), ("spikes_per_neuron", ctypes.c_int)]
+
+
 class Synapses(ctypes.Structure):
     _fields_ = [("w", c_double_p), ("act_cnt", c_int_p), ("act_ids", c_int_p), ("sup_cnt", c_int_p),
                 ("sup_ids", c_int_p), ("remodeled", c_int_p)]
@@ -110,6 +117,14 @@ def load(path=LIB_PATH):
     lib.hvc_batch_spikes.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
     lib.hvc_batch_download_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
     lib.hvc_batch_upload_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
+    ll, sz = ctypes.c_longlong, ctypes.c_size_t
+    lib.hvc_pool_footprint.argtypes = [P(Params), P(PoolCaps), ctypes.c_int, ctypes.c_int, ll, ll, ll, P(sz), P(sz)]
+    lib.hvc_pool_create.argtypes = [P(Params), ctypes.c_int, P(PoolCaps), P(vp)]
+    lib.hvc_pool_admit.argtypes = [vp, P(NetworkDesc), ctypes.c_uint32, ctypes.c_uint32, c_int_p]
+    lib.hvc_pool_release.argtypes = [vp, ctypes.c_int]
+    lib.hvc_pool_run.argtypes = [vp, ctypes.c_int, c_int_p, P(Trial), P(TrialOut)]
+    lib.hvc_pool_spikes.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
+    lib.hvc_pool_download_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
     lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
     lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
     lib.hvc_eval_math.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, c_double_p]

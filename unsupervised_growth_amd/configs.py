@@ -12,6 +12,12 @@ C3  20000 RA + 5500 I on the unit sphere, the same distance rule, 100 RA->RA
 C4  batched form: 64 independent networks, N_RA = round(exp(U(ln 500, ln 8000))),
     N_I = round(0.275 N_RA), each built like C3 with delays U(0, 8) ms and seed
     100 + b (also its noise key), 1000 ms trials.
+C5  served form: chain-test requests (50 trials, kick onset 100 ms + N(0, 1) ms
+    per trial) on grown networks of 1000 (25 %), 4000 (50 %) and 20000 (25 %)
+    HVC-RA, N_I = 0.275 N_RA, Poisson arrivals at 20 requests/s.  A grown
+    network is synthetic: its active RA->RA synapses form a feed-forward chain
+    of groups (each neuron drives `fan` neurons of the next group), the state
+    the growth rule converges to; the training neurons are the first group.
 Weights follow sample_G: U(0,1) * G_max (NetworkGrowthSimulator.cpp:739-742).
 Each builder returns (network, engine keyword arguments).
 """
@@ -128,6 +134,61 @@ def c3(seed=11, k_rr=100):
     rr_d = _delays(rng, len(rr_tgt), dl, TIMESTEP)
     net = _finish(rng, n_ra, n_i, ra2i, i2ra, rr_ptr, rr_tgt, rr_d, 100, spread_max=20.0)
     return net, dict(trial_ms=500.0, seed=(seed, 0))
+
+
+def grown_chain_network(rng, n_ra, n_i, group=20, fan=10, w_chain=3000.0, delay=(1.0, 2.0), spread_max=1.0):
+    """A network whose active RA->RA synapses form a chain of groups of `group`
+    neurons: each neuron has `fan` candidates in the next group, all active with
+    weight w_chain (raiseExcWeight adds w / Ad, Ad = 10000, HH2_test.h:89), delays
+    U(delay).  RA->I and I->RA follow the sphere distance rule of C3."""
+    pr, pi = sphere_positions(rng, n_ra), sphere_positions(rng, n_i)
+    dl = ("uniform", 0.0, 10.0)
+    ra2i = _gaussian_csr(rng, pr, pi, 0.3, 0.1, 0.25, dl, TIMESTEP)
+    i2ra = _gaussian_csr(rng, pi, pr, 0.3, 0.1, 0.5, dl, TIMESTEP)
+    order = rng.permutation(n_ra)
+    n_groups = n_ra // group
+    rows = [np.zeros(0, np.int32) for _ in range(n_ra)]
+    for g in range(n_groups - 1):
+        nxt = order[(g + 1) * group:(g + 2) * group]
+        for i in order[g * group:(g + 1) * group]:
+            rows[i] = np.sort(rng.choice(nxt, size=min(fan, len(nxt)), replace=False)).astype(np.int32)
+    rr_ptr = _csr(rows, n_ra)
+    rr_tgt = np.concatenate(rows).astype(np.int32)
+    E = len(rr_tgt)
+    rr_d = rng.uniform(delay[0], delay[1], E)
+    net = _finish(rng, n_ra, n_i, ra2i, i2ra, rr_ptr, rr_tgt, rr_d, group, spread_max=spread_max)
+    net.train_ids = np.sort(order[:group]).astype(np.int32)
+    net.rr_w = np.full(E, w_chain)
+    net.act_cnt = np.diff(rr_ptr).astype(np.int32)
+    net.act_ids = rr_tgt.copy()
+    return net
+
+
+def c5_networks(seed=500, variants=2, sizes=(1000, 4000, 20000)):
+    """Grown chain networks of the served workload: `variants` per size."""
+    out = {}
+    for k, n_ra in enumerate(sizes):
+        out[n_ra] = [grown_chain_network(np.random.default_rng(seed + 10 * k + v), n_ra, int(round(0.275 * n_ra)))
+                     for v in range(variants)]
+    return out
+
+
+def c5_requests(nets_by_size, n_requests=24, rate=20.0, trials=50, seed=7,
+                mix=((1000, 0.25), (4000, 0.5), (20000, 0.25))):
+    """(arrival s, network, key, kick onsets) of the served workload."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    sizes = [m[0] for m in mix]
+    probs = np.array([m[1] for m in mix])
+    out = []
+    for r in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        n_ra = sizes[rng.choice(len(sizes), p=probs / probs.sum())]
+        var = nets_by_size[n_ra]
+        net = var[rng.integers(len(var))]
+        onsets = 100.0 + rng.normal(0.0, 1.0, trials)
+        out.append((t, net, (seed * 1000 + r, 1), onsets))
+    return out
 
 
 CONFIGS = {"c1": c1, "c2": c2, "c3": c3}

@@ -27,16 +27,17 @@ void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint3
 void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
 void launch_reset(hipStream_t st, const NetDev& nd, int E);
-void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1);
 void launch_trace_ra_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
                          const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
                          uint8_t* fired);
 void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                         const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
-void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
-void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth);
+void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1);
+void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1);
 void launch_sync(hipStream_t st, const NetDev& nd);
 void launch_sort_posts(hipStream_t st, const NetDev& nd);
+void launch_add_i32(hipStream_t st, int* a, long long n, int delta);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
 void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out);
@@ -55,6 +56,16 @@ struct DevBuf {
 };
 
 }  // namespace
+
+// one resident network of a serving pool, in its own ids
+struct PoolNet {
+    int handle = 0;
+    int n_ra = 0, n_i = 0;
+    int E = 0, A = 0, Bn = 0;   // candidates, RA->I and I->RA connections
+    uint32_t key[2] = {0, 0};
+    std::vector<int> train_ids;
+    std::vector<double> spread;
+};
 
 struct hvc_ctx {
     hvc_params prm;
@@ -86,8 +97,32 @@ struct hvc_ctx {
     std::vector<uint32_t> keys;
     uint32_t* d_net_trial = nullptr;
     int* d_net_first = nullptr;
+    int* d_net_growth = nullptr;
+    int* d_ra_net = nullptr;
+    int* d_i_net = nullptr;
+    int* d_ra_off = nullptr;
+    int* d_i_off = nullptr;
+    uint32_t* d_key = nullptr;
+    double* d_EL = nullptr;
+    double* d_GCa = nullptr;
+    int* d_rr_ptr = nullptr;
+    int* d_rr_tgt = nullptr;
+    double* d_rr_d = nullptr;
+    int* d_perm = nullptr;
+    int *d_a_ptr = nullptr, *d_a_tgt = nullptr, *d_b_ptr = nullptr, *d_b_tgt = nullptr;
+    double *d_a_w = nullptr, *d_a_d = nullptr, *d_b_w = nullptr, *d_b_d = nullptr;
+    // serving pool
+    bool pool = false;
+    std::vector<PoolNet> pnets;
+    int next_handle = 1;
+    long long a_tot = 0, b_tot = 0;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    // capacities of the device arrays
+    int caps_ra = 0, caps_i = 0, caps_nets = 0, caps_indeg = 0;
+    long long caps_rr = 0, caps_a2i = 0, caps_i2ra = 0;
     std::vector<uint32_t> h_net_trial;
-    std::vector<int> h_net_first;
+    std::vector<int> h_net_first, h_net_growth;
 
     bool trial_done = false;
     int log_counts[3] = {0, 0, 0};
@@ -245,6 +280,7 @@ extern "C" void hvc_destroy(hvc_ctx* ctx)
         hipStreamSynchronize(ctx->stream);
         free_bufs(ctx);
         for (auto e : ctx->ev_pool) hipEventDestroy(e);
+        if (ctx->d_stage) hipFree(ctx->d_stage);
         hipStreamDestroy(ctx->stream);
     }
     delete ctx;
@@ -439,6 +475,188 @@ static hvc_status check_network(hvc_ctx* ctx, const hvc_network* net, const char
     return HVC_OK;
 }
 
+// per row: candidates sorted by (delay, target), as edge indices + e_base.  Two
+// different delays whose sums with a spike time could round to the same
+// delivery time would make that order differ from the reference's (time,
+// target) order; reject them (|d1 - d0| >= 4 ulp of the largest clock value
+// keeps the sums distinct).
+static hvc_status rr_order(hvc_ctx* ctx, const int* ptr, const int* tgt, const double* d, int n, int e_base,
+                           std::vector<int>& perm)
+{
+    perm.resize(ptr[n]);
+    for (int i = 0; i < n; ++i) {
+        int b = ptr[i], e = ptr[i + 1];
+        for (int k = b; k < e; ++k) perm[k] = k;
+        std::sort(perm.begin() + b, perm.begin() + e, [&](int x, int y) {
+            if (d[x] != d[y]) return d[x] < d[y];
+            return tgt[x] < tgt[y];
+        });
+        for (int k = b + 1; k < e; ++k) {
+            double d0 = d[perm[k - 1]], d1 = d[perm[k]];
+            double tmax = ctx->prm.trial_ms + d1;
+            if (d0 != d1 && d1 - d0 < 4 * DBL_EPSILON * tmax)
+                return fail(ctx, HVC_E_ARG, "two RA->RA delays of one row differ by less than 4 ulp of the trial clock");
+        }
+    }
+    for (auto& x : perm) x += e_base;
+    return HVC_OK;
+}
+
+// largest in-degree of a CSR connection set (self-connections skipped if asked)
+static int max_indeg(const int* ptr, const int* tgt, int n_pre, int n_post, int skip_self)
+{
+    std::vector<int> deg(std::max(n_post, 1), 0);
+    for (int i = 0; i < n_pre; ++i)
+        for (int e = ptr[i]; e < ptr[i + 1]; ++e)
+            if (!(skip_self && tgt[e] == i)) deg[tgt[e]]++;
+    return *std::max_element(deg.begin(), deg.end());
+}
+
+// device capacity of a context: neurons, connections, networks, inbox slots
+struct Caps {
+    int nets = 1, ra = 0, i = 0;
+    long long rr = 0, a2i = 0, i2ra = 0;
+    int cap_gi_ra = 8, cap_ge_ra = 8, cap_ge_i = 8;
+    long long log_cap = 0;
+};
+
+// allocates every array of ctx->nd at capacity c (contents undefined)
+static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
+{
+    hvc_status st = HVC_OK;
+    NetDev& nd = ctx->nd;
+    const int nra = std::max(c.ra, 1), ni = std::max(c.i, 1), n_nets = std::max(c.nets, 1);
+    const size_t E = (size_t)std::max<long long>(c.rr, 1);
+    const size_t A = (size_t)std::max<long long>(c.a2i, 1), Bn = (size_t)std::max<long long>(c.i2ra, 1);
+    const int NssA = std::max(ctx->prm.syn.Nss, 1);
+    ctx->caps_ra = nra; ctx->caps_i = ni; ctx->caps_nets = n_nets;
+    ctx->caps_rr = (long long)E; ctx->caps_a2i = (long long)A; ctx->caps_i2ra = (long long)Bn;
+    ctx->caps_indeg = std::min(c.cap_gi_ra, std::min(c.cap_ge_ra, c.cap_ge_i)) - 8;
+    ctx->d_ra_net = dalloc<int>(ctx, nra, &st);
+    ctx->d_i_net = dalloc<int>(ctx, ni, &st);
+    ctx->d_ra_off = dalloc<int>(ctx, n_nets + 1, &st);
+    ctx->d_i_off = dalloc<int>(ctx, n_nets + 1, &st);
+    ctx->d_key = dalloc<uint32_t>(ctx, 2 * (size_t)n_nets, &st);
+    ctx->d_net_trial = dalloc<uint32_t>(ctx, n_nets, &st);
+    ctx->d_net_first = dalloc<int>(ctx, n_nets, &st);
+    ctx->d_net_growth = dalloc<int>(ctx, n_nets, &st);
+    nd.ra_net = ctx->d_ra_net; nd.i_net = ctx->d_i_net; nd.net_ra_off = ctx->d_ra_off; nd.net_i_off = ctx->d_i_off;
+    nd.net_key = ctx->d_key; nd.net_trial = ctx->d_net_trial; nd.net_first = ctx->d_net_first;
+    nd.net_growth = ctx->d_net_growth;
+    for (int k = 0; k < 12; ++k) nd.ra[k] = dalloc<double>(ctx, nra, &st);
+    nd.ra_flags = dalloc<int>(ctx, nra, &st);
+    nd.ra_noise = dalloc<double>(ctx, 4 * (size_t)nra, &st);
+    nd.ra_draw = dalloc<uint32_t>(ctx, nra, &st);
+    ctx->d_EL = dalloc<double>(ctx, nra, &st);
+    ctx->d_GCa = dalloc<double>(ctx, nra, &st);
+    nd.EL = ctx->d_EL;
+    nd.GCa = ctx->d_GCa;
+    ctx->d_kick = dalloc<int>(ctx, nra, &st);
+    nd.kick_step = ctx->d_kick;
+    for (int k = 0; k < 7; ++k) nd.in[k] = dalloc<double>(ctx, ni, &st);
+    nd.i_flag = dalloc<int>(ctx, ni, &st);
+    nd.i_noise = dalloc<double>(ctx, 2 * (size_t)ni, &st);
+    nd.i_draw = dalloc<uint32_t>(ctx, ni, &st);
+    nd.soma_mask = dalloc<uint64_t>(ctx, nra, &st);
+    nd.dend_mask = dalloc<uint64_t>(ctx, nra, &st);
+    nd.i_mask = dalloc<uint64_t>(ctx, ni, &st);
+    long long cap = ctx->prm.spike_capacity > 0 ? ctx->prm.spike_capacity
+                                                : std::max<long long>(1 << 16, 256 * ((long long)nra + ni));
+    if (c.log_cap > 0) cap = c.log_cap;
+    if (cap > (1ll << 30)) cap = 1ll << 30;
+    nd.log_cap = (int)cap;
+    nd.soma_log = dalloc<int2>(ctx, cap, &st);
+    nd.dend_log = dalloc<int2>(ctx, cap, &st);
+    nd.i_log = dalloc<int2>(ctx, cap, &st);
+    nd.log_cnt = dalloc<int>(ctx, 4, &st);
+    nd.post_ring = dalloc<double>(ctx, (size_t)nra * kPostRing, &st);
+    nd.post_cnt = dalloc<int>(ctx, nra, &st);
+    nd.post_pend = dalloc<int>(ctx, nra, &st);
+    nd.post_list = dalloc<uint64_t>(ctx, kPostListCap, &st);
+    nd.post_list_cnt = dalloc<int>(ctx, 1, &st);
+    ctx->d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
+    ctx->d_rr_tgt = dalloc<int>(ctx, E, &st);
+    ctx->d_rr_d = dalloc<double>(ctx, E, &st);
+    ctx->d_perm = dalloc<int>(ctx, E, &st);
+    nd.rr_ptr = ctx->d_rr_ptr; nd.rr_tgt = ctx->d_rr_tgt; nd.rr_d = ctx->d_rr_d; nd.rr_perm = ctx->d_perm;
+    nd.w = dalloc<double>(ctx, E, &st);
+    nd.act = dalloc<uint8_t>(ctx, E, &st);
+    nd.sup = dalloc<uint8_t>(ctx, E, &st);
+    nd.act_list = dalloc<int>(ctx, E, &st);
+    nd.act_cnt = dalloc<int>(ctx, nra, &st);
+    nd.sup_list = dalloc<int>(ctx, (size_t)nra * NssA, &st);
+    nd.sup_cnt = dalloc<int>(ctx, nra, &st);
+    nd.remodeled = dalloc<int>(ctx, nra, &st);
+    nd.pre_ring = dalloc<double>(ctx, E * kPreRing, &st);
+    nd.pre_cnt = dalloc<int>(ctx, E, &st);
+    nd.st_step = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_mode = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_cursor = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_n = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
+    nd.st_count = dalloc<int>(ctx, nra, &st);
+    nd.st_items = dalloc<int>(ctx, (size_t)nra * kMaxStreams * NssA, &st);
+    ctx->d_a_ptr = dalloc<int>(ctx, nra + 1, &st);
+    ctx->d_a_tgt = dalloc<int>(ctx, A, &st);
+    ctx->d_a_w = dalloc<double>(ctx, A, &st);
+    ctx->d_a_d = dalloc<double>(ctx, A, &st);
+    ctx->d_b_ptr = dalloc<int>(ctx, ni + 1, &st);
+    ctx->d_b_tgt = dalloc<int>(ctx, Bn, &st);
+    ctx->d_b_w = dalloc<double>(ctx, Bn, &st);
+    ctx->d_b_d = dalloc<double>(ctx, Bn, &st);
+    nd.ra2i = FixedEdges{ctx->d_a_ptr, ctx->d_a_tgt, ctx->d_a_w, ctx->d_a_d};
+    nd.i2ra = FixedEdges{ctx->d_b_ptr, ctx->d_b_tgt, ctx->d_b_w, ctx->d_b_d};
+    auto mk_inflight = [&](int n) {
+        InFlight f;
+        f.step = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
+        f.cursor = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
+        f.head = dalloc<int>(ctx, n, &st);
+        f.count = dalloc<int>(ctx, n, &st);
+        return f;
+    };
+    nd.fl_ra2i = mk_inflight(nra);
+    nd.fl_i2ra = mk_inflight(ni);
+    nd.fl_rr = mk_inflight(nra);
+    nd.chain_e = dalloc<int>(ctx, E, &st);
+    auto mk_inbox = [&](int n, int cp) {
+        Inbox b;
+        b.cap = cp;
+        b.rec = dalloc<Record>(ctx, (size_t)n * cp, &st);
+        b.cnt = dalloc<int>(ctx, n, &st);
+        return b;
+    };
+    nd.in_gi_ra = mk_inbox(nra, c.cap_gi_ra);
+    nd.in_ge_ra = mk_inbox(nra, c.cap_ge_ra);
+    nd.in_ge_i = mk_inbox(ni, c.cap_ge_i);
+    nd.err = dalloc<unsigned int>(ctx, 1, &st);
+    nd.warn = dalloc<int>(ctx, 3 * (size_t)n_nets + 1, &st);
+    if (st != HVC_OK) free_bufs(ctx);
+    return st;
+}
+
+// the per-network tables and the network of every neuron, from ctx->ra_off / i_off / keys
+static hvc_status upload_tables(hvc_ctx* ctx)
+{
+    const int B = ctx->n_nets;
+    std::vector<int> ra_net(std::max(ctx->n_ra, 1), 0), i_net(std::max(ctx->n_i, 1), 0);
+    for (int b = 0; b < B; ++b) {
+        for (int i = ctx->ra_off[b]; i < ctx->ra_off[b + 1]; ++i) ra_net[i] = b;
+        for (int i = ctx->i_off[b]; i < ctx->i_off[b + 1]; ++i) i_net[i] = b;
+    }
+    hipStream_t s = ctx->stream;
+    if (ctx->n_ra > 0) HVC_TRY(hipMemcpyAsync(ctx->d_ra_net, ra_net.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
+    if (ctx->n_i > 0) HVC_TRY(hipMemcpyAsync(ctx->d_i_net, i_net.data(), sizeof(int) * ctx->n_i, hipMemcpyHostToDevice, s));
+    if (B > 0) {
+        HVC_TRY(hipMemcpyAsync(ctx->d_ra_off, ctx->ra_off.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_i_off, ctx->i_off.data(), sizeof(int) * (B + 1), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_key, ctx->keys.data(), sizeof(uint32_t) * 2 * B, hipMemcpyHostToDevice, s));
+    }
+    HVC_TRY(hipStreamSynchronize(s));
+    ctx->nd.n_ra = ctx->n_ra;
+    ctx->nd.n_i = ctx->n_i;
+    ctx->nd.n_nets = B;
+    return HVC_OK;
+}
+
 // upload of the (merged) network; ra_off / i_off / keys describe the networks in it
 static hvc_status upload_impl(hvc_ctx* ctx, const hvc_network* net, int n_nets, const std::vector<int>& ra_off,
                               const std::vector<int>& i_off, const uint32_t* keys, const std::vector<int>& train_net)
@@ -459,25 +677,9 @@ static hvc_status upload_impl(hvc_ctx* ctx, const hvc_network* net, int n_nets, 
     ctx->train_net = train_net;
     ctx->spread.assign(net->spread, net->spread + net->n_train);
 
-    // per row: candidates sorted by (delay, target).  Two different delays whose
-    // sums with a spike time could round to the same delivery time would make
-    // that order differ from the reference's (time, target) order; reject them
-    // (|d1 - d0| >= 4 ulp of the largest clock value keeps the sums distinct).
-    std::vector<int> perm(ctx->E);
-    for (int i = 0; i < nra; ++i) {
-        int b = net->rr_ptr[i], e = net->rr_ptr[i + 1];
-        for (int k = b; k < e; ++k) perm[k] = k;
-        std::sort(perm.begin() + b, perm.begin() + e, [&](int x, int y) {
-            if (net->rr_d[x] != net->rr_d[y]) return net->rr_d[x] < net->rr_d[y];
-            return net->rr_tgt[x] < net->rr_tgt[y];
-        });
-        for (int k = b + 1; k < e; ++k) {
-            double d0 = net->rr_d[perm[k - 1]], d1 = net->rr_d[perm[k]];
-            double tmax = ctx->prm.trial_ms + d1;
-            if (d0 != d1 && d1 - d0 < 4 * DBL_EPSILON * tmax)
-                return fail(ctx, HVC_E_ARG, "two RA->RA delays of one row differ by less than 4 ulp of the trial clock");
-        }
-    }
+    std::vector<int> perm;
+    hvc_status r = rr_order(ctx, net->rr_ptr, net->rr_tgt, net->rr_d, nra, 0, perm);
+    if (r != HVC_OK) return r;
     // a single network whose rows list every target: edge index = row start + target
     int dense = n_nets == 1 ? 1 : 0;
     for (int i = 0; i < nra && dense; ++i) {
@@ -492,154 +694,37 @@ static hvc_status upload_impl(hvc_ctx* ctx, const hvc_network* net, int n_nets, 
     sort_fixed(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_w, net->i2ra_d, ni, b_tgt, b_w, b_d);
 
     // inbox capacities: largest in-degree per connection type, plus slack
-    auto max_indeg = [](const int* ptr, const int* tgt, int n_pre, int n_post, int skip_self) {
-        std::vector<int> deg(std::max(n_post, 1), 0);
-        for (int i = 0; i < n_pre; ++i)
-            for (int e = ptr[i]; e < ptr[i + 1]; ++e)
-                if (!(skip_self && tgt[e] == i)) deg[tgt[e]]++;
-        return *std::max_element(deg.begin(), deg.end());
-    };
-    int cap_gi_ra = max_indeg(net->i2ra_ptr, net->i2ra_tgt, ni, nra, 0) + 8;
-    int cap_ge_ra = max_indeg(net->rr_ptr, net->rr_tgt, nra, nra, 1) + 8;
-    int cap_ge_i = max_indeg(net->ra2i_ptr, net->ra2i_tgt, nra, ni, 0) + 8;
-
-    // network of every neuron
-    std::vector<int> ra_net(nra), i_net(std::max(ni, 1), 0);
-    for (int b = 0; b < n_nets; ++b) {
-        for (int i = ra_off[b]; i < ra_off[b + 1]; ++i) ra_net[i] = b;
-        for (int i = i_off[b]; i < i_off[b + 1]; ++i) i_net[i] = b;
-    }
-
-    hvc_status st = HVC_OK;
-    NetDev& nd = ctx->nd;
-    nd.n_ra = nra;
-    nd.n_i = ni;
-    nd.dense = dense;
-    nd.n_nets = n_nets;
-    const int E = ctx->E;
-    const int Nss = ctx->prm.syn.Nss;
-    int* d_ra_net = dalloc<int>(ctx, nra, &st);
-    int* d_i_net = dalloc<int>(ctx, std::max(ni, 1), &st);
-    int* d_ra_off = dalloc<int>(ctx, n_nets + 1, &st);
-    int* d_i_off = dalloc<int>(ctx, n_nets + 1, &st);
-    uint32_t* d_key = dalloc<uint32_t>(ctx, 2 * (size_t)n_nets, &st);
-    ctx->d_net_trial = dalloc<uint32_t>(ctx, n_nets, &st);
-    ctx->d_net_first = dalloc<int>(ctx, n_nets, &st);
-    nd.ra_net = d_ra_net; nd.i_net = d_i_net; nd.net_ra_off = d_ra_off; nd.net_i_off = d_i_off;
-    nd.net_key = d_key; nd.net_trial = ctx->d_net_trial; nd.net_first = ctx->d_net_first;
-    for (int k = 0; k < 12; ++k) nd.ra[k] = dalloc<double>(ctx, nra, &st);
-    nd.ra_flags = dalloc<int>(ctx, nra, &st);
-    nd.ra_noise = dalloc<double>(ctx, 4 * (size_t)nra, &st);
-    nd.ra_draw = dalloc<uint32_t>(ctx, nra, &st);
-    double* d_EL = dalloc<double>(ctx, nra, &st);
-    double* d_GCa = dalloc<double>(ctx, nra, &st);
-    nd.EL = d_EL;
-    nd.GCa = d_GCa;
-    ctx->d_kick = dalloc<int>(ctx, nra, &st);
-    nd.kick_step = ctx->d_kick;
-    for (int k = 0; k < 7; ++k) nd.in[k] = dalloc<double>(ctx, ni, &st);
-    nd.i_flag = dalloc<int>(ctx, ni, &st);
-    nd.i_noise = dalloc<double>(ctx, 2 * (size_t)ni, &st);
-    nd.i_draw = dalloc<uint32_t>(ctx, ni, &st);
-    nd.soma_mask = dalloc<uint64_t>(ctx, nra, &st);
-    nd.dend_mask = dalloc<uint64_t>(ctx, nra, &st);
-    nd.i_mask = dalloc<uint64_t>(ctx, ni, &st);
-    int cap = ctx->prm.spike_capacity > 0 ? ctx->prm.spike_capacity : std::max(1 << 16, 256 * (nra + ni));
-    nd.log_cap = cap;
-    nd.soma_log = dalloc<int2>(ctx, cap, &st);
-    nd.dend_log = dalloc<int2>(ctx, cap, &st);
-    nd.i_log = dalloc<int2>(ctx, cap, &st);
-    nd.log_cnt = dalloc<int>(ctx, 4, &st);
-    nd.post_ring = dalloc<double>(ctx, (size_t)nra * kPostRing, &st);
-    nd.post_cnt = dalloc<int>(ctx, nra, &st);
-    nd.post_pend = dalloc<int>(ctx, nra, &st);
-    nd.post_list = dalloc<uint64_t>(ctx, kPostListCap, &st);
-    nd.post_list_cnt = dalloc<int>(ctx, 1, &st);
-    int* d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
-    int* d_rr_tgt = dalloc<int>(ctx, E, &st);
-    double* d_rr_d = dalloc<double>(ctx, E, &st);
-    int* d_perm = dalloc<int>(ctx, E, &st);
-    nd.rr_ptr = d_rr_ptr; nd.rr_tgt = d_rr_tgt; nd.rr_d = d_rr_d; nd.rr_perm = d_perm;
-    nd.w = dalloc<double>(ctx, E, &st);
-    nd.act = dalloc<uint8_t>(ctx, E, &st);
-    nd.sup = dalloc<uint8_t>(ctx, E, &st);
-    nd.act_list = dalloc<int>(ctx, E, &st);
-    nd.act_cnt = dalloc<int>(ctx, nra, &st);
-    nd.sup_list = dalloc<int>(ctx, (size_t)nra * std::max(Nss, 1), &st);
-    nd.sup_cnt = dalloc<int>(ctx, nra, &st);
-    nd.remodeled = dalloc<int>(ctx, nra, &st);
-    nd.pre_ring = dalloc<double>(ctx, (size_t)E * kPreRing, &st);
-    nd.pre_cnt = dalloc<int>(ctx, E, &st);
-    nd.st_step = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
-    nd.st_mode = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
-    nd.st_cursor = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
-    nd.st_n = dalloc<int>(ctx, (size_t)nra * kMaxStreams, &st);
-    nd.st_count = dalloc<int>(ctx, nra, &st);
-    nd.st_items = dalloc<int>(ctx, (size_t)nra * kMaxStreams * std::max(Nss, 1), &st);
-    int* d_a_ptr = dalloc<int>(ctx, nra + 1, &st);
-    int* d_a_tgt = dalloc<int>(ctx, a_tgt.size(), &st);
-    double* d_a_w = dalloc<double>(ctx, a_w.size(), &st);
-    double* d_a_d = dalloc<double>(ctx, a_d.size(), &st);
-    int* d_b_ptr = dalloc<int>(ctx, ni + 1, &st);
-    int* d_b_tgt = dalloc<int>(ctx, b_tgt.size(), &st);
-    double* d_b_w = dalloc<double>(ctx, b_w.size(), &st);
-    double* d_b_d = dalloc<double>(ctx, b_d.size(), &st);
-    nd.ra2i = FixedEdges{d_a_ptr, d_a_tgt, d_a_w, d_a_d};
-    nd.i2ra = FixedEdges{d_b_ptr, d_b_tgt, d_b_w, d_b_d};
-    auto mk_inflight = [&](int n) {
-        InFlight f;
-        f.step = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
-        f.cursor = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
-        f.head = dalloc<int>(ctx, n, &st);
-        f.count = dalloc<int>(ctx, n, &st);
-        return f;
-    };
-    nd.fl_ra2i = mk_inflight(nra);
-    nd.fl_i2ra = mk_inflight(ni);
-    nd.fl_rr = mk_inflight(nra);
-    nd.chain_e = dalloc<int>(ctx, E, &st);
-    auto mk_inbox = [&](int n, int c) {
-        Inbox b;
-        b.cap = c;
-        b.rec = dalloc<Record>(ctx, (size_t)std::max(n, 1) * c, &st);
-        b.cnt = dalloc<int>(ctx, n, &st);
-        return b;
-    };
-    nd.in_gi_ra = mk_inbox(nra, cap_gi_ra);
-    nd.in_ge_ra = mk_inbox(nra, cap_ge_ra);
-    nd.in_ge_i = mk_inbox(ni, cap_ge_i);
-    nd.err = dalloc<unsigned int>(ctx, 1, &st);
-    nd.warn = dalloc<int>(ctx, 3 * (size_t)n_nets + 1, &st);
-    if (st != HVC_OK) {
-        free_bufs(ctx);
-        return st;
-    }
+    Caps c;
+    c.nets = n_nets; c.ra = nra; c.i = ni;
+    c.rr = ctx->E; c.a2i = (long long)a_tgt.size(); c.i2ra = (long long)b_tgt.size();
+    c.cap_gi_ra = max_indeg(net->i2ra_ptr, net->i2ra_tgt, ni, nra, 0) + 8;
+    c.cap_ge_ra = max_indeg(net->rr_ptr, net->rr_tgt, nra, nra, 1) + 8;
+    c.cap_ge_i = max_indeg(net->ra2i_ptr, net->ra2i_tgt, nra, ni, 0) + 8;
+    if ((r = alloc_device(ctx, c)) != HVC_OK) return r;
+    ctx->nd.dense = dense;
+    if ((r = upload_tables(ctx)) != HVC_OK) return r;
 
     hipStream_t s = ctx->stream;
-    HVC_TRY(hipMemcpyAsync(d_ra_net, ra_net.data(), sizeof(int) * nra, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_i_net, i_net.data(), sizeof(int) * std::max(ni, 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_ra_off, ra_off.data(), sizeof(int) * (n_nets + 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_i_off, i_off.data(), sizeof(int) * (n_nets + 1), hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_key, keys, sizeof(uint32_t) * 2 * n_nets, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_EL, net->erest, sizeof(double) * nra, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_GCa, net->gca, sizeof(double) * nra, hipMemcpyHostToDevice, s));
-    HVC_TRY(hipMemcpyAsync(d_rr_ptr, net->rr_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
+    const int E = ctx->E;
+    HVC_TRY(hipMemcpyAsync(ctx->d_EL, net->erest, sizeof(double) * nra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_GCa, net->gca, sizeof(double) * nra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_rr_ptr, net->rr_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
     if (E > 0) {
-        HVC_TRY(hipMemcpyAsync(d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
     }
-    HVC_TRY(hipMemcpyAsync(d_a_ptr, net->ra2i_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_a_ptr, net->ra2i_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
     if (!a_tgt.empty()) {
-        HVC_TRY(hipMemcpyAsync(d_a_tgt, a_tgt.data(), sizeof(int) * a_tgt.size(), hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_a_w, a_w.data(), sizeof(double) * a_w.size(), hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_a_d, a_d.data(), sizeof(double) * a_d.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_tgt, a_tgt.data(), sizeof(int) * a_tgt.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_w, a_w.data(), sizeof(double) * a_w.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_d, a_d.data(), sizeof(double) * a_d.size(), hipMemcpyHostToDevice, s));
     }
-    HVC_TRY(hipMemcpyAsync(d_b_ptr, net->i2ra_ptr, sizeof(int) * (ni + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_b_ptr, net->i2ra_ptr, sizeof(int) * (ni + 1), hipMemcpyHostToDevice, s));
     if (!b_tgt.empty()) {
-        HVC_TRY(hipMemcpyAsync(d_b_tgt, b_tgt.data(), sizeof(int) * b_tgt.size(), hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_b_w, b_w.data(), sizeof(double) * b_w.size(), hipMemcpyHostToDevice, s));
-        HVC_TRY(hipMemcpyAsync(d_b_d, b_d.data(), sizeof(double) * b_d.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_tgt, b_tgt.data(), sizeof(int) * b_tgt.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_w, b_w.data(), sizeof(double) * b_w.size(), hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_d, b_d.data(), sizeof(double) * b_d.size(), hipMemcpyHostToDevice, s));
     }
     HVC_TRY(hipStreamSynchronize(s));
     ctx->uploaded = true;
@@ -873,20 +958,22 @@ extern "C" hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, doub
 static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs)
 {
     const int B = ctx->n_nets;
-    const int mode = trials[0].mode;
-    if (mode != 0 && mode != 1) return fail(ctx, HVC_E_ARG, "mode must be 0 (growth) or 1 (chain test)");
-    for (int b = 1; b < B; ++b)
-        if (trials[b].mode != mode) return fail(ctx, HVC_E_ARG, "all trials of a batch must have the same mode");
+    int any_growth = 0, any_chain = 0;
+    for (int b = 0; b < B; ++b) {
+        if (trials[b].mode != 0 && trials[b].mode != 1) return fail(ctx, HVC_E_ARG, "mode must be 0 (growth) or 1 (chain test)");
+        any_growth |= trials[b].mode == 0;
+        any_chain |= trials[b].mode == 1;
+    }
     HVC_TRY(hipSetDevice(ctx->device));
-    const int growth = mode == 0;
     NetDev& nd = ctx->nd;
     hipStream_t s = ctx->stream;
 
-    // training kick: first step with internal_time > spread + mean_injection_time (NGS:8933-8940)
+    // training kick: first step with internal_time > spread + mean_injection_time (NGS:8933-8940);
+    // a growth trial kicks only when training, a chain test always (NGS:12785-12801)
     ctx->h_kick.assign(ctx->n_ra, -1);
     for (size_t k = 0; k < ctx->train_ids.size(); ++k) {
         const hvc_trial& tr = trials[ctx->train_net[k]];
-        if (growth && !tr.training) continue;
+        if (tr.mode == 0 && !tr.training) continue;
         double thr = ctx->spread[k] + tr.mean_injection_time;
         for (int t = 1; t < ctx->num_steps; ++t)
             if (ctx->clock[t] > thr) {
@@ -896,34 +983,37 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     }
     ctx->h_net_trial.resize(B);
     ctx->h_net_first.resize(B);
+    ctx->h_net_growth.resize(B);
     for (int b = 0; b < B; ++b) {
         ctx->h_net_trial[b] = trials[b].trial_index;
         ctx->h_net_first[b] = trials[b].first_trial ? 1 : 0;
+        ctx->h_net_growth[b] = trials[b].mode == 0 ? 1 : 0;
     }
     HVC_TRY(hipMemcpyAsync(ctx->d_kick, ctx->h_kick.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, ctx->h_net_trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_first, ctx->h_net_first.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_growth, ctx->h_net_growth.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     launch_reset(s, nd, ctx->E);
-    if (!growth) launch_chain_edges(s, nd);
+    if (any_chain) launch_chain_edges(s, nd);
     const int nw = (int)ctx->windows.size();
     for (int w = 0; w < nw; ++w) {
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1, growth);
+            launch_integrate_f32(s, nd, t0, t1);
         else
-            launch_integrate(s, nd, t0, t1, growth);
+            launch_integrate(s, nd, t0, t1);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
-        if (growth) launch_sort_posts(s, nd);
-        launch_deliver(s, nd, t0, t1, growth);
+        if (any_growth) launch_sort_posts(s, nd);
+        launch_deliver(s, nd, t0, t1);
         tstop(ctx, kTDeliver, tm);
         tstart(ctx, kTSync, tm);
         launch_sync(s, nd);
         tstop(ctx, kTSync, tm);
     }
-    if (growth) {
+    if (any_growth) {
         tstart(ctx, kTDecay, ctx->timing);
         launch_decay(s, nd);
         tstop(ctx, kTDecay, ctx->timing);
@@ -1006,6 +1096,325 @@ extern "C" hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_t
     if (!ctx || !trials) return HVC_E_ARG;
     if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload networks first");
     return run_impl(ctx, trials, outs);
+}
+
+// ---------------------------------------------------------------------------
+// serving pool: networks admitted and released between trials
+// ---------------------------------------------------------------------------
+// Device bytes per element of the pool's arrays (the sizes alloc_device uses).
+struct PoolCost {
+    double ra, i, rr, a2i, i2ra, net;
+};
+
+static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes_per_neuron)
+{
+    const double NssA = std::max(prm->syn.Nss, 1);
+    const double inbox = (double)(max_indeg + 8) * sizeof(Record) + sizeof(int);
+    const double logs = 3.0 * spikes_per_neuron * sizeof(int2);
+    const double inflight = 2.0 * kMaxInflight * sizeof(int) + 2 * sizeof(int);
+    PoolCost c;
+    c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
+           4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs;
+    c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs;
+    c.rr = 4 + 8 + 4 + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;
+    c.a2i = 4 + 8 + 8;
+    c.i2ra = 4 + 8 + 8;
+    c.net = 2 * 4 + 2 * 4 + 4 * 4 + 3 * 4;
+    return c;
+}
+
+extern "C" hvc_status hvc_pool_footprint(const hvc_params* prm, const hvc_pool_caps* caps, int n_ra, int n_i,
+                                         long long n_rr, long long n_ra2i, long long n_i2ra, size_t* network_bytes,
+                                         size_t* pool_bytes)
+{
+    if (!prm || !caps) return HVC_E_ARG;
+    const long long spn = caps->spikes_per_neuron > 0 ? caps->spikes_per_neuron : 256;
+    PoolCost c = pool_cost(prm, caps->max_in_degree, spn);
+    if (network_bytes)
+        *network_bytes = (size_t)(n_ra * c.ra + n_i * c.i + n_rr * c.rr + n_ra2i * c.a2i + n_i2ra * c.i2ra + c.net);
+    if (pool_bytes)
+        *pool_bytes = (size_t)((double)caps->max_ra * c.ra + (double)caps->max_i * c.i + (double)caps->max_rr * c.rr +
+                               (double)caps->max_ra2i * c.a2i + (double)caps->max_i2ra * c.i2ra +
+                               (double)caps->max_nets * c.net);
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_pool_create(const hvc_params* prm, int device, const hvc_pool_caps* caps, hvc_ctx** out)
+{
+    if (!caps || !out) return HVC_E_ARG;
+    hvc_status r = hvc_create(prm, device, out);
+    if (r != HVC_OK) return r;
+    hvc_ctx* ctx = *out;
+    if (caps->max_nets <= 0 || caps->max_nets >= kMaxNets || caps->max_ra <= 0 || caps->max_i < 0 ||
+        caps->max_rr < 0 || caps->max_ra2i < 0 || caps->max_i2ra < 0 || caps->max_in_degree < 0 ||
+        caps->max_ra >= kMaxKeyNeurons)
+        return fail(ctx, HVC_E_ARG, "pool capacities out of range");
+    if (ctx->num_steps >= kMaxKeySteps) return fail(ctx, HVC_E_ARG, "trial has too many steps");
+    Caps c;
+    c.nets = caps->max_nets; c.ra = caps->max_ra; c.i = caps->max_i;
+    c.rr = caps->max_rr; c.a2i = caps->max_ra2i; c.i2ra = caps->max_i2ra;
+    c.cap_gi_ra = c.cap_ge_ra = c.cap_ge_i = caps->max_in_degree + 8;
+    const long long spn = caps->spikes_per_neuron > 0 ? caps->spikes_per_neuron : 256;
+    c.log_cap = spn * ((long long)caps->max_ra + caps->max_i);
+    if ((r = alloc_device(ctx, c)) != HVC_OK) return r;
+    ctx->pool = true;
+    ctx->n_ra = ctx->n_i = ctx->E = 0;
+    ctx->n_nets = 0;
+    ctx->ra_off.assign(1, 0);
+    ctx->i_off.assign(1, 0);
+    ctx->keys.clear();
+    ctx->rr_ptr.assign(1, 0);
+    ctx->rr_tgt.clear();
+    ctx->nd.dense = 0;
+    int zero = 0;
+    HVC_TRY(hipMemcpy(ctx->d_rr_ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HVC_TRY(hipMemcpy(ctx->d_a_ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+    HVC_TRY(hipMemcpy(ctx->d_b_ptr, &zero, sizeof(int), hipMemcpyHostToDevice));
+    if ((r = upload_tables(ctx)) != HVC_OK) return r;
+    ctx->uploaded = true;
+    return HVC_OK;
+}
+
+// host bookkeeping of the pool from its resident networks (positions = order)
+static void pool_rebuild_host(hvc_ctx* ctx)
+{
+    ctx->n_nets = (int)ctx->pnets.size();
+    ctx->ra_off.assign(1, 0);
+    ctx->i_off.assign(1, 0);
+    ctx->keys.clear();
+    ctx->train_ids.clear();
+    ctx->train_net.clear();
+    ctx->spread.clear();
+    ctx->E = 0;
+    ctx->a_tot = ctx->b_tot = 0;
+    for (size_t p = 0; p < ctx->pnets.size(); ++p) {
+        const PoolNet& pn = ctx->pnets[p];
+        const int ro = ctx->ra_off.back();
+        for (size_t k = 0; k < pn.train_ids.size(); ++k) {
+            ctx->train_ids.push_back(pn.train_ids[k] + ro);
+            ctx->train_net.push_back((int)p);
+            ctx->spread.push_back(pn.spread[k]);
+        }
+        ctx->ra_off.push_back(ro + pn.n_ra);
+        ctx->i_off.push_back(ctx->i_off.back() + pn.n_i);
+        ctx->keys.push_back(pn.key[0]);
+        ctx->keys.push_back(pn.key[1]);
+        ctx->E += pn.E;
+        ctx->a_tot += pn.A;
+        ctx->b_tot += pn.Bn;
+    }
+    ctx->n_ra = ctx->ra_off.back();
+    ctx->n_i = ctx->i_off.back();
+}
+
+extern "C" hvc_status hvc_pool_admit(hvc_ctx* ctx, const hvc_network* net, uint32_t key_lo, uint32_t key_hi,
+                                     int* handle)
+{
+    if (!ctx || !net || !handle) return HVC_E_ARG;
+    if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    hvc_status r = check_network(ctx, net, "network");
+    if (r != HVC_OK) return r;
+    const int n = net->n_ra, ni = net->n_i;
+    const int E = net->rr_ptr[n], A = net->ra2i_ptr[n], Bn = net->i2ra_ptr[ni];
+    const int indeg = std::max(max_indeg(net->i2ra_ptr, net->i2ra_tgt, ni, n, 0),
+                               std::max(max_indeg(net->rr_ptr, net->rr_tgt, n, n, 1),
+                                        max_indeg(net->ra2i_ptr, net->ra2i_tgt, n, ni, 0)));
+    if (indeg > ctx->caps_indeg) return fail(ctx, HVC_E_ARG, "network in-degree exceeds the pool's max_in_degree");
+    if (ctx->n_nets + 1 > ctx->caps_nets || ctx->n_ra + n > ctx->caps_ra || ctx->n_i + ni > ctx->caps_i ||
+        ctx->E + E > ctx->caps_rr || ctx->a_tot + A > ctx->caps_a2i || ctx->b_tot + Bn > ctx->caps_i2ra)
+        return fail(ctx, HVC_E_CAPACITY, "the pool has no room for this network now");
+    const int ro = ctx->n_ra, io = ctx->n_i, eo = ctx->E;
+    const long long ao = ctx->a_tot, bo = ctx->b_tot;
+    std::vector<int> perm;
+    if ((r = rr_order(ctx, net->rr_ptr, net->rr_tgt, net->rr_d, n, eo, perm)) != HVC_OK) return r;
+    std::vector<int> a_tgt, b_tgt;
+    std::vector<double> a_w, a_d, b_w, b_d;
+    sort_fixed(net->ra2i_ptr, net->ra2i_tgt, net->ra2i_w, net->ra2i_d, n, a_tgt, a_w, a_d);
+    sort_fixed(net->i2ra_ptr, net->i2ra_tgt, net->i2ra_w, net->i2ra_d, ni, b_tgt, b_w, b_d);
+    std::vector<int> rp(n + 1), rt(E), ap(n + 1), bp(ni + 1);
+    for (int k = 0; k <= n; ++k) rp[k] = net->rr_ptr[k] + eo;
+    for (int k = 0; k < E; ++k) rt[k] = net->rr_tgt[k] + ro;
+    for (int k = 0; k <= n; ++k) ap[k] = net->ra2i_ptr[k] + (int)ao;
+    for (int k = 0; k <= ni; ++k) bp[k] = net->i2ra_ptr[k] + (int)bo;
+    for (auto& x : a_tgt) x += io;
+    for (auto& x : b_tgt) x += ro;
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(ctx->d_rr_ptr + ro, rp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_a_ptr + ro, ap.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_b_ptr + io, bp.data(), sizeof(int) * (ni + 1), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_EL + ro, net->erest, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_GCa + ro, net->gca, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (E > 0) {
+        HVC_TRY(hipMemcpyAsync(ctx->d_rr_tgt + eo, rt.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_rr_d + eo, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_perm + eo, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+    }
+    if (A > 0) {
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_tgt + ao, a_tgt.data(), sizeof(int) * A, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_w + ao, a_w.data(), sizeof(double) * A, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_a_d + ao, a_d.data(), sizeof(double) * A, hipMemcpyHostToDevice, s));
+    }
+    if (Bn > 0) {
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_tgt + bo, b_tgt.data(), sizeof(int) * Bn, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_w + bo, b_w.data(), sizeof(double) * Bn, hipMemcpyHostToDevice, s));
+        HVC_TRY(hipMemcpyAsync(ctx->d_b_d + bo, b_d.data(), sizeof(double) * Bn, hipMemcpyHostToDevice, s));
+    }
+    HVC_TRY(hipStreamSynchronize(s));
+    PoolNet pn;
+    pn.handle = ctx->next_handle++;
+    pn.n_ra = n; pn.n_i = ni; pn.E = E; pn.A = A; pn.Bn = Bn;
+    pn.key[0] = key_lo; pn.key[1] = key_hi;
+    pn.train_ids.assign(net->train_ids, net->train_ids + net->n_train);
+    pn.spread.assign(net->spread, net->spread + net->n_train);
+    ctx->pnets.push_back(pn);
+    // host mirror of the candidate rows (edge lookups of the synapse-state calls)
+    ctx->rr_ptr.resize(ro + n + 1);
+    for (int k = 0; k <= n; ++k) ctx->rr_ptr[ro + k] = rp[k];
+    ctx->rr_tgt.insert(ctx->rr_tgt.end(), rt.begin(), rt.end());
+    pool_rebuild_host(ctx);
+    if ((r = upload_tables(ctx)) != HVC_OK) return r;
+    ctx->trial_done = false;
+    *handle = pn.handle;
+    return set_synapse_state(ctx, ro, ro + n, net->rr_w, net->act_cnt, net->act_ids, net->sup_cnt, net->sup_ids,
+                             net->remodeled);
+}
+
+static int pool_position(const hvc_ctx* ctx, int handle)
+{
+    for (size_t p = 0; p < ctx->pnets.size(); ++p)
+        if (ctx->pnets[p].handle == handle) return (int)p;
+    return -1;
+}
+
+// moves count elements of a device array from index `from` to index `to` (to <
+// from, ranges may overlap) through a staging buffer; reloc != 0 adds reloc to
+// every moved int32 element (ids and edge indices that shift with the move)
+static hvc_status move_down(hvc_ctx* ctx, void* base, size_t esz, long long from, long long to, long long count,
+                            int reloc)
+{
+    if (count <= 0) return HVC_OK;
+    const size_t bytes = (size_t)count * esz;
+    if (bytes > ctx->stage_bytes) {
+        if (ctx->d_stage) HVC_TRY(hipFree(ctx->d_stage));
+        ctx->d_stage = nullptr;
+        ctx->stage_bytes = 0;
+        HVC_TRY(hipMalloc(&ctx->d_stage, bytes));
+        ctx->stage_bytes = bytes;
+    }
+    char* b = (char*)base;
+    hipStream_t s = ctx->stream;
+    HVC_TRY(hipMemcpyAsync(ctx->d_stage, b + from * esz, bytes, hipMemcpyDeviceToDevice, s));
+    HVC_TRY(hipMemcpyAsync(b + to * esz, ctx->d_stage, bytes, hipMemcpyDeviceToDevice, s));
+    if (reloc != 0) launch_add_i32(s, (int*)(b + to * esz), count, reloc);
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_pool_release(hvc_ctx* ctx, int handle)
+{
+    if (!ctx) return HVC_E_ARG;
+    if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    const int p = pool_position(ctx, handle);
+    if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
+    const PoolNet gone = ctx->pnets[p];
+    const int dn = gone.n_ra, di = gone.n_i, dE = gone.E, dA = gone.A, dB = gone.Bn;
+    const int NssA = std::max(ctx->prm.syn.Nss, 1);
+    NetDev& nd = ctx->nd;
+    hvc_status r;
+    // networks after the released one move down, with their ids and edge indices
+    for (size_t q = p + 1; q < ctx->pnets.size(); ++q) {
+        const PoolNet& pn = ctx->pnets[q];
+        const long long ro = ctx->ra_off[q], io = ctx->i_off[q];
+        long long eo = 0, ao = 0, bo = 0;
+        for (size_t k = 0; k < q; ++k) {
+            eo += ctx->pnets[k].E;
+            ao += ctx->pnets[k].A;
+            bo += ctx->pnets[k].Bn;
+        }
+        struct Mv { void* base; size_t esz; long long from, to, count; int reloc; };
+        const Mv mv[] = {
+            {ctx->d_EL, 8, ro, ro - dn, pn.n_ra, 0},
+            {ctx->d_GCa, 8, ro, ro - dn, pn.n_ra, 0},
+            {nd.act_cnt, 4, ro, ro - dn, pn.n_ra, 0},
+            {nd.sup_cnt, 4, ro, ro - dn, pn.n_ra, 0},
+            {nd.remodeled, 4, ro, ro - dn, pn.n_ra, 0},
+            {nd.sup_list, 4, ro * NssA, (ro - dn) * NssA, (long long)pn.n_ra * NssA, -dn},
+            {ctx->d_rr_ptr, 4, ro, ro - dn, pn.n_ra, -dE},
+            {ctx->d_a_ptr, 4, ro, ro - dn, pn.n_ra, -dA},
+            {ctx->d_b_ptr, 4, io, io - di, pn.n_i, -dB},
+            {ctx->d_rr_tgt, 4, eo, eo - dE, pn.E, -dn},
+            {ctx->d_rr_d, 8, eo, eo - dE, pn.E, 0},
+            {ctx->d_perm, 4, eo, eo - dE, pn.E, -dE},
+            {nd.w, 8, eo, eo - dE, pn.E, 0},
+            {nd.act, 1, eo, eo - dE, pn.E, 0},
+            {nd.sup, 1, eo, eo - dE, pn.E, 0},
+            {nd.act_list, 4, eo, eo - dE, pn.E, -dn},
+            {ctx->d_a_tgt, 4, ao, ao - dA, pn.A, -di},
+            {ctx->d_a_w, 8, ao, ao - dA, pn.A, 0},
+            {ctx->d_a_d, 8, ao, ao - dA, pn.A, 0},
+            {ctx->d_b_tgt, 4, bo, bo - dB, pn.Bn, -dn},
+            {ctx->d_b_w, 8, bo, bo - dB, pn.Bn, 0},
+            {ctx->d_b_d, 8, bo, bo - dB, pn.Bn, 0},
+        };
+        for (const Mv& m : mv)
+            if ((r = move_down(ctx, m.base, m.esz, m.from, m.to, m.count, m.reloc)) != HVC_OK) return r;
+    }
+    // host mirror of the candidate rows
+    const int ro_g = ctx->ra_off[p];
+    const int eo_g = ctx->rr_ptr[ro_g];
+    ctx->rr_ptr.erase(ctx->rr_ptr.begin() + ro_g, ctx->rr_ptr.begin() + ro_g + dn);
+    for (size_t k = ro_g; k < ctx->rr_ptr.size(); ++k) ctx->rr_ptr[k] -= dE;
+    ctx->rr_tgt.erase(ctx->rr_tgt.begin() + eo_g, ctx->rr_tgt.begin() + eo_g + dE);
+    for (size_t k = eo_g; k < ctx->rr_tgt.size(); ++k) ctx->rr_tgt[k] -= dn;
+    ctx->pnets.erase(ctx->pnets.begin() + p);
+    pool_rebuild_host(ctx);
+    hipStream_t s = ctx->stream;
+    const int te = ctx->E, ta = (int)ctx->a_tot, tb = (int)ctx->b_tot;
+    HVC_TRY(hipMemcpyAsync(ctx->d_rr_ptr + ctx->n_ra, &te, sizeof(int), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_a_ptr + ctx->n_ra, &ta, sizeof(int), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_b_ptr + ctx->n_i, &tb, sizeof(int), hipMemcpyHostToDevice, s));
+    HVC_TRY(hipStreamSynchronize(s));
+    ctx->trial_done = false;
+    return upload_tables(ctx);
+}
+
+extern "C" hvc_status hvc_pool_run(hvc_ctx* ctx, int n, const int* handles, const hvc_trial* trials,
+                                   hvc_trial_out* outs)
+{
+    if (!ctx || (n > 0 && (!handles || !trials))) return HVC_E_ARG;
+    if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    if (n != ctx->n_nets) return fail(ctx, HVC_E_ARG, "hvc_pool_run needs one trial for every resident network");
+    if (n == 0) return HVC_OK;
+    std::vector<hvc_trial> tp(n);
+    std::vector<hvc_trial_out> op(n);
+    std::vector<int> where(n, -1);
+    for (int k = 0; k < n; ++k) {
+        int p = pool_position(ctx, handles[k]);
+        if (p < 0 || where[p] >= 0) return fail(ctx, HVC_E_ARG, "unknown or repeated network handle");
+        where[p] = k;
+        tp[p] = trials[k];
+        if (outs) op[p] = outs[k];
+    }
+    hvc_status r = run_impl(ctx, tp.data(), outs ? op.data() : nullptr);
+    if (outs)
+        for (int p = 0; p < n; ++p) outs[where[p]] = op[p];
+    return r;
+}
+
+extern "C" hvc_status hvc_pool_spikes(hvc_ctx* ctx, int handle, int kind, int* counts, double* times, int cap,
+                                      int* total)
+{
+    if (!ctx) return HVC_E_ARG;
+    const int p = pool_position(ctx, handle);
+    if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
+    return hvc_batch_spikes(ctx, p, kind, counts, times, cap, total);
+}
+
+extern "C" hvc_status hvc_pool_download_synapses(hvc_ctx* ctx, int handle, hvc_synapses* syn)
+{
+    if (!ctx) return HVC_E_ARG;
+    const int p = pool_position(ctx, handle);
+    if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
+    return hvc_batch_download_synapses(ctx, p, syn);
 }
 
 // ---------------------------------------------------------------------------

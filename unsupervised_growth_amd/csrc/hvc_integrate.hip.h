@@ -223,10 +223,11 @@ __device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool p
 // longer per-step work, and the RA blocks run beside them.
 // ---------------------------------------------------------------------------
 template <int MODEL>
-__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1, int growth)
+__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
 {
     const bool live = i < nd.n_ra;
     const int ii = live ? i : 0;
+    const int growth = nd.net_growth[nd.ra_net[ii]];
     const DecayRow* rows = (const DecayRow*)nd.decay;
     const RAUniform u = ra_uniform(nd);
     RAState s;
@@ -424,7 +425,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
 
 // one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
 template <int RA_MODEL, int I_MODEL>
-__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, int growth, int iblocks)
+__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, int iblocks)
 {
     if ((int)blockIdx.x < iblocks) {
         if (I_MODEL == 0)
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, int
         else
             integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1);
     } else {
-        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1, growth);
+        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1);
     }
 }
 
@@ -457,20 +458,20 @@ void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n
     k_trace_i<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, first_trial, trial, exc_g, inh_g, out, fired);
 }
 
-void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, int growth)
+void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1)
 {
     int iblocks = nd.n_i == 0 ? 0 : (nd.i_model == 0 ? (nd.n_i + kINeuronsPerWave - 1) / kINeuronsPerWave
                                                      : blocks_for(nd.n_i, 64));
     int rblocks = blocks_for(nd.n_ra, 64);
     dim3 grid(iblocks + rblocks);
     if (nd.ra_model == 0 && nd.i_model == 0)
-        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
+        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
     else if (nd.ra_model == 0)
-        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
+        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
     else if (nd.i_model == 0)
-        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
+        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
     else
-        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, growth, iblocks);
+        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
 }
 
 #undef HVC_LAUNCHER

@@ -522,14 +522,14 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
 //   role 0: RA->RA rows (growth) or chain-test RA->RA push
 //   role 1: RA->I push        role 2: I->RA push
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int growth, int blocks_ra)
+__global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int blocks_ra)
 {
     int b = blockIdx.x;
     int role = b / blocks_ra;
     int local = (b % blocks_ra) * blockDim.x + threadIdx.x;
     if (role == 0) {
         if (local >= nd.n_ra) return;
-        if (growth)
+        if (nd.net_growth[nd.ra_net[local]])
             rr_row(nd, local, t0, t1);
         else
             push_fixed(nd, local, nd.rr_ptr[local], nd.rr_ptr[local] + nd.act_cnt[local], nd.chain_e, nd.rr_tgt,
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_sync(NetDev nd)
 __global__ __launch_bounds__(256) void k_decay(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nd.n_ra) return;
+    if (i >= nd.n_ra || !nd.net_growth[nd.ra_net[i]]) return;
     const hvc_synaptic_params& p = nd.syn;
     for (int e = nd.rr_ptr[i]; e < nd.rr_ptr[i + 1]; ++e) {
         double w = nd.w[e];
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_decay(NetDev nd)
 __global__ void k_chain_edges(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nd.n_ra) return;
+    if (i >= nd.n_ra || nd.net_growth[nd.ra_net[i]]) return;
     int base = nd.rr_ptr[i];
     int n = nd.act_cnt[i];
     int* out = nd.chain_e + base;
@@ -668,11 +668,22 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
     if (E > 0) k_clear_pre<<<blocks_for(E, 256), 256, 0, st>>>(nd.pre_cnt, E);
 }
 
-void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, int growth)
+void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1)
 {
     int nmax = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
     int blocks_ra = blocks_for(nmax, 64);
-    k_deliver<<<3 * blocks_ra, 64, 0, st>>>(nd, t0, t1, growth, blocks_ra);
+    k_deliver<<<3 * blocks_ra, 64, 0, st>>>(nd, t0, t1, blocks_ra);
+}
+
+__global__ void k_add_i32(int* a, long long n, int delta)
+{
+    long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] += delta;
+}
+
+void launch_add_i32(hipStream_t st, int* a, long long n, int delta)
+{
+    if (n > 0) k_add_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, n, delta);
 }
 
 void launch_sort_posts(hipStream_t st, const NetDev& nd)

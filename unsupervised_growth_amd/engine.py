@@ -287,3 +287,106 @@ class Engine:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._check(self.lib.hvc_kernel_timing(self.ctx, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def pool_caps(max_nets, max_ra, max_i, max_rr, max_ra2i, max_i2ra, max_in_degree, spikes_per_neuron=0):
+    return _abi.PoolCaps(max_nets, max_ra, max_i, max_rr, max_ra2i, max_i2ra, max_in_degree, spikes_per_neuron)
+
+
+def network_sizes(net: Network):
+    """(n_ra, n_i, candidates, RA->I, I->RA) of a network."""
+    return net.n_ra, net.n_i, int(net.rr_ptr[-1]), int(net.ra2i_ptr[-1]), int(net.i2ra_ptr[-1])
+
+
+class Pool(Engine):
+    """A serving pool (hvc_pool_*): networks are admitted and released between
+    trials and every run advances all resident networks by one trial."""
+
+    def __init__(self, caps, params=None, device=0, **kw):
+        self.lib = _abi.load()
+        self.params = params if params is not None else make_params(**kw)
+        self.caps = caps
+        self.ctx = ctypes.c_void_p()
+        st = self.lib.hvc_pool_create(ctypes.byref(self.params), device, ctypes.byref(caps), ctypes.byref(self.ctx))
+        if st != _abi.HVC_OK:
+            msg = self.lib.hvc_last_error(self.ctx).decode() if self.ctx else ""
+            if self.ctx:
+                self.lib.hvc_destroy(self.ctx)
+            self.ctx = None
+            raise EngineError(st, msg)
+        self.net = None
+        self.nets = []
+        self.resident = {}      # handle -> Network
+
+    def footprint(self, net: Network):
+        """Device bytes the network occupies in this pool."""
+        b = ctypes.c_size_t()
+        self._check(self.lib.hvc_pool_footprint(ctypes.byref(self.params), ctypes.byref(self.caps),
+                                                *network_sizes(net), ctypes.byref(b), None))
+        return b.value
+
+    def pool_bytes(self):
+        b = ctypes.c_size_t()
+        self._check(self.lib.hvc_pool_footprint(ctypes.byref(self.params), ctypes.byref(self.caps), 0, 0, 0, 0, 0,
+                                                None, ctypes.byref(b)))
+        return b.value
+
+    def admit(self, net: Network, key):
+        """Handle of the admitted network, or None when it does not fit now."""
+        d, keep = self._desc(net)
+        h = ctypes.c_int()
+        st = self.lib.hvc_pool_admit(self.ctx, ctypes.byref(d), int(key[0]), int(key[1]), ctypes.byref(h))
+        if st == _abi.HVC_E_CAPACITY:
+            return None
+        self._check(st)
+        self.resident[h.value] = net
+        return h.value
+
+    def release(self, handle):
+        self._check(self.lib.hvc_pool_release(self.ctx, handle))
+        self.resident.pop(handle, None)
+
+    def _pool_spikes(self, handle, kind, n):
+        c = np.zeros(n, np.int32)
+        tot = ctypes.c_int()
+        ip, dp = _abi.c_int_p, _abi.c_double_p
+        self._check(self.lib.hvc_pool_spikes(self.ctx, handle, kind, _p(c, ip) if n else None, None, 0,
+                                             ctypes.byref(tot)))
+        times = np.zeros(max(tot.value, 1))
+        self._check(self.lib.hvc_pool_spikes(self.ctx, handle, kind, _p(c, ip) if n else None, _p(times, dp),
+                                             len(times), ctypes.byref(tot)))
+        return np.split(times[:tot.value], np.cumsum(c)[:-1]) if n else []
+
+    def run(self, trials):
+        """trials: {handle: run_trial keyword dict} for every resident network;
+        returns {handle: TrialResult}."""
+        hs = list(trials)
+        n = len(hs)
+        handles = np.asarray(hs, np.int32)
+        ts = (_abi.Trial * max(n, 1))()
+        outs = (_abi.TrialOut * max(n, 1))()
+        for k, h in enumerate(hs):
+            kw = trials[h]
+            ts[k] = _abi.Trial(int(kw.get("mode", 1)), int(bool(kw.get("training", True))),
+                               int(kw.get("trial_index", 0)) & 0xffffffff, int(bool(kw.get("first_trial", False))),
+                               float(kw.get("mean_injection_time", 100.0)))
+        st = self.lib.hvc_pool_run(self.ctx, n, _p(handles, _abi.c_int_p), ts, outs)
+        if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
+            self._check(st)
+        res = {}
+        for k, h in enumerate(hs):
+            net = self.resident[h]
+            res[h] = TrialResult(self._pool_spikes(h, 0, net.n_ra), self._pool_spikes(h, 1, net.n_ra),
+                                 self._pool_spikes(h, 2, net.n_i), outs[k].bad_values, tuple(outs[k].warnings))
+        return res
+
+    def download_synapses(self, handle) -> SynapseState:
+        net = self.resident[handle]
+        E, n = net.n_candidates, net.n_ra
+        w = np.zeros(max(E, 1))
+        ac, sc, rem = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        ai, si = np.zeros(max(E, 1), np.int32), np.zeros(max(n * max(1, self.params.syn.Nss), 1), np.int32)
+        s = _abi.Synapses(_p(w, _abi.c_double_p), _p(ac, _abi.c_int_p), _p(ai, _abi.c_int_p),
+                          _p(sc, _abi.c_int_p), _p(si, _abi.c_int_p), _p(rem, _abi.c_int_p))
+        self._check(self.lib.hvc_pool_download_synapses(self.ctx, handle, ctypes.byref(s)))
+        return SynapseState(w[:E], ac, ai[:int(ac.sum())].copy(), sc, si[:int(sc.sum())].copy(), rem)

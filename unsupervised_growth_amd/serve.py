@@ -1,0 +1,121 @@
+"""Serving loop: trial and chain-test requests on networks of mixed size,
+continuously batched on one device.
+
+Requests wait in arrival order; at every round boundary the loop admits the
+waiting requests whose networks fit into the pool (its capacities fix the HBM
+footprint), then runs one trial of every resident network together
+(hvc_pool_run: one launch per kernel and coupling window for all of them) and
+releases the networks whose requests are complete.  A network stays on the
+device from admission to release, so a growth request's synapse state evolves
+in place; a request that arrives while others run joins at the next round.
+Admission is first come, first served: a request that does not fit keeps the
+ones behind it waiting, so large networks are not starved.
+
+Each request's result equals what the same network and trials give alone (the
+pool keys every network's noise by its own key and ids and sums over its own
+neurons only; see tests/test_gpu_serve.py).
+"""
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Pool
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds after the start (rounds with a virtual clock)
+    net: object                    # Network
+    key: tuple                     # noise-stream key of the request
+    onsets: np.ndarray             # mean_injection_time of each trial (ms)
+    mode: int = 1                  # 1: chain test, 0: growth trials
+    first_trial_index: int = 0
+    handle: int = None
+    admitted: float = None
+    finished: float = None
+    results: list = field(default_factory=list)
+    synapses: object = None        # growth requests: synapse state at release
+
+    @property
+    def n_trials(self):
+        return len(self.onsets)
+
+    def trial_kw(self, k):
+        return dict(mode=self.mode, training=True, trial_index=self.first_trial_index + k,
+                    first_trial=(self.mode == 0 and k == 0), mean_injection_time=float(self.onsets[k]))
+
+
+class Server:
+    """Continuous batching of requests on a Pool."""
+
+    def __init__(self, pool: Pool):
+        self.pool = pool
+
+    def run(self, requests, realtime=True, keep_results=True, log=None):
+        """Serve the requests; with realtime the arrival times are seconds on the
+        wall clock, otherwise round numbers.  Returns statistics."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        active = {}
+        t0 = time.perf_counter()
+        rounds = 0
+        updates = 0
+        busy = 0.0
+
+        def now():
+            return (time.perf_counter() - t0) if realtime else float(rounds)
+
+        while pending or waiting or active:
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            while waiting:
+                req = waiting[0]
+                h = self.pool.admit(req.net, req.key)
+                if h is None:
+                    if not active:
+                        raise RuntimeError("request %d does not fit into an empty pool" % req.rid)
+                    break
+                waiting.popleft()
+                req.handle, req.admitted = h, now()
+                req.results = []
+                active[h] = req
+            if not active:
+                if realtime and pending:
+                    time.sleep(max(0.0, pending[0].arrival - now()))
+                elif not realtime:
+                    rounds += 1
+                continue
+            tr0 = time.perf_counter()
+            res = self.pool.run({h: r.trial_kw(len(r.results)) for h, r in active.items()})
+            busy += time.perf_counter() - tr0
+            rounds += 1
+            steps = int(round(self.pool.params.trial_ms / self.pool.params.dt)) - 1
+            for h, r in list(active.items()):
+                r.results.append(res[h] if keep_results else None)
+                updates += (r.net.n_ra + r.net.n_i) * steps
+                if len(r.results) == r.n_trials:
+                    if r.mode == 0:
+                        r.synapses = self.pool.download_synapses(h)
+                    self.pool.release(h)
+                    r.finished = now()
+                    del active[h]
+            if log:
+                log("round %d: %d resident, %d waiting, %d pending" % (rounds, len(active), len(waiting),
+                                                                       len(pending)))
+        wall = time.perf_counter() - t0
+        lat = np.array([r.finished - r.arrival for r in requests])
+        return {
+            "requests": len(requests),
+            "rounds": rounds,
+            "wall_s": wall,
+            "busy_s": busy,
+            "neuron_updates": updates,
+            "neuron_updates_per_s": updates / wall if wall > 0 else 0.0,
+            "latency_p50": float(np.percentile(lat, 50)),
+            "latency_p99": float(np.percentile(lat, 99)),
+            "latency_max": float(lat.max()),
+        }
