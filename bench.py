@@ -41,6 +41,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nets", type=int, default=64, help="c4: number of networks")
     ap.add_argument("--check-nets", type=int, default=4, help="c4: networks re-run alone for comparison")
+    ap.add_argument("--tune-windows", type=int, default=400, help="c4: coupling windows per autotuner candidate")
+    ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
+    ap.add_argument("--trials", type=int, default=50, help="c5: chain-test trials per request")
+    ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
+    ap.add_argument("--variants", type=int, default=2, help="c5: networks built per size")
+    ap.add_argument("--hbm-gib", type=float, default=8.0, help="c5: device memory of the serving pool")
     return ap.parse_args()
 
 
@@ -91,6 +97,9 @@ def bench_batch(args):
     precision = 0 if args.precision == "fp64" else 1
     eng = Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
     eng.upload_batch(nets, keys)
+    t_tune = time.perf_counter()
+    tuning, tune_ms = eng.autotune(windows=args.tune_windows) if args.tune_windows > 0 else (eng.tuning(), None)
+    t_tune = time.perf_counter() - t_tune
     steps_per_trial = int(round(kw["trial_ms"] / 0.02)) - 1
     n_neurons = sum(n.n_ra + n.n_i for n in nets)
 
@@ -170,9 +179,73 @@ def bench_batch(args):
                      "avg_launch_ms": kern["integrate"],
                      "kernels_ms_per_trial": {k: v * (1 if k == "decay" else windows) for k, v in kern.items()}},
         "batch_vs_alone": worst_alone,
+        "autotune": {"tuning": tuning, "s": round(t_tune, 2),
+                     "ms_per_window": None if tune_ms is None else [float(x) for x in tune_ms]},
     }
     if precision_check is not None:
         line["precision_check"] = precision_check
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Served form (configs c5): Poisson arrivals of chain-test requests on grown
+    networks of mixed size, continuously batched on one pool."""
+    import numpy as np
+    from unsupervised_growth_amd import configs
+    from unsupervised_growth_amd.engine import Pool, network_sizes, pool_caps
+    from unsupervised_growth_amd.serve import Request, Server
+
+    t_build = time.perf_counter()
+    nets = configs.c5_networks(variants=args.variants)
+    reqs0 = configs.c5_requests(nets, n_requests=args.requests, rate=args.rate, trials=args.trials)
+    t_build = time.perf_counter() - t_build
+    all_nets = [n for v in nets.values() for n in v]
+    sizes = [network_sizes(n) for n in all_nets]
+    indeg = 0
+    for n in all_nets:
+        indeg = max(indeg, int(np.bincount(n.i2ra_tgt, minlength=n.n_ra).max(initial=0)),
+                    int(np.bincount(n.ra2i_tgt, minlength=max(n.n_i, 1)).max(initial=0)),
+                    int(np.bincount(n.rr_tgt, minlength=n.n_ra).max(initial=0)))
+    # capacities from the HBM budget: neurons and connections in the workload's proportions
+    per_ra = {k: max(s[j] / s[0] for s in sizes) * 1.25 for j, k in ((1, "i"), (2, "rr"), (3, "a2i"), (4, "i2ra"))}
+    precision = 0 if args.precision == "fp64" else 1
+    kw = dict(trial_ms=500.0, precision=precision, ra_model=args.ra_model, i_model=args.i_model)
+    probe = pool_caps(1, 1, 1, 1, 1, 1, indeg + 8, 128)
+    from unsupervised_growth_amd.engine import make_params
+    import ctypes
+    from unsupervised_growth_amd import _abi
+    lib = _abi.load()
+    prm = make_params(**kw)
+    unit = ctypes.c_size_t()
+    lib.hvc_pool_footprint(ctypes.byref(prm), ctypes.byref(probe), 1000, int(1000 * per_ra["i"]),
+                           int(1000 * per_ra["rr"]), int(1000 * per_ra["a2i"]), int(1000 * per_ra["i2ra"]),
+                           ctypes.byref(unit), None)
+    max_ra = int(args.hbm_gib * (1 << 30) / (unit.value / 1000.0))
+    caps = pool_caps(64, max_ra, int(max_ra * per_ra["i"]), int(max_ra * per_ra["rr"]), int(max_ra * per_ra["a2i"]),
+                     int(max_ra * per_ra["i2ra"]), indeg + 8, 128)
+    out = {}
+    with Pool(caps, **kw) as pool:
+        reqs = [Request(rid=k, arrival=a, net=n, key=key, onsets=on, mode=1, first_trial_index=0)
+                for k, (a, n, key, on) in enumerate(reqs0)]
+        stats = Server(pool).run(reqs, realtime=True, keep_results=False)
+        out = dict(stats, pool_gib=pool.pool_bytes() / (1 << 30))
+    lat = sorted(r.finished - r.arrival for r in reqs)
+    line = {
+        "metric": "served chain-test requests: p50/p99 latency and neuron-updates/s",
+        "value": out["neuron_updates_per_s"], "unit": "neuron-updates/s", "n_gpus": 1,
+        "steps": args.requests, "warmup": 0, "ms_per_step": 1000.0 * out["wall_s"] / args.requests,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == 0 else "f32",
+        "data": "synthetic grown chain networks, Poisson arrivals (seeded)",
+        "config": {"workload": "%d chain-test requests x %d trials of 500 ms, Poisson arrivals %g/s, N_RA 1000/4000/20000 "
+                               "(25/50/25 %%), N_I = 0.275 N_RA, continuous batching on a %.1f GiB pool"
+                               % (args.requests, args.trials, args.rate, out["pool_gib"]),
+                   "config": "c5", "network_build_s": round(t_build, 1)},
+        "latency_s": {"p50": out["latency_p50"], "p99": out["latency_p99"], "max": out["latency_max"],
+                      "min": lat[0]},
+        "requests_per_s": args.requests / out["wall_s"],
+        "rounds": out["rounds"], "busy_fraction": out["busy_s"] / out["wall_s"],
+    }
     print(json.dumps(line), flush=True)
 
 
@@ -184,6 +257,9 @@ def main():
         sys.exit(2)
     if args.config == "c4":
         bench_batch(args)
+        return
+    if args.config == "c5":
+        bench_serve(args)
         return
     from unsupervised_growth_amd import configs, opcount
     from unsupervised_growth_amd.engine import Engine

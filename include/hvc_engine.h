@@ -158,6 +158,23 @@ hvc_status hvc_batch_spikes(hvc_ctx* ctx, int net, int kind, int* counts, double
 hvc_status hvc_batch_download_synapses(hvc_ctx* ctx, int net, hvc_synapses* syn);
 hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* syn);
 
+/* Launch shapes of the per-window kernels.  hvc_autotune times the candidates
+ * on the resident networks (a scratch chain test through the first `windows`
+ * coupling windows, kicked early so that activity propagates; synapse state is
+ * not touched and the next trial starts from rest as always), keeps the fastest
+ * and returns it with its per-window device times (ms) of integrate, deliver
+ * and sync.  Results do not depend on the launch shapes.  No counterpart in the
+ * reference. */
+typedef struct {
+    int integrate_block;   /* threads per block of the integration kernel: 64, 128, 256 */
+    int i_lanes;           /* HVC-I DP8: 1 = five lanes per neuron, 0 = one thread per neuron */
+    int deliver_block;     /* 64, 128, 256 */
+    int sync_block;        /* 64 .. 512 */
+} hvc_tuning;
+hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* tuning);
+hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* tuning);
+hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, double* ms);
+
 /* Serving pool: a context whose networks join and leave between trials.  The
  * resident networks are packed side by side in device memory (a release moves
  * the networks behind it down, on the device, with their ids and edge

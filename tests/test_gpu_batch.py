@@ -61,3 +61,47 @@ def test_batch_equals_each_network_alone(precision):
         for k in ("w", "act_cnt", "act_ids", "sup_cnt", "sup_ids", "remodeled"):
             np.testing.assert_array_equal(getattr(syns[b], k), getattr(syn, k), err_msg="net %d %s" % (b, k))
     assert spikes > 0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_launch_shapes_do_not_change_results(precision):
+    """Every launch shape the autotuner may pick gives the same bits."""
+    from unsupervised_growth_amd.engine import Engine
+    nets = _nets()
+    keys = [(100 + b, 7) for b in range(len(nets))]
+    shapes = [dict(integrate_block=64, i_lanes=1, deliver_block=64, sync_block=256),
+              dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512),
+              dict(integrate_block=128, i_lanes=1, deliver_block=256, sync_block=128)]
+    runs = []
+    for shape in shapes:
+        with Engine(precision=precision, trial_ms=200.0) as e:
+            e.upload_batch(nets, keys)
+            e.set_tuning(**shape)
+            assert e.tuning() == shape
+            res = [e.run_batch([_trial(b, tr) for b in range(len(nets))]) for tr in range(2)]
+            res.append(e.run_batch([_trial(b, 2, mode=1) for b in range(len(nets))]))
+            runs.append((res, [e.download_synapses(b) for b in range(len(nets))]))
+    for res, syns in runs[1:]:
+        for tr in range(3):
+            for b in range(len(nets)):
+                for kind in ("soma", "dend", "interneuron"):
+                    _same(getattr(res[tr][b], kind), getattr(runs[0][0][tr][b], kind), "trial %d net %d" % (tr, b))
+        for b in range(len(nets)):
+            np.testing.assert_array_equal(syns[b].w, runs[0][1][b].w)
+
+
+def test_autotune_keeps_a_valid_shape_and_results():
+    from unsupervised_growth_amd.engine import Engine
+    nets = _nets()
+    keys = [(100 + b, 7) for b in range(len(nets))]
+    with Engine(precision=1, trial_ms=200.0) as e:
+        e.upload_batch(nets, keys)
+        ref = e.run_batch([_trial(b, 0, mode=1) for b in range(len(nets))])
+        shape, ms = e.autotune(windows=60)
+        print("autotune:", shape, ms, flush=True)
+        assert shape["integrate_block"] in (64, 128, 256) and shape["deliver_block"] in (64, 128, 256)
+        assert shape["sync_block"] in (128, 256, 512) and np.all(ms > 0)
+        again = e.run_batch([_trial(b, 0, mode=1) for b in range(len(nets))])
+    for b in range(len(nets)):
+        for kind in ("soma", "dend", "interneuron"):
+            _same(getattr(again[b], kind), getattr(ref[b], kind), "net %d %s" % (b, kind))

@@ -24,7 +24,8 @@ EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_sy
            "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_upload_batch", "hvc_run_batch",
            "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses", "hvc_pool_footprint",
            "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
-           "hvc_pool_download_synapses", "hvc_destroy", "hvc_last_error"]
+           "hvc_pool_download_synapses", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
+           "hvc_last_error"]
 
 
 class EngineNotBuilt(RuntimeError):
@@ -80,6 +81,11 @@ class PoolCaps(ctypes.Structure):
                 ("max_in_degree", ctypes.c_int), ("spikes_per_neuron", ctypes.c_int)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("integrate_block", ctypes.c_int), ("i_lanes", ctypes.c_int), ("deliver_block", ctypes.c_int),
+                ("sync_block", ctypes.c_int)]
+
+
 class Synapses(ctypes.Structure):
     _fields_ = [("w", c_double_p), ("act_cnt", c_int_p), ("act_ids", c_int_p), ("sup_cnt", c_int_p),
                 ("sup_ids", c_int_p), ("remodeled", c_int_p)]
@@ -125,6 +131,9 @@ def load(path=LIB_PATH):
     lib.hvc_pool_run.argtypes = [vp, ctypes.c_int, c_int_p, P(Trial), P(TrialOut)]
     lib.hvc_pool_spikes.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
     lib.hvc_pool_download_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
+    lib.hvc_get_tuning.argtypes = [vp, P(Tuning)]
+    lib.hvc_set_tuning.argtypes = [vp, P(Tuning)]
+    lib.hvc_autotune.argtypes = [vp, ctypes.c_int, P(Tuning), c_double_p]
     lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
     lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
     lib.hvc_eval_math.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, c_double_p]

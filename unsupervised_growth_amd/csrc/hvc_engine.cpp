@@ -27,15 +27,15 @@ void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint3
 void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
 void launch_reset(hipStream_t st, const NetDev& nd, int E);
-void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1);
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
 void launch_trace_ra_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
                          const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
                          uint8_t* fired);
 void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                         const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
-void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1);
-void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1);
-void launch_sync(hipStream_t st, const NetDev& nd);
+void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
+void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
+void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu);
 void launch_sort_posts(hipStream_t st, const NetDev& nd);
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta);
 void launch_decay(hipStream_t st, const NetDev& nd);
@@ -111,6 +111,8 @@ struct hvc_ctx {
     int* d_perm = nullptr;
     int *d_a_ptr = nullptr, *d_a_tgt = nullptr, *d_b_ptr = nullptr, *d_b_tgt = nullptr;
     double *d_a_w = nullptr, *d_a_d = nullptr, *d_b_w = nullptr, *d_b_d = nullptr;
+    Tuning tuning;
+
     // serving pool
     bool pool = false;
     std::vector<PoolNet> pnets;
@@ -1001,16 +1003,16 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1);
+            launch_integrate_f32(s, nd, t0, t1, ctx->tuning);
         else
-            launch_integrate(s, nd, t0, t1);
+            launch_integrate(s, nd, t0, t1, ctx->tuning);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
         if (any_growth) launch_sort_posts(s, nd);
-        launch_deliver(s, nd, t0, t1);
+        launch_deliver(s, nd, t0, t1, ctx->tuning);
         tstop(ctx, kTDeliver, tm);
         tstart(ctx, kTSync, tm);
-        launch_sync(s, nd);
+        launch_sync(s, nd, ctx->tuning);
         tstop(ctx, kTSync, tm);
     }
     if (any_growth) {
@@ -1096,6 +1098,150 @@ extern "C" hvc_status hvc_run_batch(hvc_ctx* ctx, const hvc_trial* trials, hvc_t
     if (!ctx || !trials) return HVC_E_ARG;
     if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload networks first");
     return run_impl(ctx, trials, outs);
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: launch shapes of the per-window kernels for the resident networks
+// ---------------------------------------------------------------------------
+static hvc_status check_tuning(hvc_ctx* ctx, const hvc_tuning* t)
+{
+    auto ok_block = [](int b, int hi) { return b >= 64 && b <= hi && (b % 64) == 0; };
+    if (!ok_block(t->integrate_block, 256) || !ok_block(t->deliver_block, 256) || !ok_block(t->sync_block, 512) ||
+        (t->i_lanes != 0 && t->i_lanes != 1))
+        return fail(ctx, HVC_E_ARG, "tuning out of range (blocks: multiples of 64 up to 256, sync up to 512)");
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* t)
+{
+    if (!ctx || !t) return HVC_E_ARG;
+    t->integrate_block = ctx->tuning.integrate_block;
+    t->i_lanes = ctx->tuning.i_lanes;
+    t->deliver_block = ctx->tuning.deliver_block;
+    t->sync_block = ctx->tuning.sync_block;
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* t)
+{
+    if (!ctx || !t) return HVC_E_ARG;
+    hvc_status r = check_tuning(ctx, t);
+    if (r != HVC_OK) return r;
+    ctx->tuning.integrate_block = t->integrate_block;
+    ctx->tuning.i_lanes = t->i_lanes;
+    ctx->tuning.deliver_block = t->deliver_block;
+    ctx->tuning.sync_block = t->sync_block;
+    return HVC_OK;
+}
+
+// A scratch chain test of every resident network (synapse state untouched, the
+// training kick 5 ms after the start so that chains are active) through the
+// first `windows` coupling windows; average device time per window of each
+// kernel class under tuning tu.
+static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, double ms[3])
+{
+    NetDev& nd = ctx->nd;
+    hipStream_t s = ctx->stream;
+    const int B = ctx->n_nets;
+    ctx->h_kick.assign(ctx->n_ra, -1);
+    for (size_t k = 0; k < ctx->train_ids.size(); ++k) {
+        double thr = ctx->spread[k] + 5.0;
+        for (int t = 1; t < ctx->num_steps; ++t)
+            if (ctx->clock[t] > thr) {
+                ctx->h_kick[ctx->train_ids[k]] = t;
+                break;
+            }
+    }
+    std::vector<int> zeros(B, 0);
+    std::vector<uint32_t> trial(B, 0);
+    HVC_TRY(hipMemcpyAsync(ctx->d_kick, ctx->h_kick.data(), sizeof(int) * ctx->n_ra, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_growth, zeros.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_first, zeros.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
+    launch_reset(s, nd, ctx->E);
+    launch_chain_edges(s, nd);
+    const int nw = std::min(windows, (int)ctx->windows.size());
+    std::vector<hipEvent_t> ev(6 * (size_t)nw);
+    for (auto& e : ev) HVC_TRY(hipEventCreate(&e));
+    hvc_status r = HVC_OK;
+    for (int w = 0; w < nw; ++w) {
+        int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
+        hipEventRecord(ev[6 * w + 0], s);
+        if (ctx->prm.precision == 1)
+            launch_integrate_f32(s, nd, t0, t1, tu);
+        else
+            launch_integrate(s, nd, t0, t1, tu);
+        hipEventRecord(ev[6 * w + 1], s);
+        launch_deliver(s, nd, t0, t1, tu);
+        hipEventRecord(ev[6 * w + 2], s);
+        hipEventRecord(ev[6 * w + 3], s);
+        launch_sync(s, nd, tu);
+        hipEventRecord(ev[6 * w + 4], s);
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) r = fail(ctx, HVC_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    ms[0] = ms[1] = ms[2] = 0;
+    for (int w = 0; w < nw && r == HVC_OK; ++w) {
+        float a = 0, b = 0, c = 0;
+        hipEventElapsedTime(&a, ev[6 * w + 0], ev[6 * w + 1]);
+        hipEventElapsedTime(&b, ev[6 * w + 1], ev[6 * w + 2]);
+        hipEventElapsedTime(&c, ev[6 * w + 3], ev[6 * w + 4]);
+        ms[0] += a; ms[1] += b; ms[2] += c;
+    }
+    for (auto& x : ev) hipEventDestroy(x);
+    for (int k = 0; k < 3; ++k) ms[k] /= std::max(nw, 1);
+    unsigned int err = 0;
+    HVC_TRY(hipMemcpy(&err, nd.err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err && r == HVC_OK) r = fail(ctx, HVC_E_CAPACITY, "autotune: engine capacity exceeded in the scratch run");
+    ctx->trial_done = false;
+    return r;
+}
+
+extern "C" hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, double* ms_out)
+{
+    if (!ctx || windows <= 0) return HVC_E_ARG;
+    if (!ctx->uploaded || ctx->n_nets == 0) return fail(ctx, HVC_E_STATE, "upload networks first");
+    HVC_TRY(hipSetDevice(ctx->device));
+    Tuning tu = ctx->tuning;
+    double ms[3], best_ms[3];
+    hvc_status r;
+    // integrate: block size and the HVC-I layout
+    double bi = 1e30;
+    Tuning ti = tu;
+    // five lanes per HVC-I neuron exist for DP8 only
+    for (int lanes = (ctx->prm.i_model == 0 ? 1 : 0); lanes >= 0; --lanes) {
+        for (int bs : {64, 128, 256}) {
+            Tuning c = tu;
+            c.integrate_block = bs;
+            c.i_lanes = lanes;
+            if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
+            if (ms[0] < bi) { bi = ms[0]; ti = c; }
+        }
+    }
+    tu = ti;
+    double bd = 1e30;
+    Tuning td = tu;
+    for (int bs : {64, 128, 256}) {
+        Tuning c = tu;
+        c.deliver_block = bs;
+        if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
+        if (ms[1] < bd) { bd = ms[1]; td = c; }
+    }
+    tu = td;
+    double bsy = 1e30;
+    Tuning ts = tu;
+    for (int bs : {128, 256, 512}) {
+        Tuning c = tu;
+        c.sync_block = bs;
+        if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
+        if (ms[2] < bsy) { bsy = ms[2]; ts = c; }
+    }
+    tu = ts;
+    if ((r = timed_windows(ctx, windows, tu, best_ms)) != HVC_OK) return r;
+    ctx->tuning = tu;
+    if (best) hvc_get_tuning(ctx, best);
+    if (ms_out) for (int k = 0; k < 3; ++k) ms_out[k] = best_ms[k];
+    return HVC_OK;
 }
 
 // ---------------------------------------------------------------------------

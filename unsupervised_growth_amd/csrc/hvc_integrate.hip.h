@@ -324,6 +324,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
 }
 
 // HVC-I, one lane per neuron (RK4 form)
+template <int MODEL>
 __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
 {
     const bool live = i < nd.n_i;
@@ -345,7 +346,7 @@ __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
             i_noise_check(ps, time, nd.dt, s.Ge, ne);
             i_noise_check(ps, time, nd.dt, s.Gi, ni);
         }
-        i_step<1>(s, (real)nd.dt, rows[t]);
+        i_step<MODEL>(s, (real)nd.dt, rows[t]);
         bool fired = live && f;
         if (fired) mask |= 1ull << (t - t0);
         wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
@@ -423,17 +424,18 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
     }
 }
 
-// one 64-lane wave per block; blocks [0, iblocks) integrate HVC-I, the rest HVC-RA
+// blocks [0, iblocks) integrate HVC-I (five lanes per neuron, 12 neurons per
+// wave, or one thread per neuron), the rest HVC-RA (one thread per neuron)
 template <int RA_MODEL, int I_MODEL>
-__global__ __launch_bounds__(64) void k_integrate(NetDev nd, int t0, int t1, int iblocks)
+__global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks, int i_lanes)
 {
     if ((int)blockIdx.x < iblocks) {
-        if (I_MODEL == 0)
-            integrate_i_lanes(nd, blockIdx.x, t0, t1);
+        if (I_MODEL == 0 && i_lanes)
+            integrate_i_lanes(nd, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
         else
-            integrate_i_thread(nd, blockIdx.x * 64 + threadIdx.x, t0, t1);
+            integrate_i_thread<I_MODEL>(nd, blockIdx.x * blockDim.x + threadIdx.x, t0, t1);
     } else {
-        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * 64 + threadIdx.x, t0, t1);
+        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * blockDim.x + threadIdx.x, t0, t1);
     }
 }
 
@@ -458,20 +460,22 @@ void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n
     k_trace_i<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, first_trial, trial, exc_g, inh_g, out, fired);
 }
 
-void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1)
+void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
-    int iblocks = nd.n_i == 0 ? 0 : (nd.i_model == 0 ? (nd.n_i + kINeuronsPerWave - 1) / kINeuronsPerWave
-                                                     : blocks_for(nd.n_i, 64));
-    int rblocks = blocks_for(nd.n_ra, 64);
+    const int bs = tu.integrate_block;
+    const int lanes = (nd.i_model == 0 && tu.i_lanes) ? 1 : 0;
+    const int i_per_block = lanes ? (bs / 64) * kINeuronsPerWave : bs;
+    int iblocks = nd.n_i == 0 ? 0 : (nd.n_i + i_per_block - 1) / i_per_block;
+    int rblocks = blocks_for(nd.n_ra, bs);
     dim3 grid(iblocks + rblocks);
     if (nd.ra_model == 0 && nd.i_model == 0)
-        k_integrate<0, 0><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
+        k_integrate<0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
     else if (nd.ra_model == 0)
-        k_integrate<0, 1><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
+        k_integrate<0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
     else if (nd.i_model == 0)
-        k_integrate<1, 0><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
+        k_integrate<1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
     else
-        k_integrate<1, 1><<<grid, 64, 0, st>>>(nd, t0, t1, iblocks);
+        k_integrate<1, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
 }
 
 #undef HVC_LAUNCHER

@@ -40,6 +40,14 @@ struct NoiseId {
     uint32_t k0, k1, trial, local;
 };
 
+// launch shapes of the per-window kernels (chosen by the autotuner)
+struct Tuning {
+    int integrate_block = 64;   // threads per block of k_integrate (64, 128, 256)
+    int i_lanes = 1;            // HVC-I DP8: 1 = five lanes per neuron, 0 = one thread per neuron
+    int deliver_block = 64;     // threads per block of k_deliver
+    int sync_block = 256;       // threads per block of k_sync
+};
+
 struct Record {
     uint64_t key;   // (arrival step << 32) | presynaptic id
     double w;

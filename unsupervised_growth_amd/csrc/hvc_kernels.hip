@@ -564,7 +564,7 @@ __device__ __forceinline__ double inbox_sum(const Inbox& ib, int tgt, int n)
 }
 
 // sync: apply the window's conductance updates (NGS:9309-9366)
-__global__ __launch_bounds__(256) void k_sync(NetDev nd)
+__global__ __launch_bounds__(512) void k_sync(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) *nd.post_list_cnt = 0;  // the window's post-event list has been consumed
@@ -668,11 +668,11 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
     if (E > 0) k_clear_pre<<<blocks_for(E, 256), 256, 0, st>>>(nd.pre_cnt, E);
 }
 
-void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1)
+void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
     int nmax = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
-    int blocks_ra = blocks_for(nmax, 64);
-    k_deliver<<<3 * blocks_ra, 64, 0, st>>>(nd, t0, t1, blocks_ra);
+    int blocks_ra = blocks_for(nmax, tu.deliver_block);
+    k_deliver<<<3 * blocks_ra, tu.deliver_block, 0, st>>>(nd, t0, t1, blocks_ra);
 }
 
 __global__ void k_add_i32(int* a, long long n, int delta)
@@ -691,10 +691,10 @@ void launch_sort_posts(hipStream_t st, const NetDev& nd)
     k_sort_posts<<<1, 1024, 0, st>>>(nd);
 }
 
-void launch_sync(hipStream_t st, const NetDev& nd)
+void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu)
 {
     int n = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
-    k_sync<<<blocks_for(n, 256), 256, 0, st>>>(nd);
+    k_sync<<<blocks_for(n, tu.sync_block), tu.sync_block, 0, st>>>(nd);
 }
 
 void launch_decay(hipStream_t st, const NetDev& nd)

@@ -280,6 +280,23 @@ class Engine:
         self._check(self.lib.hvc_eval_math(self.ctx, fn, len(a), _p(a, dp), _p(b, dp), _p(out, dp)))
         return out
 
+    def tuning(self):
+        t = _abi.Tuning()
+        self._check(self.lib.hvc_get_tuning(self.ctx, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
+    def set_tuning(self, **kw):
+        t = _abi.Tuning(**{**self.tuning(), **kw})
+        self._check(self.lib.hvc_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def autotune(self, windows=200):
+        """Time the launch-shape candidates on the resident networks and keep the
+        fastest; returns (tuning, per-window ms of integrate, deliver, sync)."""
+        t = _abi.Tuning()
+        ms = np.zeros(3)
+        self._check(self.lib.hvc_autotune(self.ctx, windows, ctypes.byref(t), _p(ms, _abi.c_double_p)))
+        return {f: getattr(t, f) for f, _ in t._fields_}, ms
+
     def set_timing(self, on=True):
         self._check(self.lib.hvc_set_timing(self.ctx, int(bool(on))))
 
