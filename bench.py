@@ -5,14 +5,19 @@ A step is one growth trial (500 ms, 24 999 neuron steps, STDP on) of the
 20 000 HVC-RA + 5 500 HVC-I network (configs.c3): 100 RA->RA candidates per
 RA by distance, delays U(0, 10) ms, RK4 form (HH2_buffer::Runge4_step for
 HVC-RA; HVC-I use HHI_buffer::DP8_step because HHI_buffer::Runge4_step
-diverges at the reference's dt = 0.02 ms).
+diverges at the reference's dt = 0.02 ms), in the precision form of that
+configuration: FP32 neuron state, FP64 clock, spike times and STDP.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|fp32] [--config c3]
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64] [--config c1|c2|c3]
+    python bench.py --config c4 [--precision fp32|fp64]     # batched form
+    python bench.py --config c5 [--precision fp32|fp64]     # served form
 
 Prints one JSON line: neuron-updates/s = (N_RA + N_I) * 24 999 * K / wall time,
 with inputs resident on the device, plus the roofline of the dominant kernel
-(HIP-event durations on the engine stream) and a CPU baseline (the oracle's
-restatement of the same trial, one host thread, on a bounded sample).
+(HIP-event durations on the engine stream), a CPU baseline (the oracle's
+restatement of the same trial, one host thread, on a bounded sample) and, for
+FP32, the per-trial spike-time error against the FP64 form (bit-exact with the
+oracle) and the FP64 form's own throughput on the same trials.
 """
 import argparse
 import json
@@ -26,6 +31,11 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
 
+# HBM bytes per launch of the integration kernel, from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
+# profiles/integrate_c3_hbm_traffic.txt; keyed by (config, precision, HVC-I lanes)
+TRAFFIC_PROFILED = {("c3", 1, 1): 6.588e6, ("c3", 0, 1): 7.105e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -33,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--ra-model", type=int, default=1, help="0 Debraband (drop-in), 1 RK4 (HH2_buffer)")
     ap.add_argument("--i-model", type=int, default=0, help="0 DP8, 1 RK4")
     ap.add_argument("--cpu-sample-ms", type=float, default=5.0,
@@ -41,7 +51,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nets", type=int, default=64, help="c4: number of networks")
     ap.add_argument("--check-nets", type=int, default=4, help="c4: networks re-run alone for comparison")
-    ap.add_argument("--tune-windows", type=int, default=400, help="c4: coupling windows per autotuner candidate")
+    ap.add_argument("--trial-ms", type=float, default=None, help="trial length (default: the config's)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="read and write this many MiB in one elementwise kernel before the trials "
+                         "(calibrates the HBM byte counters)")
+    ap.add_argument("--tune-windows", type=int, default=400, help="coupling windows per autotuner candidate (0: no tuning)")
+    ap.add_argument("--shape", default=None,
+                    help="launch shape integrate_block,i_lanes,deliver_block,sync_block (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
     ap.add_argument("--trials", type=int, default=50, help="c5: chain-test trials per request")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
@@ -67,22 +83,15 @@ def cpu_baseline(net, kw, args):
                       % (sample_ms, steps, net.n_ra, net.n_i, el)}
 
 
-def fp32_check(net, kw, args, syn0, trial_index, r32):
-    """Spike-time error of the FP32 form: the first timed trial again in the FP64
-    form (bit-exact with the oracle) from the same synapse state."""
-    from unsupervised_growth_amd.engine import Engine
-    from unsupervised_growth_amd.spikes import compare_results
-    with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=0, **kw) as e64:
-        e64.upload(net)
-        e64.upload_synapses(syn0)
-        r64 = e64.run_trial(mode=0, training=True, trial_index=trial_index, first_trial=(trial_index == 0),
-                            mean_injection_time=100.0)
-    c = compare_results(r64, r32, window=1.0)
-    return {"reference": "FP64 form (bit-exact with the oracle), same trial and synapse state",
-            "max_abs_dt_ms": max(v["max_abs_dt_ms"] for v in c.values()),
-            "unpaired_spikes": sum(v["unpaired"] for v in c.values()),
-            "paired_within_one_step": sum(v["paired_within_one_step"] for v in c.values()),
-            "spikes": sum(v["spikes_ref"] for v in c.values()), "dt_ms": 0.02, "per_kind": c}
+def apply_shape(eng, args):
+    """The launch shape: given on the command line, else chosen by the autotuner."""
+    if args.shape:
+        ib, lanes, db, sb = (int(x) for x in args.shape.split(","))
+        eng.set_tuning(integrate_block=ib, i_lanes=lanes, deliver_block=db, sync_block=sb)
+    elif args.tune_windows > 0:
+        shape, ms = eng.autotune(windows=args.tune_windows)
+        eng.tune_ms = ms
+    return eng.tuning()
 
 
 def bench_batch(args):
@@ -98,7 +107,8 @@ def bench_batch(args):
     eng = Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
     eng.upload_batch(nets, keys)
     t_tune = time.perf_counter()
-    tuning, tune_ms = eng.autotune(windows=args.tune_windows) if args.tune_windows > 0 else (eng.tuning(), None)
+    tuning = apply_shape(eng, args)
+    tune_ms = getattr(eng, "tune_ms", None)
     t_tune = time.perf_counter() - t_tune
     steps_per_trial = int(round(kw["trial_ms"] / 0.02)) - 1
     n_neurons = sum(n.n_ra + n.n_i for n in nets)
@@ -263,45 +273,58 @@ def main():
         return
     from unsupervised_growth_amd import configs, opcount
     from unsupervised_growth_amd.engine import Engine
+    from unsupervised_growth_amd.spikes import compare_results
 
     t_build = time.perf_counter()
     net, kw = configs.CONFIGS[args.config]()
     t_build = time.perf_counter() - t_build
+    if args.trial_ms:
+        kw["trial_ms"] = args.trial_ms
     precision = 0 if args.precision == "fp64" else 1
     eng = Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
     eng.upload(net)
-    steps_per_trial = int(kw["trial_ms"] / 0.02) - 1
+    tuning = apply_shape(eng, args)
+    if args.copy_probe_mib > 0:
+        # an elementwise kernel of known traffic (the engine's exp over N bytes of
+        # doubles: reads and writes N bytes, N far above the L2 size): its byte
+        # counters scale the kernels'
+        import numpy as np
+        eng.eval_math(0, np.zeros(args.copy_probe_mib << 17))
+    steps_per_trial = int(round(kw["trial_ms"] / 0.02)) - 1
+
+    def trial_kw(tr):
+        return dict(mode=0, training=True, trial_index=tr, first_trial=(tr == 0), mean_injection_time=100.0)
 
     trial = 0
     for _ in range(args.warmup):
-        eng.run_trial(mode=0, training=True, trial_index=trial, first_trial=(trial == 0),
-                      mean_injection_time=100.0)
+        eng.run_trial(**trial_kw(trial))
         trial += 1
     eng.set_timing(True)
-    syn0 = eng.download_synapses() if precision == 1 else None
     first_timed = trial
-    t0 = time.perf_counter()
+    wall = 0.0
     spikes = 0
-    results = []
+    results, states = [], []
     for _ in range(args.steps):
-        r = eng.run_trial(mode=0, training=True, trial_index=trial, first_trial=(trial == 0),
-                          mean_injection_time=100.0)
+        # the synapse state each trial starts from (outside the timed region), so
+        # that the FP64 form can repeat the trial from the same state
+        if precision == 1:
+            states.append(eng.download_synapses())
+        t0 = time.perf_counter()
+        r = eng.run_trial(**trial_kw(trial))
+        wall += time.perf_counter() - t0
         spikes += sum(map(len, r.soma)) + sum(map(len, r.interneuron))
         results.append(r)
         trial += 1
-    wall = time.perf_counter() - t0
-    precision_check = None
-    if precision == 1:
-        precision_check = fp32_check(net, kw, args, syn0, first_timed, results[0])
     ms_per_step = 1000.0 * wall / args.steps
     value = (net.n_ra + net.n_i) * steps_per_trial * args.steps / wall
 
     # roofline of the dominant kernel (HIP events on the engine stream)
-    windows = 2000 if kw["trial_ms"] == 500.0 else int(kw["trial_ms"] / 0.25)
+    windows = 2000 if kw["trial_ms"] == 500.0 else int(round(kw["trial_ms"] / 0.25))
     kern = {}
     for name in ("integrate", "deliver", "sync", "decay"):
         avg, n = eng.kernel_timing(name)
         kern[name] = {"avg_ms": avg, "samples": n}
+    eng.close()
     per_trial = {k: v["avg_ms"] * (1 if k == "decay" else windows) for k, v in kern.items()}
     dom = max(per_trial, key=per_trial.get)
     ra_flops = opcount.flops_per_step("ra", args.ra_model)
@@ -311,13 +334,45 @@ def main():
     avg_s = kern["integrate"]["avg_ms"] / 1000.0
     peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
     achieved = flops_launch / avg_s / 1e12 if avg_s > 0 else 0.0
+    traffic = TRAFFIC_PROFILED.get((args.config, precision, tuning["i_lanes"]))
     roofline = {"bound": "mfma", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "note": "VALU-bound (no dense contraction); peak = MI355X %s vector/matrix rate; achieved = "
-                        "algorithmic flops per launch (opcount: %d per RA step, %d per I step) / HIP-event "
-                        "launch duration" % ("FP64" if precision == 0 else "FP32", ra_flops, i_flops),
+                "frac": achieved / peak, "traffic": traffic,
+                "traffic_note": None if traffic is None else "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                                "(scale x2.0 / x1.0 from a kernel of known traffic), profiles/integrate_c3_hbm_traffic.txt; "
+                                "algorithmic state traffic ~7.5 MB per launch",
+                "note": "VALU-bound (no dense contraction); peak = MI355X %s vector rate; achieved = "
+                        "algorithmic flops per launch (opcount: %d per RA step, %d per I step, exp = 20) / "
+                        "HIP-event launch duration" % ("FP64" if precision == 0 else "FP32", ra_flops, i_flops),
                 "flops_per_launch": flops_launch, "avg_launch_ms": kern["integrate"]["avg_ms"],
                 "dominant": dom, "kernels_ms_per_trial": per_trial}
+
+    # FP32 form: each timed trial again in the FP64 form (bit-exact with the
+    # oracle) from the same synapse state; per-trial spike-time error
+    precision_check = None
+    fp64_form = None
+    if precision == 1:
+        per = []
+        wall64 = 0.0
+        with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=0, **kw) as e64:
+            e64.upload(net)
+            apply_shape(e64, args)
+            for k, syn in enumerate(states):
+                e64.upload_synapses(syn)
+                t0 = time.perf_counter()
+                r64 = e64.run_trial(**trial_kw(first_timed + k))
+                wall64 += time.perf_counter() - t0
+                c = compare_results(r64, results[k], window=1.0)
+                per.append({"trial": first_timed + k,
+                            "max_abs_dt_ms": max(v["max_abs_dt_ms"] for v in c.values()),
+                            "unpaired_spikes": sum(v["unpaired"] for v in c.values()),
+                            "paired_within_one_step": sum(v["paired_within_one_step"] for v in c.values()),
+                            "spikes": sum(v["spikes_ref"] for v in c.values())})
+        precision_check = {"reference": "FP64 form (bit-exact with the oracle), each trial from the same synapse state",
+                           "dt_ms": 0.02, "max_abs_dt_ms": max(p["max_abs_dt_ms"] for p in per),
+                           "unpaired_spikes": sum(p["unpaired_spikes"] for p in per), "per_trial": per}
+        fp64_form = {"value": (net.n_ra + net.n_i) * steps_per_trial * len(states) / wall64,
+                     "ms_per_step": 1000.0 * wall64 / len(states),
+                     "note": "the same trials in the plain FP64 GPU form (bit-exact with the oracle)"}
     cpu = None if args.no_cpu_baseline else cpu_baseline(net, kw, args)
     line = {
         "metric": "RK4 neuron-updates/s at 20k RA + 5.5k I; p50/p99 served trial latency",
@@ -326,18 +381,19 @@ def main():
         "dtype": "f64" if precision == 0 else "f32",
         "data": "synthetic network (distance rules, seeded), counter-based noise streams",
         "config": {"workload": "growth trial, %d HVC-RA + %d HVC-I, %d RA->RA candidates, %g ms trials, "
-                               "RA %s + I %s, STDP on" % (net.n_ra, net.n_i, net.n_candidates, kw["trial_ms"],
-                                                          "RK4-3/8" if args.ra_model else "Debraband",
-                                                          "DP8" if args.i_model == 0 else "RK4"),
-                   "config": args.config, "network_build_s": round(t_build, 1), "spikes_per_trial": spikes / args.steps},
+                               "RA %s + I %s, STDP on, %s state" % (
+                                   net.n_ra, net.n_i, net.n_candidates, kw["trial_ms"],
+                                   "RK4-3/8" if args.ra_model else "Debraband",
+                                   "DP8" if args.i_model == 0 else "RK4", "FP64" if precision == 0 else "FP32"),
+                   "config": args.config, "network_build_s": round(t_build, 1),
+                   "spikes_per_trial": spikes / args.steps, "tuning": tuning},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if precision_check is not None:
         line["precision_check"] = precision_check
+        line["fp64_form"] = fp64_form
     print(json.dumps(line), flush=True)
-    eng.close()
-
 
 if __name__ == "__main__":
     main()

@@ -426,11 +426,13 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
 
 // blocks [0, iblocks) integrate HVC-I (five lanes per neuron, 12 neurons per
 // wave, or one thread per neuron), the rest HVC-RA (one thread per neuron)
-template <int RA_MODEL, int I_MODEL>
-__global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks, int i_lanes)
+// (the HVC-I layout is a template parameter: a kernel holds the register
+// allocation of one layout only)
+template <int RA_MODEL, int I_MODEL, int LANES>
+__global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks)
 {
     if ((int)blockIdx.x < iblocks) {
-        if (I_MODEL == 0 && i_lanes)
+        if (LANES)
             integrate_i_lanes(nd, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
         else
             integrate_i_thread<I_MODEL>(nd, blockIdx.x * blockDim.x + threadIdx.x, t0, t1);
@@ -468,14 +470,21 @@ void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, in
     int iblocks = nd.n_i == 0 ? 0 : (nd.n_i + i_per_block - 1) / i_per_block;
     int rblocks = blocks_for(nd.n_ra, bs);
     dim3 grid(iblocks + rblocks);
-    if (nd.ra_model == 0 && nd.i_model == 0)
-        k_integrate<0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
-    else if (nd.ra_model == 0)
-        k_integrate<0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
-    else if (nd.i_model == 0)
-        k_integrate<1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
-    else
-        k_integrate<1, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks, lanes);
+    if (nd.ra_model == 0) {
+        if (nd.i_model == 1)
+            k_integrate<0, 1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        else if (lanes)
+            k_integrate<0, 0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        else
+            k_integrate<0, 0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+    } else {
+        if (nd.i_model == 1)
+            k_integrate<1, 1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        else if (lanes)
+            k_integrate<1, 0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        else
+            k_integrate<1, 0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+    }
 }
 
 #undef HVC_LAUNCHER
