@@ -29,3 +29,21 @@ extern "C" int refnoise_mt_raw(unsigned seed, int n, uint32_t* out)
     for (int k = 0; k < n; ++k) out[k] = (uint32_t)g();
     return 0;
 }
+
+// The reference's Poisson_noise::random(G[k]) for a sequence of G, one draw each.
+extern "C" int refnoise_random_seq(unsigned seed, int n, const double* G, double* out)
+{
+    Poisson_noise p;
+    p.set_seed(seed);
+    for (int k = 0; k < n; ++k) out[k] = p.random(G[k]);
+    return 0;
+}
+
+// The reference's Poisson_noise::sample_integer(0, hi[k]) for a sequence of hi.
+extern "C" int refnoise_sample_integer(unsigned seed, int n, const int* hi, int* out)
+{
+    Poisson_noise p;
+    p.set_seed(seed);
+    for (int k = 0; k < n; ++k) out[k] = p.sample_integer(0, hi[k]);
+    return 0;
+}

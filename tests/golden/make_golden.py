@@ -9,9 +9,11 @@ reference classes HH2_test / HH2_buffer / HHI_buffer produced:
   ref_poisson_noise.npz          : draws of the real poisson_noise.cpp (random,
                                    get_spike_time) and the raw mt19937 stream
                                    with the same seed.
+  ref_host_rng.npz               : Poisson_noise::random(G) for varying G and
+                                   sample_integer(0, hi), seed 15000.
 Variant "ora" links the oracle's exp/log in place of libm's (bit-exact
 targets); variant "glibc" keeps libm (tolerance targets).
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--host-rng]
 """
 import ctypes
 import os
@@ -51,7 +53,26 @@ def main():
     lib.refnoise_mt_raw(ctypes.c_uint(12345), 1024, raw.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
     out["mt_raw"] = raw
     np.savez_compressed(os.path.join(HERE, "ref_poisson_noise.npz"), **out)
+    host_rng(lib)
+
+
+def host_rng(lib):
+    # the host generator of the growth loop: random(G) with varying G and
+    # sample_integer(0, hi), seed 15000 (the reference's rank-0 seed)
+    rng = np.random.default_rng(3)
+    G = np.concatenate([[1.0, 2.0 * np.pi, 0.0, 300.0], rng.uniform(0, 10, 508)])
+    r = np.zeros(len(G))
+    lib.refnoise_random_seq(ctypes.c_uint(15000), len(G), G.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    hi = np.concatenate([[0, 1, 2, 2999, 2147483646], rng.integers(0, 100000, 507)]).astype(np.int32)
+    si = np.zeros(len(hi), np.int32)
+    lib.refnoise_sample_integer(ctypes.c_uint(15000), len(hi), hi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                si.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    np.savez_compressed(os.path.join(HERE, "ref_host_rng.npz"), G=G, random=r, hi=hi, sample_integer=si)
 
 
 if __name__ == "__main__":
-    main()
+    if "--host-rng" in sys.argv:      # only ref_host_rng.npz
+        host_rng(ref("noise"))
+    else:
+        main()

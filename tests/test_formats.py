@@ -1,0 +1,173 @@
+"""The reference's binary formats: byte layouts written out by hand from the
+reference writers (c++/src/NetworkGrowthSimulator.cpp:17531-18550) against the
+writers of unsupervised_growth_amd.formats, and reader / writer round trips
+with the tuples python_scripts/reading.py returns."""
+import struct
+
+import numpy as np
+import pytest
+
+from unsupervised_growth_amd import formats as F
+
+
+def _bytes(path):
+    with open(path, "rb") as f:
+        return f.read()
+
+
+def test_connections_layout_and_round_trip(tmp_path):
+    p = tmp_path / "RA_I_connections.bin"
+    tg, w, ln, dl = [[2, 0], [], [1]], [[0.5, 0.25], [], [1.5]], [[10.0, 20.0], [], [30.0]], [[0.1, 0.2], [], [0.3]]
+    F.write_connections(p, tg, w, ln, dl)
+    want = struct.pack("<i", 3)
+    for i in range(3):
+        want += struct.pack("<ii", i, len(tg[i]))
+        for k in range(len(tg[i])):
+            want += struct.pack("<iddd", tg[i][k], w[i][k], ln[i][k], dl[i][k])
+    assert _bytes(p) == want
+    assert F.read_connections(p) == (3, tg, w, ln, dl)
+
+
+def test_coordinates_weights_synapses_delays(tmp_path):
+    xyz = np.array([[0.1, 0.2, 0.3], [-0.5, 0.5, 0.7]])
+    F.write_coordinates(tmp_path / "RA_xy.bin", xyz, 0.25)
+    assert _bytes(tmp_path / "RA_xy.bin") == struct.pack("<iid", 2, 3, 0.25) + struct.pack("<6d", *xyz.reshape(-1))
+    n, dim, mid, c = F.read_coordinates_full(tmp_path / "RA_xy.bin")
+    assert (n, dim, mid) == (2, 3, 0.25) and np.array_equal(c, xyz)
+    F.write_coordinates(tmp_path / "sq.bin", xyz, 0.5, dim=2)
+    assert F.read_coordinates(tmp_path / "sq.bin").shape == (2, 2)
+
+    w = np.arange(9, dtype=float).reshape(3, 3) / 7
+    F.write_weights(tmp_path / "weights_5.bin", w, 5)
+    assert _bytes(tmp_path / "weights_5.bin") == struct.pack("<ii", 3, 5) + w.astype("<f8").tobytes()
+    n, t, ww = F.read_weights(tmp_path / "weights_5.bin")
+    assert (n, t) == (3, 5) and np.array_equal(ww, w)
+
+    lists = [[4, 1], [], [0, 2, 3]]
+    F.write_synapses(tmp_path / "act.bin", lists, 9)
+    want = struct.pack("<ii", 3, 9) + b"".join(struct.pack("<i", len(x)) + struct.pack("<%di" % len(x), *x)
+                                               for x in lists)
+    assert _bytes(tmp_path / "act.bin") == want
+    assert F.read_synapses(tmp_path / "act.bin") == (3, 9, lists)
+
+    rows = [[0.0, 1.5], [2.5], []]
+    F.write_axonal_delays(tmp_path / "d.bin", rows, 4)
+    assert F.read_axonal_delays(tmp_path / "d.bin") == (3, 4, rows)
+
+
+def test_spike_times(tmp_path):
+    trains = [np.array([10.5, 200.25]), np.array([]), np.array([42.0])]
+    F.write_spike_times(tmp_path / "s.bin", trains, 17, 500.0)
+    want = struct.pack("<idi", 17, 500.0, 3) + struct.pack("<i2d", 2, 10.5, 200.25) + struct.pack("<i", 0) + \
+        struct.pack("<id", 1, 42.0)
+    assert _bytes(tmp_path / "s.bin") == want
+    t, dur, got = F.read_spike_times(tmp_path / "s.bin")
+    assert (t, dur) == (17, 500.0) and [g.tolist() for g in got] == [x.tolist() for x in trains]
+    # reading.read_time_info lists only neurons that fired
+    assert F.read_time_info(tmp_path / "s.bin") == (17, 500.0, [[10.5, 200.25], [42.0]], [[0, 0], [2]])
+
+
+def test_maturation_history_indicators(tmp_path):
+    m, s = np.array([1, 0], np.int32), np.array([1000, 100000], np.int32)
+    e, g = np.array([-80.0, -55.5]), np.array([55.0, 3.25])
+    F.write_maturation_properties(tmp_path / "mp.bin", m, s, e, g, 3)
+    want = struct.pack("<ii", 2, 3) + struct.pack("<iidd", 1, 1000, -80.0, 55.0) + \
+        struct.pack("<iidd", 0, 100000, -55.5, 3.25)
+    assert _bytes(tmp_path / "mp.bin") == want
+    n, t, mm, ss, ee, gg = F.read_maturation_properties(tmp_path / "mp.bin")
+    assert (n, t) == (2, 3) and mm.tolist() == [1, 0] and ss.tolist() == [1000, 100000]
+    assert ee.tolist() == e.tolist() and gg.tolist() == g.tolist()
+
+    h = np.arange(2 * 1000, dtype=np.int32).reshape(2, 1000) % 3
+    F.write_activity_history(tmp_path / "ah.bin", h, 8)
+    assert _bytes(tmp_path / "ah.bin")[:12] == struct.pack("<iii", 2, 1000, 8)
+    n, t, hh = F.read_activity_history(tmp_path / "ah.bin")
+    assert (n, t) == (2, 8) and np.array_equal(hh, h)
+
+    F.write_indicators(tmp_path / "rem.bin", [0, 1, 1], 6)
+    assert _bytes(tmp_path / "rem.bin") == struct.pack("<5i", 3, 6, 0, 1, 1)
+    assert F.read_remodeled_indicators(tmp_path / "rem.bin")[2].tolist() == [0, 1, 1]
+
+
+def test_appended_statistics_and_training(tmp_path):
+    p = tmp_path / "weight_statistics.bin"
+    F.append_weight_statistics(p, 0, 0.5, 0.1)
+    F.append_weight_statistics(p, 10, 0.75, 0.2)
+    assert F.read_weight_statistics(p) == ([0, 10], [0.5, 0.75], [0.1, 0.2])
+    F.append_weight_statistics(p, 0, 1.0, 0.0)     # trial 0 starts a new file (NGS:17585-17588)
+    assert F.read_weight_statistics(p) == ([0], [1.0], [0.0])
+    q = tmp_path / "num_synapses.bin"
+    F.append_num_synapses(q, 0, 5, 1)
+    F.append_num_synapses(q, 1, 7, 2)
+    assert _bytes(q) == struct.pack("<6i", 0, 5, 1, 1, 7, 2)
+    assert F.read_num_synapses(q) == ([0, 1], [5, 7], [1, 2])
+    F.write_training_neurons(tmp_path / "tn.bin", [3, 1, 2])
+    assert F.read_training_neurons(tmp_path / "tn.bin").tolist() == [3, 1, 2]
+    F.write_training_spread(tmp_path / "ts.bin", [0.5, -0.5, 0.0])
+    assert F.read_training_spread(tmp_path / "ts.bin").tolist() == [0.5, -0.5, 0.0]
+    F.write_num_neurons(tmp_path / "nn.bin", 300, 30)
+    assert F.read_num_neurons(tmp_path / "nn.bin") == (300, 30)
+    r = tmp_path / "replaced_neurons.bin"
+    F.append_replaced_neurons(r, 0, [])
+    F.append_replaced_neurons(r, 1200, [4, 9])
+    assert F.read_replaced_neurons(r) == {0: [], 1200: [4, 9]}
+
+
+def test_weight_statistics_closed_form():
+    rng = np.random.default_rng(1)
+    w = rng.random((7, 7)) * (rng.random((7, 7)) < 0.3)
+    mean, std = F.weight_statistics(w)
+    assert abs(mean - w.mean()) < 1e-15
+    assert abs(std - w.std(ddof=1)) < 1e-14
+
+
+def test_jitter(tmp_path):
+    cols = [np.arange(4) * (k + 1) / 3.0 for k in range(8)]
+    F.write_jitter(tmp_path / "jitter.bin", 20, cols)
+    got = F.read_jitter(tmp_path / "jitter.bin")
+    assert got[:2] == (4, 20)
+    for k in range(8):
+        assert np.array_equal(got[2 + k], cols[k])
+    assert _bytes(tmp_path / "jitter.bin")[8:16] == struct.pack("<d", cols[0][0])
+
+
+def test_growth_state_round_trip(tmp_path):
+    """A full state written with the reference's file names reads back as the
+    same network (read_network_state, NGS:1943-2030)."""
+    rng = np.random.default_rng(3)
+    n, ni, W = 6, 3, 1000
+    d = str(tmp_path)
+    xyz = rng.normal(size=(n, 3))
+    xyz /= np.linalg.norm(xyz, axis=1, keepdims=True)
+    ixyz = rng.normal(size=(ni, 3))
+    ixyz /= np.linalg.norm(ixyz, axis=1, keepdims=True)
+    a = ([[0, 2], [1], [], [2], [0], [1, 0]], [[0.1, 0.2], [0.3], [], [0.4], [0.5], [0.6, 0.7]])
+    b = ([[5, 1], [2, 3, 4], [0]], [[1.1, 1.2], [1.3, 1.4, 1.5], [1.6]])
+    ln = lambda t: [[float(k) for k in range(len(x))] for x in t]  # noqa: E731
+    e = "_40"
+    F.write_num_neurons(tmp_path / "num_neurons.bin", n, ni)
+    F.write_coordinates(tmp_path / ("RA_xy%s.bin" % e), xyz, 0.3)
+    F.write_coordinates(tmp_path / "I_xy.bin", ixyz, 0.3)
+    F.write_connections(tmp_path / ("RA_I_connections%s.bin" % e), a[0], a[1], ln(a[0]), ln(a[0]))
+    F.write_connections(tmp_path / ("I_RA_connections%s.bin" % e), b[0], b[1], ln(b[0]), ln(b[0]))
+    w = rng.random((n, n))
+    F.write_weights(tmp_path / ("weights%s.bin" % e), w, 40)
+    F.write_axonal_delays(tmp_path / ("axonal_delays_RA2RA%s.bin" % e), rng.random((n, n)).tolist(), 40)
+    F.write_synapses(tmp_path / ("RA_RA_active_connections%s.bin" % e), [[1], [], [0, 4], [], [], []], 40)
+    F.write_synapses(tmp_path / ("RA_RA_super_connections%s.bin" % e), [[1], [], [], [], [], []], 40)
+    F.write_maturation_properties(tmp_path / ("maturation_properties%s.bin" % e), [1, 0, 0, 1, 0, 0],
+                                  [1000] * n, [-80.0] * n, [55.0] * n, 40)
+    F.write_activity_history(tmp_path / ("activity_history%s.bin" % e), np.zeros((n, W), np.int32), 40)
+    F.write_indicators(tmp_path / ("replacement_history%s.bin" % e), [41] * n, 40)
+    F.write_indicators(tmp_path / ("remodeled_indicators%s.bin" % e), [1, 0, 0, 0, 0, 0], 40)
+    F.write_training_neurons(tmp_path / "training_neurons.bin", [3, 0])
+    F.write_training_spread(tmp_path / "training_spread.bin", [0.0, 0.0])
+    s = F.read_network_state(d, 40)
+    assert s.n_ra == n and s.trial == 40 and s.training.tolist() == [0, 3]
+    net = F.state_to_network(s)
+    assert net.n_candidates == n * n and np.array_equal(net.rr_w, w.reshape(-1))
+    assert net.act_cnt.tolist() == [1, 0, 2, 0, 0, 0] and net.act_ids.tolist() == [1, 0, 4]
+    assert net.i2ra_ptr.tolist() == [0, 2, 5, 6] and net.i2ra_tgt.tolist() == [5, 1, 2, 3, 4, 0]
+    assert net.ra2i_w.tolist() == [0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7]
+    with pytest.raises(FileNotFoundError):
+        F.read_network_state(d, 41)

@@ -1,0 +1,248 @@
+"""Host side of the growth loop on the CPU: the reference's generator, topology
+generation, and the bookkeeping steps (activity history, maturation), the
+product's C++ (libhvc_engine.so, no device needed) against the oracle's
+restatement (oracle/ora_growth.py), which is itself pinned by known answers
+and by the reference's own Poisson_noise (tests/golden/ref_host_rng.npz)."""
+import math
+import os
+import sys
+from collections import deque
+
+import numpy as np
+import pytest
+
+from oracle_lib import ROOT
+
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import ora_growth as og  # noqa: E402
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+# ------------------------------------------------------------------ generator
+def test_mt19937_standard_known_answer():
+    """C++ [rand.predef]: the 10000th invocation of a default-constructed
+    std::mt19937 produces 4123659995."""
+    g = og.Mt19937()
+    for _ in range(9999):
+        g()
+    assert g() == 4123659995
+
+
+def test_mt19937_matches_reference_build_and_numpy():
+    raw = np.load(os.path.join(GOLDEN, "ref_poisson_noise.npz"))["mt_raw"]
+    g = og.Mt19937(12345)
+    assert [g() for _ in range(len(raw))] == raw.tolist()
+    # numpy's legacy seeding of MT19937 is the same init_genrand recurrence
+    bg = np.random.RandomState(15000)._bit_generator
+    g = og.Mt19937(15000)
+    assert [g() for _ in range(2000)] == bg.random_raw(2000).tolist()
+
+
+def test_product_generator_stream():
+    from unsupervised_growth_amd.topology import mt19937
+    g = og.Mt19937(15000)
+    words = [g() for _ in range(1500)]
+    np.testing.assert_array_equal(mt19937(15000, 500), words[:500])
+    np.testing.assert_array_equal(mt19937(15000, 500, skip=1000), words[1000:])
+
+
+def test_random_and_sample_integer_match_reference_poisson_noise():
+    """Poisson_noise::random(G) and sample_integer(0, hi) of the reference's
+    own poisson_noise.cpp (compiled into oracle/_ref), seed 15000."""
+    f = np.load(os.path.join(GOLDEN, "ref_host_rng.npz"))
+    g = og.Mt19937(15000)
+    got = [g.random(x) for x in f["G"]]
+    np.testing.assert_array_equal(np.array(got), f["random"])
+    g = og.Mt19937(15000)
+    got = [g.sample_integer(0, int(h)) for h in f["hi"]]
+    np.testing.assert_array_equal(np.array(got), f["sample_integer"])
+
+
+def test_distance_closed_forms():
+    assert og.distance_on_sphere((1, 0, 0), (0, 1, 0)) == math.acos(0.0)
+    assert og.distance_on_sphere((0, 0, 1), (0, 0, -1)) == math.pi
+    v = (0.6, 0.8, 0.0)
+    assert og.distance_on_sphere(v, v) == 0.0 or og.distance_on_sphere(v, v) < 1e-7
+    assert og.distance2d((0, 0), (3, 4)) == 5.0
+
+
+# ------------------------------------------------------------------ topology
+SPHERE = dict(arrangement="sphere", A_RA2I=0.35, SIGMA_RA2I=60.0, B_I2RA=0.3, SIGMA_I2RA=70.0)
+
+
+def _params(d, **kw):
+    from unsupervised_growth_amd.topology import TopologyParams
+    return TopologyParams(**{**d, **kw})
+
+
+def _flat(lists):
+    return [x for row in lists for x in row]
+
+
+@pytest.mark.parametrize("arrangement,n_ra,n_i", [("sphere", 140, 40), ("square", 90, 36), ("square", 60, 30)])
+def test_topology_equals_restatement(arrangement, n_ra, n_i):
+    from unsupervised_growth_amd.topology import generate
+    p = dict(SPHERE, arrangement=arrangement)
+    if arrangement == "square":
+        p.update(SIGMA_RA2I=45.0, SIGMA_I2RA=45.0)
+    t = generate(n_ra, n_i, 10, _params(p), seed=15000)
+    o = og.generate_topology(n_ra, n_i, 10, p, 15000)
+    assert t.model_interneuron_distance == o["mid"]
+    dim = 3 if arrangement == "sphere" else 2
+    np.testing.assert_array_equal(t.ra_xyz[:, :dim], np.array(o["ra"])[:, :dim])
+    np.testing.assert_array_equal(t.i_xyz[:, :dim], np.array(o["i"])[:, :dim])
+    np.testing.assert_array_equal(np.diff(t.ra2i_ptr), [len(x) for x in o["ra2i"]["tgt"]])
+    np.testing.assert_array_equal(t.ra2i_tgt, _flat(o["ra2i"]["tgt"]))
+    np.testing.assert_array_equal(t.ra2i_len, _flat(o["ra2i"]["len"]))
+    np.testing.assert_array_equal(np.diff(t.i2ra_ptr), [len(x) for x in o["i2ra"]["tgt"]])
+    np.testing.assert_array_equal(t.i2ra_tgt, _flat(o["i2ra"]["tgt"]))
+    np.testing.assert_array_equal(t.i2ra_len, _flat(o["i2ra"]["len"]))
+    assert not t.ra2i_w.any() and not t.i2ra_w.any() and not t.ra2i_d.any()
+    np.testing.assert_array_equal(t.training_clustered, o["clustered"])
+    np.testing.assert_array_equal(t.training_random, o["random"])
+    assert len(set(t.training_random.tolist())) == 10
+    assert len(t.ra2i_tgt) > 0 and len(t.i2ra_tgt) > 0
+
+
+def test_topology_invariants():
+    """No two neurons closer than MIN_INTERSOMATIC_DISTANCE (NGS:169); lengths
+    are distances in microns; connection frequency falls with distance."""
+    from unsupervised_growth_amd.topology import generate
+    t = generate(300, 60, 10, _params(SPHERE), seed=7)
+    mid = t.model_interneuron_distance
+    pts = np.vstack([t.ra_xyz, t.i_xyz])
+    dots = np.clip(pts @ pts.T, -1, 1)
+    ang = np.arccos(dots[np.triu_indices(len(pts), 1)])
+    assert ang.min() >= mid * 10.0 / 40.0 * (1 - 1e-12)
+    np.testing.assert_allclose(np.linalg.norm(pts, axis=1), 1.0, rtol=1e-12)
+    # lengths equal great-circle distance x 40 / MODEL_INTERNEURON_DISTANCE
+    src = np.repeat(np.arange(t.n_ra), np.diff(t.ra2i_ptr))
+    d = np.arccos(np.clip((t.ra_xyz[src] * t.i_xyz[t.ra2i_tgt]).sum(1), -1, 1)) * 40.0 / mid
+    np.testing.assert_allclose(t.ra2i_len, d, rtol=1e-9, atol=1e-9)
+    # empirical connection probability near vs far follows the Gaussian rule
+    all_d = np.arccos(np.clip(t.ra_xyz @ t.i_xyz.T, -1, 1)) * 40.0 / mid
+    conn = np.zeros_like(all_d, bool)
+    conn[src, t.ra2i_tgt] = True
+    near, far = all_d < 40, all_d > 200
+    assert conn[near].mean() > 0.2 and conn[far].mean() < 0.01
+
+
+def test_prepare_delays_and_weights():
+    from unsupervised_growth_amd.topology import generate, prepare
+    p = dict(SPHERE, Gei_max=0.5, Gie_max=1.5, delay_constant=2.0)
+    t = generate(80, 24, 6, _params(p), seed=15000)
+    t2, rr = prepare(t, _params(p), seed=15000)
+    np.testing.assert_array_equal(t2.ra2i_d, t.ra2i_len * 0.01 * 2.0)
+    np.testing.assert_array_equal(t2.i2ra_d, t.i2ra_len * 0.01 * 2.0)
+    g = og.Mt19937(15000)
+    iw = [g.random(1.5) for _ in range(len(t.i2ra_tgt))]
+    aw = [g.random(0.5) for _ in range(len(t.ra2i_tgt))]
+    np.testing.assert_array_equal(t2.i2ra_w, iw)
+    np.testing.assert_array_equal(t2.ra2i_w, aw)
+    mid = t.model_interneuron_distance
+    for i in range(0, 80, 7):
+        for j in range(0, 80, 5):
+            ln = og.distance_on_sphere(t.ra_xyz[i], t.ra_xyz[j]) * 40.0 / mid if i != j else 0.0
+            assert rr[i, j] == ln * 0.01 * 2.0
+    assert np.all(np.diag(rr) == 0)
+
+
+# ------------------------------------------------------------------ bookkeeping
+def _ring_logical(hist, head):
+    w = hist.shape[1]
+    return hist[:, (head + np.arange(w)) % w]
+
+
+@pytest.mark.parametrize("window", [1000, 1300])
+def test_activity_update_equals_restatement(window):
+    from unsupervised_growth_amd.driver import activity_update
+    rng = np.random.default_rng(window)
+    n = 40
+    rates = rng.uniform(0, 0.9, n)
+    rates[:5] = 0.0            # silent neurons
+    rates[5:8] = 0.5           # near the driven threshold
+    hist = np.zeros((n, window), np.int32)
+    head = 0
+    scale = np.full(n, og.SPONTANEOUS, np.int32)
+    oh = [deque([0] * window, maxlen=window) for _ in range(n)]
+    oscale = [og.SPONTANEOUS] * n
+    after = rng.integers(0, 600, n).astype(np.int32)
+    replaced_seen = 0
+    for t in range(1500):
+        after += 1
+        counts = (rng.random(n) < rates) * rng.integers(1, 4, n)
+        head, rep, nd = activity_update(hist, head, counts, after, scale, 0.01)
+        orep, ond = og.activity_update(oh, counts, after.tolist(), oscale, 0.01)
+        assert rep.tolist() == orep and nd == ond, t
+        replaced_seen += len(rep)
+        after[rep] = 0          # as update_replaced_neurons does
+        hist[rep] = 0
+        for r in rep:
+            oh[r].extend([0] * window)
+    np.testing.assert_array_equal(scale, oscale)
+    np.testing.assert_array_equal(_ring_logical(hist, head), np.array([list(d) for d in oh]))
+    assert replaced_seen > 0 and (scale == og.DRIVEN).any() and (scale == og.SPONTANEOUS).any()
+
+
+def test_activity_update_closed_forms():
+    """Driven exactly when >= 500 of the last 1000 trials had a spike; silent
+    neurons are replaced only once more than 1000 trials passed since their
+    last replacement (NGS:9586-9612)."""
+    from unsupervised_growth_amd.driver import activity_update
+    w = 1000
+    hist = np.zeros((4, w), np.int32)
+    hist[0, :499] = 1      # 499 + this trial's spike = 500 -> driven
+    hist[1, :498] = 1      # 498 + 1 = 499 -> not driven
+    hist[2, :999] = 3      # the oldest drops out: 999 + 1
+    scale = np.full(4, og.SPONTANEOUS, np.int32)
+    counts = np.array([1, 1, 2, 0])
+    head, rep, nd = activity_update(hist, 0, counts, np.array([5, 5, 5, 1001]), scale, 0.01)
+    assert scale.tolist() == [og.DRIVEN, og.SPONTANEOUS, og.DRIVEN, og.SPONTANEOUS] and nd == 2
+    assert rep.tolist() == [3]
+    logical = _ring_logical(hist, head)
+    assert logical[0, 0] == 1 and logical[2, 0] == 2 and logical[3].sum() == 0
+    # exactly 1000 trials since replacement: not replaced yet
+    _, rep, _ = activity_update(np.zeros((1, w), np.int32), 0, [0], [1000], np.full(1, 1000, np.int32), 0.01)
+    assert rep.tolist() == []
+    with pytest.raises(ValueError):
+        activity_update(np.zeros((1, 999), np.int32), 0, [0], [0], np.full(1, 1000, np.int32), 0.01)
+
+
+def test_maturation_update_equals_restatement_and_closed_form():
+    from unsupervised_growth_amd.driver import maturation_update
+    from unsupervised_growth_amd.network import MaturationParams
+    m = MaturationParams(E_REST_MATURE=-80.0, E_REST_IMMATURE=-55.0, GCA_MATURE=55.0, GCA_IMMATURE=0.0)
+    md = dict(E_REST_MATURE=-80.0, GCA_MATURE=55.0, E_REST_IMMATURE=-55.0, GCA_IMMATURE=0.0)
+    n = 6
+    scale = np.array([1000, 1000, 100000, 1000, 2000, 1000], np.int32)
+    mature = np.array([0, 0, 0, 1, 0, 0], np.int32)
+    erest = np.array([-55.0, -79.0, -55.0, -70.0, -60.0, -80.4])
+    gca = np.array([0.0, 50.0, 0.0, 10.0, 5.0, 54.0])
+    o = dict(mature=mature.tolist(), erest=erest.tolist(), gca=gca.tolist())
+    e0, g0 = erest.copy(), gca.copy()
+    first_mature = {}
+    for k in range(1, 7001):
+        got = maturation_update(m, scale, mature, erest, gca)
+        want = og.maturation_update(md, scale.tolist(), o["mature"], o["erest"], o["gca"])
+        assert got == want
+        for i in range(n):
+            if mature[i] and i not in first_mature:
+                first_mature[i] = k
+        if k == 100:   # closed form while immature: x_m + (x0 - x_m) e^(-k/s)
+            for i in (0, 2, 4):
+                ce = -80.0 + (e0[i] + 80.0) * math.exp(-k / scale[i])
+                assert abs(erest[i] - ce) < 1e-10 * abs(ce) + 1e-11
+                cg = 55.0 + (g0[i] - 55.0) * math.exp(-k / scale[i])
+                assert abs(gca[i] - cg) < 1e-10 * abs(cg) + 1e-11
+    np.testing.assert_array_equal(mature, o["mature"])
+    np.testing.assert_array_equal(erest, o["erest"])
+    np.testing.assert_array_equal(gca, o["gca"])
+    # neuron 0 (s = 1000, Erest -55 -> -80): mature when Erest <= -80.5 or GCa >= 54.5;
+    # GCa reaches 54.5 first, 55 e^(-k/1000) <= 0.5 -> k >= 1000 ln 110 = 4700.5
+    assert first_mature[0] == math.ceil(1000 * math.log(110))
+    assert first_mature[1] == math.ceil(1000 * math.log(10))     # 55 - 5 e^(-k/1000) >= 54.5
+    assert first_mature[5] == math.ceil(1000 * math.log(2))      # 55 - 1 e^(-k/1000) >= 54.5
+    assert erest[3] == -70.0 and gca[3] == 10.0                  # mature neurons are not updated
+    assert mature[2] == 0 and mature[4] == 0                     # 100000 and 2000 trials scales: not yet
+    assert erest[0] == -80.0 and gca[0] == 55.0

@@ -26,6 +26,15 @@ EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_sy
            "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
            "hvc_pool_download_synapses", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
            "hvc_last_error"]
+# include/hvc_topology.h
+TOPOLOGY_EXPORTS = ["hvc_topology_generate", "hvc_topology_prepare", "hvc_topology_free", "hvc_mt19937"]
+# include/hvc_driver.h
+DRIVER_EXPORTS = ["hvc_driver_create", "hvc_driver_set_output", "hvc_driver_trial", "hvc_driver_write_state",
+                  "hvc_driver_maturation", "hvc_driver_history", "hvc_driver_coordinates", "hvc_driver_replaced",
+                  "hvc_driver_connections", "hvc_driver_rr_delays", "hvc_driver_trial_number",
+                  "hvc_driver_engine", "hvc_driver_last_error", "hvc_driver_destroy", "hvc_activity_update",
+                  "hvc_maturation_update"]
+ALL_EXPORTS = EXPORTS + TOPOLOGY_EXPORTS + DRIVER_EXPORTS
 
 
 class EngineNotBuilt(RuntimeError):
@@ -91,6 +100,44 @@ class Synapses(ctypes.Structure):
                 ("sup_ids", c_int_p), ("remodeled", c_int_p)]
 
 
+class TopologyParams(ctypes.Structure):
+    _fields_ = [("arrangement", ctypes.c_int), ("A_RA2I", ctypes.c_double), ("SIGMA_RA2I", ctypes.c_double),
+                ("B_I2RA", ctypes.c_double), ("SIGMA_I2RA", ctypes.c_double), ("Gei_max", ctypes.c_double),
+                ("Gie_max", ctypes.c_double), ("delay_constant", ctypes.c_double)]
+
+
+class Topology(ctypes.Structure):
+    _fields_ = [("n_ra", ctypes.c_int), ("n_i", ctypes.c_int), ("n_tr", ctypes.c_int), ("dim", ctypes.c_int),
+                ("model_interneuron_distance", ctypes.c_double), ("ra_xyz", c_double_p), ("i_xyz", c_double_p),
+                ("ra2i_ptr", c_int_p), ("ra2i_tgt", c_int_p), ("ra2i_w", c_double_p), ("ra2i_len", c_double_p),
+                ("ra2i_d", c_double_p),
+                ("i2ra_ptr", c_int_p), ("i2ra_tgt", c_int_p), ("i2ra_w", c_double_p), ("i2ra_len", c_double_p),
+                ("i2ra_d", c_double_p),
+                ("training_clustered", c_int_p), ("training_random", c_int_p)]
+
+
+class MaturationParamsC(ctypes.Structure):
+    _fields_ = [("E_REST_MATURE", ctypes.c_double), ("E_REST_IMMATURE", ctypes.c_double),
+                ("GCA_MATURE", ctypes.c_double), ("GCA_IMMATURE", ctypes.c_double),
+                ("DEATH_RATE_THRESHOLD", ctypes.c_double), ("RATE_WINDOW_LONG", ctypes.c_int)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("ra_xyz", c_double_p), ("i_xyz", c_double_p), ("model_interneuron_distance", ctypes.c_double)]
+
+
+class DriverResume(ctypes.Structure):
+    _fields_ = [("mature", c_int_p), ("maturation_scale", c_int_p), ("trials_after_replacement", c_int_p),
+                ("history", c_int_p), ("trial_number", ctypes.c_int)]
+
+
+class DriverReport(ctypes.Structure):
+    _fields_ = [("trial", ctypes.c_int), ("mean_injection_time", ctypes.c_double), ("soma_spikes", ctypes.c_int),
+                ("dend_spikes", ctypes.c_int), ("i_spikes", ctypes.c_int), ("bad_values", ctypes.c_int),
+                ("became_driven", ctypes.c_int), ("matured", ctypes.c_int), ("replaced", ctypes.c_int),
+                ("saved", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -141,8 +188,39 @@ def load(path=LIB_PATH):
     lib.hvc_destroy.restype = None
     lib.hvc_last_error.argtypes = [vp]
     lib.hvc_last_error.restype = ctypes.c_char_p
-    for name in EXPORTS:
-        if name not in ("hvc_destroy", "hvc_last_error"):
+    lib.hvc_topology_generate.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(TopologyParams),
+                                          ctypes.c_uint32, P(P(Topology))]
+    lib.hvc_topology_prepare.argtypes = [P(Topology), P(TopologyParams), ctypes.c_uint32, c_double_p]
+    lib.hvc_topology_free.argtypes = [P(Topology)]
+    lib.hvc_topology_free.restype = None
+    lib.hvc_mt19937.argtypes = [ctypes.c_uint32, ll, ctypes.c_int, P(ctypes.c_uint32)]
+    lib.hvc_driver_create.argtypes = [P(Params), ctypes.c_int, P(NetworkDesc), P(Geometry), P(TopologyParams),
+                                      P(MaturationParamsC), P(DriverResume), ctypes.c_uint32, P(vp)]
+    lib.hvc_driver_set_output.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+    lib.hvc_driver_trial.argtypes = [vp, ctypes.c_int, P(DriverReport)]
+    lib.hvc_driver_write_state.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    lib.hvc_driver_maturation.argtypes = [vp, c_int_p, c_int_p, c_double_p, c_double_p, c_int_p]
+    lib.hvc_driver_history.argtypes = [vp, c_int_p]
+    lib.hvc_driver_coordinates.argtypes = [vp, c_double_p, c_double_p]
+    lib.hvc_driver_replaced.argtypes = [vp, c_int_p, c_int_p, ctypes.c_int, c_int_p]
+    lib.hvc_driver_connections.argtypes = [vp, P(ll), P(ll), c_int_p, c_int_p, c_double_p, c_double_p, c_int_p,
+                                           c_int_p, c_double_p, c_double_p]
+    lib.hvc_driver_rr_delays.argtypes = [vp, c_double_p]
+    lib.hvc_driver_trial_number.argtypes = [vp]
+    lib.hvc_driver_engine.argtypes = [vp]
+    lib.hvc_driver_engine.restype = vp
+    lib.hvc_driver_last_error.argtypes = [vp]
+    lib.hvc_driver_last_error.restype = ctypes.c_char_p
+    lib.hvc_driver_destroy.argtypes = [vp]
+    lib.hvc_driver_destroy.restype = None
+    lib.hvc_activity_update.argtypes = [ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                        ctypes.c_double, c_int_p, c_int_p, c_int_p]
+    lib.hvc_maturation_update.argtypes = [ctypes.c_int, P(MaturationParamsC), c_int_p, c_int_p, c_double_p,
+                                          c_double_p, c_int_p]
+    special = ("hvc_destroy", "hvc_last_error", "hvc_topology_free", "hvc_driver_engine", "hvc_driver_last_error",
+               "hvc_driver_destroy")
+    for name in ALL_EXPORTS:
+        if name not in special:
             getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib

@@ -827,6 +827,14 @@ extern "C" hvc_status hvc_patch_neurons(hvc_ctx* ctx, const int* ids, int n, con
 {
     if (!ctx || (n > 0 && (!ids || !erest || !gca))) return HVC_E_ARG;
     if (!ctx->uploaded) return fail(ctx, HVC_E_STATE, "upload a network first");
+    bool range = n > 0;   // consecutive ids: one copy per array
+    for (int k = 0; k < n && range; ++k) range = ids[k] == ids[0] + k;
+    if (range) {
+        if (ids[0] < 0 || ids[n - 1] >= ctx->n_ra) return fail(ctx, HVC_E_ARG, "neuron id out of range");
+        HVC_TRY(hipMemcpyAsync((double*)ctx->nd.EL + ids[0], erest, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HVC_TRY(hipMemcpyAsync((double*)ctx->nd.GCa + ids[0], gca, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        n = 0;
+    }
     for (int k = 0; k < n; ++k) {
         if (ids[k] < 0 || ids[k] >= ctx->n_ra) return fail(ctx, HVC_E_ARG, "neuron id out of range");
         HVC_TRY(hipMemcpyAsync((double*)ctx->nd.EL + ids[k], erest + k, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
