@@ -120,6 +120,29 @@ hvc_ctx* hvc_driver_engine(hvc_driver* d);
 const char* hvc_driver_last_error(const hvc_driver* d);
 void hvc_driver_destroy(hvc_driver* d);
 
+/* Chain test of the current network, test_chain (NGS:1313-1482): num_trials
+ * trials of trial_no_stdp_fixedSpread (kick at 100 ms, no STDP, synapse state
+ * untouched), noise streams of trial index 0x80000000 + k.  With a directory,
+ * each trial's spikes go to test_spike_times_{soma,dend,interneuron}_<k>.bin
+ * (write_chain_test, NGS:18595-18604) and the statistics to jitter.bin.  stats
+ * (n_ra x 8, may be NULL) receives the jitter statistics in the file's order. */
+hvc_status hvc_driver_chain_test(hvc_driver* d, int num_trials, const char* directory, double* stats);
+
+/* Statistics of a chain test, calculate_and_write_jitter (NGS:1641-1782): per
+ * HVC-RA neuron over the trials added, probability of a somatic spike, mean
+ * number of somatic spikes per trial, mean and standard deviation of the first
+ * somatic spike time (-1 when undefined), and the same four for dendritic
+ * spikes.  Trials are added in order; spike times neuron by neuron. */
+typedef struct hvc_chain_stats hvc_chain_stats;
+hvc_status hvc_chain_stats_create(int n_ra, hvc_chain_stats** out);
+hvc_status hvc_chain_stats_add(hvc_chain_stats* s, const int* soma_counts, const double* soma_times,
+                               const int* dend_counts, const double* dend_times);
+/* *num_trials = trials added; out: n_ra x 8 doubles (jitter.bin order). */
+hvc_status hvc_chain_stats_result(const hvc_chain_stats* s, int* num_trials, double* out);
+/* write_jitter (NGS:17818-17857): N_RA, num_trials, 8 doubles per neuron. */
+hvc_status hvc_chain_stats_write(const hvc_chain_stats* s, const char* path);
+void hvc_chain_stats_destroy(hvc_chain_stats* s);
+
 /* The bookkeeping steps on host arrays, as the driver runs them.
  * history: [n][window] ring shared by all neurons, logical entry 0 = newest at
  * index *head; a push writes at (*head - 1) mod window.  check_neuron_activity

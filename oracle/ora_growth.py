@@ -312,3 +312,34 @@ def replace_neurons(s, R, g, topo, mat):
         s["trials_after"][r] = 0
         for _ in range(len(s["history"][r])):
             s["history"][r].append(0)
+
+
+# ---------------------------------------------------------------- chain test
+def chain_statistics(trials):
+    """calculate_and_write_jitter (NGS:1641-1782) over trials given as (soma,
+    dend) lists of per-neuron spike-time lists, in trial order.  Returns a list
+    of 8-tuples per neuron in jitter.bin's order (write_jitter, NGS:17818-17857)."""
+    n = len(trials[0][0])
+    T = float(len(trials))
+    out = []
+    for i in range(n):
+        row = []
+        for kind in (0, 1):
+            first = [tr[kind][i][0] for tr in trials if len(tr[kind][i]) >= 1]
+            counts = [len(tr[kind][i]) for tr in trials]
+            total = 0.0
+            for c in counts:
+                total += c
+            mean = -1
+            if first:
+                s = 0.0
+                for x in first:
+                    s += x
+                mean = s / float(len(first))
+            acc = 0
+            for x in first:
+                acc += (x - mean) * (x - mean)
+            std = math.sqrt(acc / (float(len(first)) - 1)) if len(first) > 1 else -1
+            row += [float(len(first)) / T, total / T, float(mean), float(std)]
+        out.append(tuple(row))
+    return out

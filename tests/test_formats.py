@@ -171,3 +171,27 @@ def test_growth_state_round_trip(tmp_path):
     assert net.ra2i_w.tolist() == [0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7]
     with pytest.raises(FileNotFoundError):
         F.read_network_state(d, 41)
+
+
+def test_network_topology_files_round_trip(tmp_path):
+    """generate_network_topology's files (NGS:141-154) read back as the same
+    topology (read_network_topology, NGS:2628-2650); the Pajek file lists
+    every neuron and connection."""
+    from unsupervised_growth_amd.growth import read_network_topology, write_network_topology
+    from unsupervised_growth_amd.topology import TopologyParams, generate
+    p = TopologyParams(arrangement="sphere", A_RA2I=0.35, SIGMA_RA2I=60.0, B_I2RA=0.3, SIGMA_I2RA=70.0)
+    t = generate(50, 16, 5, p, seed=15000)
+    write_network_topology(str(tmp_path), t, p)
+    u, q = read_network_topology(str(tmp_path))
+    assert q.arrangement == "sphere" and q.SIGMA_I2RA == 70.0
+    for k in ("ra_xyz", "i_xyz", "ra2i_ptr", "ra2i_tgt", "ra2i_len", "i2ra_ptr", "i2ra_tgt", "i2ra_len",
+              "training_clustered", "training_random"):
+        np.testing.assert_array_equal(getattr(u, k), getattr(t, k), err_msg=k)
+    assert u.model_interneuron_distance == t.model_interneuron_distance
+    lines = (tmp_path / "network_topology.net").read_text().splitlines()
+    assert lines[0] == "*Vertices 66" and lines[67] == "*Arcs"
+    assert len(lines) == 68 + len(t.ra2i_tgt) + len(t.i2ra_tgt)
+    yellow = [int(l.split()[0]) - 1 for l in lines[1:51] if l.endswith("Yellow")]
+    assert yellow == sorted(t.training_random.tolist())
+    x = "%g" % t.ra_xyz[0, 0]
+    assert lines[1].startswith('1 "0" %s ' % x)

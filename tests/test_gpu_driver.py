@@ -210,3 +210,76 @@ def test_maturation_patch_and_resume(tmp_path):
         a = open(os.path.join(d, name + "_1.bin"), "rb").read()
         b = open(os.path.join(str(out), name + "_1.bin"), "rb").read()
         assert a == b, name
+
+
+def test_chain_test_statistics(tmp_path):
+    """test_chain on the driver's network: its trial files equal chain-test
+    trials of a fresh engine, and jitter.bin equals the statistics of those."""
+    from unsupervised_growth_amd import formats as F
+    from unsupervised_growth_amd.chain import ChainStats
+    from unsupervised_growth_amd.engine import Engine, make_params
+    t, net, p, mat = _setup(n_ra=90, n_i=26)
+    d = str(tmp_path)
+    print("driver chain test: n_ra=%d n_i=%d" % (net.n_ra, net.n_i), flush=True)
+    with _make(net, t, p, mat, None) as drv:
+        stats = drv.chain_test(4, d)
+        assert drv.trial_number == 0
+    cs = ChainStats(net.n_ra)
+    with Engine(make_params(precision=0, trial_ms=TRIAL_MS, seed=(7, 0))) as e:
+        e.upload(net)
+        for k in range(4):
+            res = e.run_trial(mode=1, training=True, trial_index=0x80000000 + k, first_trial=(k == 0),
+                              mean_injection_time=100.0)
+            cs.add_result(res)
+            for kind in ("soma", "dend", "interneuron"):
+                tr, dur, trains = F.read_spike_times(os.path.join(d, "test_spike_times_%s_%d.bin" % (kind, k)))
+                assert tr == 0 and dur == TRIAL_MS
+                for a, b in zip(getattr(res, kind), trains):
+                    np.testing.assert_array_equal(a, b)
+    n, want = cs.array()
+    got = F.read_jitter(os.path.join(d, "jitter.bin"))
+    assert got[:2] == (net.n_ra, 4)
+    for k, name in enumerate(F.JITTER_FIELDS):
+        np.testing.assert_array_equal(got[2 + k], want[:, k])
+        np.testing.assert_array_equal(stats[name], want[:, k])
+    assert (want[:, 0] > 0).any()
+
+
+def test_growth_runs_end_to_end(tmp_path):
+    """The reference's run sequence on the driver: topology files
+    (generate_network_topology), new_chain_growth with periodic saves,
+    continue_chain_growth from a saved trial, test_chain on a saved state.
+    The resumed run reads what the first run wrote and repeats the saved trial
+    number first, as the reference does."""
+    from unsupervised_growth_amd import formats as F
+    from unsupervised_growth_amd.config import GrowthConfig
+    from unsupervised_growth_amd.growth import continue_growth, new_growth, test_chain, write_network_topology
+    from unsupervised_growth_amd.topology import TopologyParams, generate
+    p = TopologyParams(**TOPO)
+    t = generate(70, 20, 6, p, seed=SEED)
+    net_dir, out, out2, out3 = (tmp_path / k for k in ("net", "out", "out2", "chain"))
+    for x in (net_dir, out, out2, out3):
+        x.mkdir()
+    write_network_topology(str(net_dir), t, p)
+    cfg = GrowthConfig()
+    cfg.connection.Gei_max, cfg.connection.Gie_max, cfg.connection.delay_constant = 0.25, 0.5, 2.0
+    cfg.maturation.GCA_IMMATURE, cfg.maturation.E_REST_IMMATURE = 10.0, -60.0
+    print("growth end to end: n_ra=%d n_i=%d" % (t.n_ra, t.n_i), flush=True)
+    with new_growth(str(net_dir), str(net_dir / "training_neurons_random.bin"), str(out), cfg, trial_ms=TRIAL_MS,
+                    save_freq_short=1, save_freq_long=2) as drv:
+        reps = drv.run(3)
+    assert [r.saved for r in reps] == [3, 1, 3]
+    assert F.read_num_synapses(str(out / "num_synapses.bin"))[0] == [0, 1, 2]
+    st = F.read_network_state(str(out), 2)
+    assert st.trial == 2 and st.n_ra == 70 and sorted(st.training.tolist()) == sorted(t.training_random.tolist())
+    with continue_growth(str(out), 2, str(out2), trial_ms=TRIAL_MS, save_freq_long=1) as drv:
+        r = drv.trial()
+        assert r.trial == 2 and drv.trial_number == 3
+    _, _, w_saved = F.read_weights(str(out / "weights_2.bin"))
+    _, _, w_again = F.read_weights(str(out2 / "weights_2.bin"))
+    assert w_saved.shape == w_again.shape
+    stats = test_chain(str(out), 2, 3, str(out3), trial_ms=TRIAL_MS)
+    assert stats["num_test_trials"] == 3
+    got = F.read_jitter(str(out3 / "jitter.bin"))
+    np.testing.assert_array_equal(got[2], stats["probability_soma_spike"])
+    assert os.path.exists(str(out3 / "test_spike_times_soma_2.bin"))

@@ -246,3 +246,41 @@ def test_maturation_update_equals_restatement_and_closed_form():
     assert erest[3] == -70.0 and gca[3] == 10.0                  # mature neurons are not updated
     assert mature[2] == 0 and mature[4] == 0                     # 100000 and 2000 trials scales: not yet
     assert erest[0] == -80.0 and gca[0] == 55.0
+
+
+# ------------------------------------------------------------------ chain test
+def test_chain_statistics_equal_restatement(tmp_path):
+    from unsupervised_growth_amd import formats as F
+    from unsupervised_growth_amd.chain import ChainStats
+    rng = np.random.default_rng(9)
+    n, T = 12, 7
+    trials = []
+    for _ in range(T):
+        soma, dend = [], []
+        for i in range(n):
+            k = 0 if i < 2 else int(rng.integers(0, 4))          # neurons 0, 1 never fire
+            if i == 2:
+                k = 1 if len(trials) == 0 else 0                  # one trial only: deviation undefined
+            soma.append(np.sort(rng.uniform(90, 300, k)))
+            dend.append(np.sort(rng.uniform(90, 300, int(rng.integers(0, 3)))) if i >= 2 else np.zeros(0))
+        trials.append((soma, dend))
+    cs = ChainStats(n)
+    for soma, dend in trials:
+        cs.add(soma, dend)
+    num, got = cs.array()
+    want = np.array(og.chain_statistics([([list(x) for x in s], [list(x) for x in d]) for s, d in trials]))
+    assert num == T
+    np.testing.assert_array_equal(got, want)
+    assert got[0].tolist() == [0.0, 0.0, -1.0, -1.0, 0.0, 0.0, -1.0, -1.0]
+    assert got[2, 0] == 1.0 / T and got[2, 3] == -1.0 and got[2, 2] == trials[0][0][2][0]
+    cs.write(tmp_path / "jitter.bin")
+    r = F.read_jitter(tmp_path / "jitter.bin")
+    assert r[:2] == (n, T)
+    for k in range(8):
+        np.testing.assert_array_equal(r[2 + k], got[:, k])
+    # closed form: first spike times 10, 12, 14 in three trials -> mean 12, std 2
+    cs = ChainStats(1)
+    for t in (10.0, 12.0, 14.0):
+        cs.add([np.array([t, t + 50])], [np.zeros(0)])
+    _, a = cs.array()
+    assert a[0, :4].tolist() == [1.0, 2.0, 12.0, 2.0]

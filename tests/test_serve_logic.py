@@ -1,6 +1,8 @@
 """Host logic of the serving loop with a stand-in pool (no device): first come,
 first served admission by capacity, rounds over every resident request,
 release when complete, latency statistics."""
+from types import SimpleNamespace
+
 import numpy as np
 
 from unsupervised_growth_amd.serve import Request, Server
@@ -37,7 +39,13 @@ class FakePool:
 
     def run(self, trials):
         self.rounds.append(sorted(trials))
-        return {h: ("result", h, kw["trial_index"]) for h, kw in trials.items()}
+        out = {}
+        for h, kw in trials.items():
+            n = self.resident[h]
+            # neuron 0 fires at the kick time + trial index, the others stay silent
+            soma = [np.array([kw["mean_injection_time"] + kw["trial_index"] % 100])] + [np.zeros(0)] * (n - 1)
+            out[h] = SimpleNamespace(tag=("result", h, kw["trial_index"]), soma=soma, dend=[np.zeros(0)] * n)
+        return out
 
     def download_synapses(self, h):
         return ("syn", h)
@@ -63,8 +71,13 @@ def test_fifo_admission_and_release():
     assert pool.rounds[2] == [2, 3]
     # every request ran its trials with consecutive trial indices
     for r in reqs:
-        assert [x[2] for x in r.results] == [r.first_trial_index + k for k in range(r.n_trials)]
-    assert reqs[3].synapses == ("syn", reqs[3].handle)
+        assert [x.tag[2] for x in r.results] == [r.first_trial_index + k for k in range(r.n_trials)]
+    assert reqs[3].synapses == ("syn", reqs[3].handle) and reqs[3].stats is None
+    # chain-test payload: neuron 0 fired once per trial at 100 + trial index % 100 (0 and 1 for request 0)
+    st = reqs[0].stats
+    assert st["num_test_trials"] == 2 and st["probability_soma_spike"].tolist() == [1.0] + [0.0] * 5
+    assert st["mean_first_soma_spike_time"][0] == 100.5 and st["mean_first_soma_spike_time"][1] == -1.0
+    assert st["std_first_soma_spike_time"][0] == np.std([100.0, 101.0], ddof=1)
     assert pool.used == 0 and not pool.resident
     assert stats["requests"] == 4 and stats["rounds"] == len(pool.rounds) + 2
 

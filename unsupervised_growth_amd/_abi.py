@@ -32,8 +32,9 @@ TOPOLOGY_EXPORTS = ["hvc_topology_generate", "hvc_topology_prepare", "hvc_topolo
 DRIVER_EXPORTS = ["hvc_driver_create", "hvc_driver_set_output", "hvc_driver_trial", "hvc_driver_write_state",
                   "hvc_driver_maturation", "hvc_driver_history", "hvc_driver_coordinates", "hvc_driver_replaced",
                   "hvc_driver_connections", "hvc_driver_rr_delays", "hvc_driver_trial_number",
-                  "hvc_driver_engine", "hvc_driver_last_error", "hvc_driver_destroy", "hvc_activity_update",
-                  "hvc_maturation_update"]
+                  "hvc_driver_engine", "hvc_driver_last_error", "hvc_driver_destroy", "hvc_driver_chain_test",
+                  "hvc_chain_stats_create", "hvc_chain_stats_add", "hvc_chain_stats_result", "hvc_chain_stats_write",
+                  "hvc_chain_stats_destroy", "hvc_activity_update", "hvc_maturation_update"]
 ALL_EXPORTS = EXPORTS + TOPOLOGY_EXPORTS + DRIVER_EXPORTS
 
 
@@ -217,8 +218,15 @@ def load(path=LIB_PATH):
                                         ctypes.c_double, c_int_p, c_int_p, c_int_p]
     lib.hvc_maturation_update.argtypes = [ctypes.c_int, P(MaturationParamsC), c_int_p, c_int_p, c_double_p,
                                           c_double_p, c_int_p]
+    lib.hvc_driver_chain_test.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, c_double_p]
+    lib.hvc_chain_stats_create.argtypes = [ctypes.c_int, P(vp)]
+    lib.hvc_chain_stats_add.argtypes = [vp, c_int_p, c_double_p, c_int_p, c_double_p]
+    lib.hvc_chain_stats_result.argtypes = [vp, c_int_p, c_double_p]
+    lib.hvc_chain_stats_write.argtypes = [vp, ctypes.c_char_p]
+    lib.hvc_chain_stats_destroy.argtypes = [vp]
+    lib.hvc_chain_stats_destroy.restype = None
     special = ("hvc_destroy", "hvc_last_error", "hvc_topology_free", "hvc_driver_engine", "hvc_driver_last_error",
-               "hvc_driver_destroy")
+               "hvc_driver_destroy", "hvc_chain_stats_destroy")
     for name in ALL_EXPORTS:
         if name not in special:
             getattr(lib, name).restype = ctypes.c_int

@@ -882,6 +882,54 @@ extern "C" hvc_status hvc_driver_trial(hvc_driver* d, int training, hvc_driver_r
     return HVC_OK;
 }
 
+extern "C" hvc_status hvc_driver_chain_test(hvc_driver* d, int num_trials, const char* directory, double* stats)
+{
+    if (!d || num_trials <= 0) return HVC_E_ARG;
+    if (d->dead) return fail(d, HVC_E_STATE, "a trial failed check_bad_values: the network is not usable");
+    if (!d->ctx) return fail(d, HVC_E_STATE, "no engine context");
+    hvc_chain_stats* cs = nullptr;
+    hvc_status r = hvc_chain_stats_create(d->n_ra, &cs);
+    if (r != HVC_OK) return fail(d, r, "chain statistics");
+    for (int k = 0; k < num_trials && r == HVC_OK; ++k) {
+        // trial_no_stdp_fixedSpread: kick at mean_injection_time = 100 ms (NGS:12785)
+        hvc_trial t{};
+        t.mode = 1;
+        t.training = 1;
+        t.trial_index = 0x80000000u + static_cast<uint32_t>(k);
+        t.first_trial = d->first ? 1 : 0;
+        t.mean_injection_time = 100.0;
+        hvc_trial_out o{};
+        r = hvc_run_trial(d->ctx, &t, &o);
+        d->first = false;
+        if (r == HVC_E_NUMERIC) {
+            r = fail(d, r, "check_bad_values failed in the chain test (NGS:1410-1417)");
+            break;
+        }
+        if (r != HVC_OK) {
+            r = engine_fail(d, r, "chain-test trial");
+            break;
+        }
+        if ((r = fetch_spikes(d)) != HVC_OK) break;
+        r = hvc_chain_stats_add(cs, d->cnt[0].data(), d->times[0].data(), d->cnt[1].data(), d->times[1].data());
+        if (r != HVC_OK) {
+            r = fail(d, r, "chain statistics");
+            break;
+        }
+        if (directory) {   // write_chain_test (NGS:18595-18604)
+            const std::string e = "_" + std::to_string(k) + ".bin";
+            bool ok = write_spikes(d, join(directory, "test_spike_times_soma" + e), 0);
+            ok = write_spikes(d, join(directory, "test_spike_times_dend" + e), 1) && ok;
+            ok = write_spikes(d, join(directory, "test_spike_times_interneuron" + e), 2) && ok;
+            if (!ok) r = fail(d, HVC_E_ARG, std::string("cannot write chain-test spikes into ") + directory);
+        }
+    }
+    if (r == HVC_OK && stats) r = hvc_chain_stats_result(cs, nullptr, stats);
+    if (r == HVC_OK && directory && (r = hvc_chain_stats_write(cs, join(directory, "jitter.bin").c_str())) != HVC_OK)
+        fail(d, r, std::string("cannot write jitter.bin into ") + directory);
+    hvc_chain_stats_destroy(cs);
+    return r;
+}
+
 extern "C" hvc_status hvc_driver_write_state(hvc_driver* d, const char* directory, const char* file_ending, int what)
 {
     if (!d || !directory || what < 0 || what > 2) return HVC_E_ARG;

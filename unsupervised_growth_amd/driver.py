@@ -19,8 +19,6 @@ from .engine import Engine, EngineError, _f64, _i32, _p, make_params
 from .network import MaturationParams, Network
 from .topology import TopologyParams
 
-TRAINING_SPREAD_NONE = None
-
 
 def _mat_c(m: MaturationParams):
     c = _abi.MaturationParamsC()
@@ -181,6 +179,17 @@ class GrowthDriver:
     def rr_delays(self):
         d = np.zeros(self.net.n_candidates)
         self._check(self.lib.hvc_driver_rr_delays(self.d, _p(d, _abi.c_double_p)))
+        return d
+
+    def chain_test(self, num_trials, directory=None):
+        """test_chain (NGS:1313-1482) on the current network: returns the jitter
+        statistics as a dict of per-neuron arrays (formats.JITTER_FIELDS)."""
+        from .formats import JITTER_FIELDS
+        out = np.zeros((self.n_ra, 8))
+        self._check(self.lib.hvc_driver_chain_test(self.d, num_trials, directory.encode() if directory else None,
+                                                   _p(out, _abi.c_double_p)))
+        d = {k: out[:, i].copy() for i, k in enumerate(JITTER_FIELDS)}
+        d["num_test_trials"] = num_trials
         return d
 
     def engine_ctx(self):

@@ -55,14 +55,32 @@ class NoiseParams:
 
 @dataclass
 class MaturationParams:
-    """MaturationParameters (ConfigurationNetworkGrowth.h:37-75), the fields the
-    live growth path reads."""
+    """MaturationParameters (ConfigurationNetworkGrowth.h:37-75) with the
+    reference's defaults.  The live growth path reads E_REST_*, GCA_*,
+    DEATH_RATE_THRESHOLD and RATE_WINDOW_LONG; the other properties are kept so
+    that configuration files round-trip."""
     E_REST_MATURE: float = -80.0
     E_REST_IMMATURE: float = -80.0
     GCA_MATURE: float = 55.0
     GCA_IMMATURE: float = 0.0
     DEATH_RATE_THRESHOLD: float = 0.01
     RATE_WINDOW_LONG: int = 2000
+    E_GABA_MATURE: float = -80.0
+    E_GABA_IMMATURE: float = -55.0
+    AD_MATURE: float = 10000.0
+    AD_IMMATURE: float = 1000.0
+    GK_MATURE: float = 8.0
+    GK_IMMATURE: float = 8.0
+    GNA_MATURE: float = 60.0
+    GNA_IMMATURE: float = 60.0
+    GCAK_MATURE: float = 150.0
+    GCAK_IMMATURE: float = 0.0
+    GSL_MATURE: float = 0.1
+    GSL_IMMATURE: float = 0.1
+    GDL_MATURE: float = 0.1
+    GDL_IMMATURE: float = 0.1
+    RC_MATURE: float = 55.0
+    RC_IMMATURE: float = 5.5
 
 
 @dataclass

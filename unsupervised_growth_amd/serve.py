@@ -13,7 +13,10 @@ ones behind it waiting, so large networks are not starved.
 
 Each request's result equals what the same network and trials give alone (the
 pool keys every network's noise by its own key and ids and sums over its own
-neurons only; see tests/test_gpu_serve.py).
+neurons only; see tests/test_gpu_serve.py).  A chain-test request's payload
+also carries the chain-test statistics of test_chain / calculate_and_write_jitter
+(c++/src/NetworkGrowthSimulator.cpp:1313-1482, 1641-1782): first-spike
+probability, mean and jitter per neuron, accumulated round by round.
 """
 import time
 from collections import deque
@@ -21,6 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from .chain import ChainStats
 from .engine import Pool
 
 
@@ -38,6 +42,8 @@ class Request:
     finished: float = None
     results: list = field(default_factory=list)
     synapses: object = None        # growth requests: synapse state at release
+    stats: dict = None             # chain-test requests: jitter statistics at release
+    _chain: object = None
 
     @property
     def n_trials(self):
@@ -82,6 +88,7 @@ class Server:
                 waiting.popleft()
                 req.handle, req.admitted = h, now()
                 req.results = []
+                req._chain = ChainStats(req.net.n_ra) if req.mode == 1 else None
                 active[h] = req
             if not active:
                 if realtime and pending:
@@ -96,10 +103,16 @@ class Server:
             steps = int(round(self.pool.params.trial_ms / self.pool.params.dt)) - 1
             for h, r in list(active.items()):
                 r.results.append(res[h] if keep_results else None)
+                if r._chain is not None:
+                    r._chain.add_result(res[h])
                 updates += (r.net.n_ra + r.net.n_i) * steps
                 if len(r.results) == r.n_trials:
                     if r.mode == 0:
                         r.synapses = self.pool.download_synapses(h)
+                    else:
+                        r.stats = r._chain.result()
+                        r._chain.close()
+                        r._chain = None
                     self.pool.release(h)
                     r.finished = now()
                     del active[h]
