@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--tune-windows", type=int, default=400, help="coupling windows per autotuner candidate (0: no tuning)")
     ap.add_argument("--shape", default=None,
                     help="launch shape integrate_block,i_lanes,deliver_block,sync_block (skips the autotuner)")
+    ap.add_argument("--driver-trials", type=int, default=2,
+                    help="c3: growth-loop trials through the C++ driver (bookkeeping included); 0 skips")
     ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
     ap.add_argument("--trials", type=int, default=50, help="c5: chain-test trials per request")
     ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
@@ -81,6 +83,32 @@ def cpu_baseline(net, kw, args):
     return {"value": (net.n_ra + net.n_i) * steps / el, "unit": "neuron-updates/s", "cores": 1, "kind": "port",
             "sample": "oracle growth trial of %g ms (%d steps) on the same %d RA + %d I network, %.1f s wall"
                       % (sample_ms, steps, net.n_ra, net.n_i, el)}
+
+
+def bench_driver(net, kw, args, precision, tuning):
+    """The growth loop of chain_growth through the C++ driver: trials with
+    random injection times plus the bookkeeping after each (activity history,
+    maturation, the Erest / GCa patch; no replacement happens this early)."""
+    from unsupervised_growth_amd import configs
+    from unsupervised_growth_amd.driver import GrowthDriver
+    from unsupervised_growth_amd.engine import make_params
+    from unsupervised_growth_amd.network import MaturationParams
+    ra, ii, mid, top = configs.c3_geometry()
+    params = make_params(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
+    with GrowthDriver(net, ra, ii, mid, top, MaturationParams(), params=params) as drv:
+        drv.set_tuning(**tuning)
+        drv.trial()                    # warm-up (first trial also matures the mature-valued neurons)
+        wall, reps = 0.0, []
+        for _ in range(args.driver_trials):
+            t0 = time.perf_counter()
+            reps.append(drv.trial())
+            wall += time.perf_counter() - t0
+    steps = int(round(kw["trial_ms"] / 0.02)) - 1
+    return {"value": (net.n_ra + net.n_i) * steps * args.driver_trials / wall, "unit": "neuron-updates/s",
+            "ms_per_step": 1000.0 * wall / args.driver_trials, "trials": args.driver_trials,
+            "spikes_per_trial": sum(r.soma_spikes + r.i_spikes for r in reps) / len(reps),
+            "note": "growth loop through the C++ driver (hvc_driver_trial): trial with a random injection time, "
+                    "spike read-back, check_neuron_activity, maturation update and the Erest/GCa patch"}
 
 
 def apply_shape(eng, args):
@@ -373,6 +401,8 @@ def main():
         fp64_form = {"value": (net.n_ra + net.n_i) * steps_per_trial * len(states) / wall64,
                      "ms_per_step": 1000.0 * wall64 / len(states),
                      "note": "the same trials in the plain FP64 GPU form (bit-exact with the oracle)"}
+    growth_loop = bench_driver(net, kw, args, precision, tuning) if (args.config == "c3" and
+                                                                      args.driver_trials > 0) else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(net, kw, args)
     line = {
         "metric": "RK4 neuron-updates/s at 20k RA + 5.5k I; p50/p99 served trial latency",
@@ -393,6 +423,8 @@ def main():
     if precision_check is not None:
         line["precision_check"] = precision_check
         line["fp64_form"] = fp64_form
+    if growth_loop is not None:
+        line["growth_loop"] = growth_loop
     print(json.dumps(line), flush=True)
 
 if __name__ == "__main__":

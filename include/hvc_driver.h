@@ -151,8 +151,10 @@ void hvc_chain_stats_destroy(hvc_chain_stats* s);
  * (1000); < death_rate_threshold after more than 1000 trials since replacement
  * marks the neuron for replacement (to_replace, *n_replace, ascending).
  * trials_after_replacement is read as already advanced for this trial
- * (NGS:5831-5832). */
-hvc_status hvc_activity_update(int n, int window, int* history, int* head, const int* soma_counts,
+ * (NGS:5831-5832).  recent (may be NULL): per neuron, the number of the newest
+ * 1000 history entries that are > 0, kept up to date across calls so that a
+ * call costs O(n) instead of O(1000 n); NULL counts from the history. */
+hvc_status hvc_activity_update(int n, int window, int* history, int* head, int* recent, const int* soma_counts,
                                const int* trials_after_replacement, int* maturation_scale,
                                double death_rate_threshold, int* to_replace, int* n_replace, int* became_driven);
 

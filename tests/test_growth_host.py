@@ -165,6 +165,8 @@ def test_activity_update_equals_restatement(window):
     hist = np.zeros((n, window), np.int32)
     head = 0
     scale = np.full(n, og.SPONTANEOUS, np.int32)
+    hist2, head2, scale2 = hist.copy(), 0, scale.copy()      # the O(n) form with running counts
+    recent = np.zeros(n, np.int32)
     oh = [deque([0] * window, maxlen=window) for _ in range(n)]
     oscale = [og.SPONTANEOUS] * n
     after = rng.integers(0, 600, n).astype(np.int32)
@@ -173,15 +175,22 @@ def test_activity_update_equals_restatement(window):
         after += 1
         counts = (rng.random(n) < rates) * rng.integers(1, 4, n)
         head, rep, nd = activity_update(hist, head, counts, after, scale, 0.01)
+        head2, rep2, nd2 = activity_update(hist2, head2, counts, after, scale2, 0.01, recent=recent)
         orep, ond = og.activity_update(oh, counts, after.tolist(), oscale, 0.01)
         assert rep.tolist() == orep and nd == ond, t
+        assert rep2.tolist() == orep and nd2 == ond and head2 == head, t
         replaced_seen += len(rep)
         after[rep] = 0          # as update_replaced_neurons does
         hist[rep] = 0
+        hist2[rep] = 0
+        recent[rep] = 0
         for r in rep:
             oh[r].extend([0] * window)
     np.testing.assert_array_equal(scale, oscale)
+    np.testing.assert_array_equal(scale2, oscale)
     np.testing.assert_array_equal(_ring_logical(hist, head), np.array([list(d) for d in oh]))
+    np.testing.assert_array_equal(hist2, hist)
+    np.testing.assert_array_equal(recent, (_ring_logical(hist, head)[:, :1000] > 0).sum(1))
     assert replaced_seen > 0 and (scale == og.DRIVEN).any() and (scale == og.SPONTANEOUS).any()
 
 

@@ -215,7 +215,7 @@ def load(path=LIB_PATH):
     lib.hvc_driver_destroy.argtypes = [vp]
     lib.hvc_driver_destroy.restype = None
     lib.hvc_activity_update.argtypes = [ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
-                                        ctypes.c_double, c_int_p, c_int_p, c_int_p]
+                                        c_int_p, ctypes.c_double, c_int_p, c_int_p, c_int_p]
     lib.hvc_maturation_update.argtypes = [ctypes.c_int, P(MaturationParamsC), c_int_p, c_int_p, c_double_p,
                                           c_double_p, c_int_p]
     lib.hvc_driver_chain_test.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, c_double_p]

@@ -12,10 +12,14 @@ import ctypes
 import numpy as np
 
 from . import _abi
+from .engine import SpikeTrains
 from .formats import JITTER_FIELDS
 
 
 def _flat(trains):
+    if isinstance(trains, SpikeTrains):
+        return (np.ascontiguousarray(trains.counts, np.int32),
+                np.ascontiguousarray(trains.times if trains.total else np.zeros(1), np.float64))
     counts = np.array([len(t) for t in trains], np.int32)
     times = np.concatenate([np.asarray(t, np.float64) for t in trains]) if len(trains) else np.zeros(0)
     return counts, np.ascontiguousarray(times if len(times) else np.zeros(1))

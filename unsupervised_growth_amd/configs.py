@@ -136,6 +136,20 @@ def c3(seed=11, k_rr=100):
     return net, dict(trial_ms=500.0, seed=(seed, 0))
 
 
+def c3_geometry(seed=11):
+    """Positions of the c3 network (its first draws) and the topology
+    parameters that resample a replaced neuron by the same rule, in the
+    reference's units (distance on the sphere x 40 / MODEL_INTERNEURON_DISTANCE
+    microns): (ra_xyz, i_xyz, model_interneuron_distance, TopologyParams)."""
+    from .topology import TopologyParams, sphere_model_interneuron_distance
+    rng = np.random.default_rng(seed)
+    pr, pi = sphere_positions(rng, 20000), sphere_positions(rng, 5500)
+    mid = sphere_model_interneuron_distance(5500)
+    width = 0.1 * 40.0 / mid
+    return pr, pi, mid, TopologyParams(arrangement="sphere", A_RA2I=0.3, SIGMA_RA2I=width, B_I2RA=0.3,
+                                       SIGMA_I2RA=width, Gei_max=0.25, Gie_max=0.5, delay_constant=1.0)
+
+
 def grown_chain_network(rng, n_ra, n_i, group=20, fan=10, w_chain=3000.0, delay=(1.0, 2.0), spread_max=1.0):
     """A network whose active RA->RA synapses form a chain of groups of `group`
     neurons: each neuron has `fan` candidates in the next group, all active with

@@ -51,6 +51,7 @@ struct hvc_driver {
     std::vector<int> mature, scale, trials_after;
     std::vector<int> history;   // [n_ra][W] ring, logical entry 0 at head
     int head = 0;
+    std::vector<int> recent;    // per neuron: trials with a spike among the newest 1000 of history
     std::vector<double> ra_xyz, i_xyz;
     double mid = 0;
     RefRng rng{0};
@@ -337,6 +338,7 @@ hvc_status replace_neurons(hvc_driver* d, const std::vector<int>& R)
         s.remodeled[n] = 0;
         d->trials_after[n] = 0;
         std::fill(d->history.begin() + static_cast<size_t>(n) * W, d->history.begin() + static_cast<size_t>(n + 1) * W, 0);
+        d->recent[n] = 0;
         d->rep_ids.push_back(n);
         d->rep_trials.push_back(d->trial_number);
     }
@@ -663,24 +665,32 @@ hvc_status write_initial(hvc_driver* d, const std::string& dir)
 // ---------------------------------------------------------------------------
 // bookkeeping on host arrays
 // ---------------------------------------------------------------------------
-extern "C" hvc_status hvc_activity_update(int n, int window, int* history, int* head, const int* soma_counts,
-                                          const int* trials_after_replacement, int* maturation_scale,
-                                          double death_rate_threshold, int* to_replace, int* n_replace,
-                                          int* became_driven)
+extern "C" hvc_status hvc_activity_update(int n, int window, int* history, int* head, int* recent,
+                                          const int* soma_counts, const int* trials_after_replacement,
+                                          int* maturation_scale, double death_rate_threshold, int* to_replace,
+                                          int* n_replace, int* became_driven)
 {
     const int kWindow = 1000;           // NGS:9586
     const double kDriven = 0.50;        // NGS:9588
     if (n < 0 || window < kWindow || !history || !head || *head < 0 || *head >= window ||
         (n > 0 && (!soma_counts || !trials_after_replacement || !maturation_scale || !to_replace)) || !n_replace)
         return HVC_E_ARG;
-    int h = (*head + window - 1) % window;
+    const int h0 = *head;
+    const int h = (h0 + window - 1) % window;
     *head = h;
     int nr = 0, nd = 0;
     for (int i = 0; i < n; ++i) {
         int* row = history + static_cast<size_t>(i) * window;
-        row[h] = soma_counts[i];
         int fired = 0;
-        for (int j = 0; j < kWindow; ++j) fired += row[(h + j) % window] > 0 ? 1 : 0;
+        if (recent) {
+            // the entry 999 trials back leaves the 1000-trial window, the new count enters
+            fired = recent[i] - (row[(h0 + kWindow - 1) % window] > 0 ? 1 : 0) + (soma_counts[i] > 0 ? 1 : 0);
+            row[h] = soma_counts[i];
+            recent[i] = fired;
+        } else {
+            row[h] = soma_counts[i];
+            for (int j = 0; j < kWindow; ++j) fired += row[(h + j) % window] > 0 ? 1 : 0;
+        }
         double robustness = static_cast<double>(fired) / static_cast<double>(kWindow);
         if (robustness >= kDriven && maturation_scale[i] != hvc_host::kMaturationScaleDriven) {
             maturation_scale[i] = hvc_host::kMaturationScaleDriven;
@@ -779,6 +789,9 @@ extern "C" hvc_status hvc_driver_create(const hvc_params* params, int device, co
         d->trials_after.assign(resume->trials_after_replacement, resume->trials_after_replacement + nra);
     if (resume && resume->history) d->history.assign(resume->history, resume->history + static_cast<size_t>(nra) * W);
     d->trial_number = resume ? resume->trial_number : 0;
+    d->recent.assign(nra, 0);
+    for (int i = 0; i < nra; ++i)
+        for (int j = 0; j < 1000; ++j) d->recent[i] += d->history[static_cast<size_t>(i) * W + j] > 0 ? 1 : 0;
     for (int i = 0; i < nra; ++i)
         if (d->scale[i] == 0) return fail(d, HVC_E_ARG, "maturation scale 0");
     d->rng = RefRng(seed);
@@ -853,8 +866,8 @@ extern "C" hvc_status hvc_driver_trial(hvc_driver* d, int training, hvc_driver_r
     for (int& a : d->trials_after) a += 1;
     std::vector<int> to_replace(d->n_ra);
     int n_replace = 0;
-    r = hvc_activity_update(d->n_ra, d->mat.RATE_WINDOW_LONG, d->history.data(), &d->head, d->cnt[0].data(),
-                            d->trials_after.data(), d->scale.data(), d->mat.DEATH_RATE_THRESHOLD, to_replace.data(),
+    r = hvc_activity_update(d->n_ra, d->mat.RATE_WINDOW_LONG, d->history.data(), &d->head, d->recent.data(),
+                            d->cnt[0].data(), d->trials_after.data(), d->scale.data(), d->mat.DEATH_RATE_THRESHOLD, to_replace.data(),
                             &n_replace, &rep.became_driven);
     if (r != HVC_OK) return fail(d, r, "activity update");
     to_replace.resize(n_replace);

@@ -192,24 +192,34 @@ class GrowthDriver:
         d["num_test_trials"] = num_trials
         return d
 
+    def set_tuning(self, integrate_block, i_lanes, deliver_block, sync_block):
+        """Launch shapes of the driver's engine (kept across rebuilds)."""
+        t = _abi.Tuning(integrate_block, i_lanes, deliver_block, sync_block)
+        st = self.lib.hvc_set_tuning(self.engine_ctx(), ctypes.byref(t))
+        if st != _abi.HVC_OK:
+            raise EngineError(st, "hvc_set_tuning")
+
     def engine_ctx(self):
         """The current engine context (valid until the next trial that replaces neurons)."""
         return self.lib.hvc_driver_engine(self.d)
 
 
 def activity_update(history, head, soma_counts, trials_after_replacement, maturation_scale,
-                    death_rate_threshold):
+                    death_rate_threshold, recent=None):
     """check_neuron_activity (NGS:9573-9627) on host arrays; history is the ring
-    [n][window] with logical entry 0 at `head`.  Updates history and
-    maturation_scale in place; returns (new head, neurons to replace, driven count)."""
+    [n][window] with logical entry 0 at `head`; recent (optional, int32 per
+    neuron) the running count of spiking trials among the newest 1000.  Updates
+    history, recent and maturation_scale in place; returns (new head, neurons
+    to replace, driven count)."""
     lib = _abi.load()
     n, window = history.shape
     assert history.dtype == np.int32 and history.flags.c_contiguous
     assert maturation_scale.dtype == np.int32 and maturation_scale.flags.c_contiguous
+    assert recent is None or (recent.dtype == np.int32 and recent.flags.c_contiguous)
     h = ctypes.c_int(head)
     to_rep = np.zeros(max(n, 1), np.int32)
     nr, nd = ctypes.c_int(), ctypes.c_int()
-    st = lib.hvc_activity_update(n, window, _p(history, _abi.c_int_p), ctypes.byref(h),
+    st = lib.hvc_activity_update(n, window, _p(history, _abi.c_int_p), ctypes.byref(h), _p(recent, _abi.c_int_p),
                                  _p(_i32(soma_counts), _abi.c_int_p), _p(_i32(trials_after_replacement), _abi.c_int_p),
                                  _p(maturation_scale, _abi.c_int_p), float(death_rate_threshold),
                                  _p(to_rep, _abi.c_int_p), ctypes.byref(nr), ctypes.byref(nd))

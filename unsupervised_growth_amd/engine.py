@@ -4,6 +4,7 @@
 stream.  Every call goes through the C ABI; there is no CPU fallback.
 """
 import ctypes
+from collections.abc import Sequence
 from dataclasses import dataclass
 
 import numpy as np
@@ -35,11 +36,51 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+class SpikeTrains(Sequence):
+    """Spike times neuron by neuron, as the engine returns them (a count per
+    neuron and the times concatenated); item i is neuron i's array (a view).
+    Offsets are computed on first access, so a result that is only counted or
+    passed on costs no per-neuron work."""
+
+    def __init__(self, counts, times):
+        self.counts = counts
+        self.times = times
+        self._off = None
+
+    def _offsets(self):
+        if self._off is None:
+            self._off = np.zeros(len(self.counts) + 1, np.int64)
+            np.cumsum(self.counts, out=self._off[1:])
+        return self._off
+
+    def __len__(self):
+        return len(self.counts)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        off = self._offsets()
+        return self.times[off[i]:off[i + 1]]
+
+    def __iter__(self):
+        off = self._offsets()
+        for i in range(len(self.counts)):
+            yield self.times[off[i]:off[i + 1]]
+
+    @property
+    def total(self):
+        return len(self.times)
+
+
 @dataclass
 class TrialResult:
-    soma: list          # per HVC-RA neuron: array of somatic spike times (ms)
-    dend: list          # per HVC-RA neuron: dendritic spike times
-    interneuron: list   # per HVC-I neuron: spike times
+    soma: Sequence          # per HVC-RA neuron: array of somatic spike times (ms) (SpikeTrains)
+    dend: Sequence          # per HVC-RA neuron: dendritic spike times
+    interneuron: Sequence   # per HVC-I neuron: spike times
     bad_values: int
     warnings: tuple
     ra_state: np.ndarray = None
@@ -182,7 +223,7 @@ class Engine:
             tot = ctypes.c_int()
             self._check(self.lib.hvc_trial_spikes(self.ctx, kind, _p(c, ip) if len(c) else None,
                                                   _p(times, dp), len(times), ctypes.byref(tot)))
-            lists.append(np.split(times[:tot.value], np.cumsum(c)[:-1]) if len(c) else [])
+            lists.append(SpikeTrains(c, times[:tot.value]) if len(c) else [])
         return TrialResult(lists[0], lists[1], lists[2], o.bad_values, tuple(o.warnings), ras,
                            ins[:n_i] if ins is not None else None)
 
@@ -195,7 +236,7 @@ class Engine:
         times = np.zeros(max(tot.value, 1))
         self._check(self.lib.hvc_batch_spikes(self.ctx, net_index, kind, _p(c, ip) if n else None,
                                               _p(times, dp), len(times), ctypes.byref(tot)))
-        return np.split(times[:tot.value], np.cumsum(c)[:-1]) if n else []
+        return SpikeTrains(c, times[:tot.value]) if n else []
 
     def run_batch(self, trials):
         """One trial of every network of a batch; trials: list of dicts with the
@@ -372,7 +413,7 @@ class Pool(Engine):
         times = np.zeros(max(tot.value, 1))
         self._check(self.lib.hvc_pool_spikes(self.ctx, handle, kind, _p(c, ip) if n else None, _p(times, dp),
                                              len(times), ctypes.byref(tot)))
-        return np.split(times[:tot.value], np.cumsum(c)[:-1]) if n else []
+        return SpikeTrains(c, times[:tot.value]) if n else []
 
     def run(self, trials):
         """trials: {handle: run_trial keyword dict} for every resident network;

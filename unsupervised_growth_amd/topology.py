@@ -165,6 +165,20 @@ def growth_network(t: Topology, rr_delay, training, maturation: MaturationParams
                    train_ids=training, spread=np.zeros(len(training)) if spread is None else np.asarray(spread, float))
 
 
+def sphere_model_interneuron_distance(n_i):
+    """MODEL_INTERNEURON_DISTANCE of the sphere arrangement (NGS:296-312): the
+    distance on the sphere between the first two points of the HVC-I lattice."""
+    import math
+    dphi = math.pi * (3.0 - math.sqrt(5.0))
+    dz = 2 / float(n_i)
+    z = 1 - dz / 2.0
+    x1, y1 = math.sqrt(1.0 - z * z), 0.0
+    x2 = math.cos(dphi) * math.sqrt(1.0 - (z - dz) * (z - dz))
+    y2 = math.sin(dphi) * math.sqrt(1.0 - (z - dz) * (z - dz))
+    c = x1 * x2 + y1 * y2 + z * (z - dz)
+    return 0.0 if c > 1.0 else math.acos(c)
+
+
 def mt19937(seed, n, skip=0):
     """n raw words of std::mt19937 seeded `seed` after `skip` draws."""
     lib = _abi.load()
