@@ -57,7 +57,7 @@ def parse():
                          "(calibrates the HBM byte counters)")
     ap.add_argument("--tune-windows", type=int, default=400, help="coupling windows per autotuner candidate (0: no tuning)")
     ap.add_argument("--shape", default=None,
-                    help="launch shape integrate_block,i_lanes,deliver_block,sync_block (skips the autotuner)")
+                    help="launch shape integrate_block,i_lanes,deliver_block,sync_block[,ra_lanes] (skips the autotuner)")
     ap.add_argument("--driver-trials", type=int, default=2,
                     help="c3: growth-loop trials through the C++ driver (bookkeeping included); 0 skips")
     ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
@@ -114,8 +114,9 @@ def bench_driver(net, kw, args, precision, tuning):
 def apply_shape(eng, args):
     """The launch shape: given on the command line, else chosen by the autotuner."""
     if args.shape:
-        ib, lanes, db, sb = (int(x) for x in args.shape.split(","))
-        eng.set_tuning(integrate_block=ib, i_lanes=lanes, deliver_block=db, sync_block=sb)
+        v = [int(x) for x in args.shape.split(",")]
+        eng.set_tuning(integrate_block=v[0], i_lanes=v[1], deliver_block=v[2], sync_block=v[3],
+                       ra_lanes=v[4] if len(v) > 4 else 1)
     elif args.tune_windows > 0:
         shape, ms = eng.autotune(windows=args.tune_windows)
         eng.tune_ms = ms

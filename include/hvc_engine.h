@@ -167,9 +167,10 @@ hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* 
  * reference. */
 typedef struct {
     int integrate_block;   /* threads per block of the integration kernel: 64, 128, 256 */
-    int i_lanes;           /* HVC-I DP8: 1 = five lanes per neuron, 0 = one thread per neuron */
+    int i_lanes;           /* HVC-I DP8: 2 = four lanes per neuron, 1 = five lanes, 0 = one thread per neuron */
     int deliver_block;     /* 64, 128, 256 */
     int sync_block;        /* 64 .. 512 */
+    int ra_lanes;          /* HVC-RA RK4: 1 = four lanes per neuron, 0 = one thread per neuron */
 } hvc_tuning;
 hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* tuning);
 hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* tuning);

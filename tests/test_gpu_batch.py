@@ -65,16 +65,18 @@ def test_batch_equals_each_network_alone(precision):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_launch_shapes_do_not_change_results(precision):
-    """Every launch shape the autotuner may pick gives the same bits."""
+    """Every launch shape the autotuner may pick gives the same bits (RK4 form)."""
     from unsupervised_growth_amd.engine import Engine
     nets = _nets()
     keys = [(100 + b, 7) for b in range(len(nets))]
-    shapes = [dict(integrate_block=64, i_lanes=1, deliver_block=64, sync_block=256),
-              dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512),
-              dict(integrate_block=128, i_lanes=1, deliver_block=256, sync_block=128)]
+    shapes = [dict(integrate_block=64, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1),
+              dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512, ra_lanes=0),
+              dict(integrate_block=128, i_lanes=1, deliver_block=256, sync_block=128, ra_lanes=0),
+              dict(integrate_block=256, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1)]
     runs = []
     for shape in shapes:
-        with Engine(precision=precision, trial_ms=200.0) as e:
+        # the RK4 form for HVC-RA, where four lanes per neuron exist
+        with Engine(precision=precision, trial_ms=200.0, ra_model=1) as e:
             e.upload_batch(nets, keys)
             e.set_tuning(**shape)
             assert e.tuning() == shape

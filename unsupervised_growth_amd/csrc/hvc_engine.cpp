@@ -1115,7 +1115,7 @@ static hvc_status check_tuning(hvc_ctx* ctx, const hvc_tuning* t)
 {
     auto ok_block = [](int b, int hi) { return b >= 64 && b <= hi && (b % 64) == 0; };
     if (!ok_block(t->integrate_block, 256) || !ok_block(t->deliver_block, 256) || !ok_block(t->sync_block, 512) ||
-        (t->i_lanes != 0 && t->i_lanes != 1))
+        t->i_lanes < 0 || t->i_lanes > 2 || (t->ra_lanes != 0 && t->ra_lanes != 1))
         return fail(ctx, HVC_E_ARG, "tuning out of range (blocks: multiples of 64 up to 256, sync up to 512)");
     return HVC_OK;
 }
@@ -1125,6 +1125,7 @@ extern "C" hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* t)
     if (!ctx || !t) return HVC_E_ARG;
     t->integrate_block = ctx->tuning.integrate_block;
     t->i_lanes = ctx->tuning.i_lanes;
+    t->ra_lanes = ctx->tuning.ra_lanes;
     t->deliver_block = ctx->tuning.deliver_block;
     t->sync_block = ctx->tuning.sync_block;
     return HVC_OK;
@@ -1137,6 +1138,7 @@ extern "C" hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* t)
     if (r != HVC_OK) return r;
     ctx->tuning.integrate_block = t->integrate_block;
     ctx->tuning.i_lanes = t->i_lanes;
+    ctx->tuning.ra_lanes = t->ra_lanes;
     ctx->tuning.deliver_block = t->deliver_block;
     ctx->tuning.sync_block = t->sync_block;
     return HVC_OK;
@@ -1213,19 +1215,21 @@ extern "C" hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, 
     Tuning tu = ctx->tuning;
     double ms[3], best_ms[3];
     hvc_status r;
-    // integrate: block size and the HVC-I layout
+    // integrate: block size and the HVC-I and HVC-RA layouts (five lanes per
+    // HVC-I neuron exist for DP8 only, four per HVC-RA neuron for RK4 only)
     double bi = 1e30;
     Tuning ti = tu;
-    // five lanes per HVC-I neuron exist for DP8 only
-    for (int lanes = (ctx->prm.i_model == 0 ? 1 : 0); lanes >= 0; --lanes) {
-        for (int bs : {64, 128, 256}) {
-            Tuning c = tu;
-            c.integrate_block = bs;
-            c.i_lanes = lanes;
-            if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
-            if (ms[0] < bi) { bi = ms[0]; ti = c; }
+    for (int ra_lanes = (ctx->prm.ra_model == 1 ? 1 : 0); ra_lanes >= 0; --ra_lanes)
+        for (int lanes = (ctx->prm.i_model == 0 ? 2 : 0); lanes >= 0; --lanes) {
+            for (int bs : {64, 128, 256}) {
+                Tuning c = tu;
+                c.integrate_block = bs;
+                c.i_lanes = lanes;
+                c.ra_lanes = ra_lanes;
+                if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
+                if (ms[0] < bi) { bi = ms[0]; ti = c; }
+            }
         }
-    }
     tu = ti;
     double bd = 1e30;
     Tuning td = tu;

@@ -42,11 +42,12 @@ __device__ __forceinline__ void ra_step_debraband(RAState& s, real el, real gca,
                      (ra_debraband<true>(bad, s, el, gca, u, J1s, J2s, J1d, J2d, E_dt, E_half)))
 }
 
+template <bool LANES = false>
 __device__ __forceinline__ void ra_step_runge4(RAState& s, real el, real gca, const RAUniform& u, real E_third,
-                                               real E_twothird, real E_dt)
+                                               real E_twothird, real E_dt, const RAGateSlot* sl = nullptr)
 {
-    HVC_GUARDED_STEP(RAState, (ra_runge4<false>(bad, s, el, gca, u, E_third, E_twothird, E_dt)),
-                     (ra_runge4<true>(bad, s, el, gca, u, E_third, E_twothird, E_dt)))
+    HVC_GUARDED_STEP(RAState, (ra_runge4<false, LANES>(bad, s, el, gca, u, sl, E_third, E_twothird, E_dt)),
+                     (ra_runge4<true, LANES>(bad, s, el, gca, u, sl, E_third, E_twothird, E_dt)))
 }
 
 template <int MODEL>
@@ -63,6 +64,15 @@ __device__ __forceinline__ void i_step_lanes(IState& s, real& xlo, int role, int
 {
     HVC_GUARDED_STEP(IState, (i_dp8_lanes<false>(bad, s, xlo, role, base, dt, row)),
                      (i_dp8_lanes<true>(bad, s, xlo, role, base, dt, row)))
+}
+
+__device__ __forceinline__ void i_step_quad(IState& s, real& xlo, real& wlo, int role, const GateSlot& ga,
+                                            const GateSlot& gb, real dt, const DecayRow& row)
+{
+    // the low parts are state too: a repeated step starts from their saved values
+    real xlo0 = xlo, wlo0 = wlo;
+    HVC_GUARDED_STEP(IState, (i_dp8_quad<false>(bad, s, xlo, wlo, role, ga, gb, dt, row)),
+                     (xlo = xlo0, wlo = wlo0, i_dp8_quad<true>(bad, s, xlo, wlo, role, ga, gb, dt, row)))
 }
 #undef HVC_GUARDED_STEP
 
@@ -87,6 +97,17 @@ __device__ __forceinline__ RAUniform ra_uniform(const NetDev& nd)
 {
     return RAUniform{(real)nd.sq_sig_s, (real)nd.sq_sig_d, (real)nd.mu_term_s, (real)nd.mu_term_d, (real)nd.dt};
 }
+
+// The coupling window's per-step tables, staged in LDS by k_integrate: decay
+// rows of the steps t0..t1 and the clock at t0 - 1 .. t1 (every step of every
+// neuron reads them; from HBM / L2 each read would stall the step).
+struct Win {
+    const DecayRow* rows;   // rows[t - t0]
+    const double* clock;    // clock[t - t0 + 1] = clock at step t
+    int t0;
+    __device__ __forceinline__ const DecayRow& row(int t) const { return rows[t - t0]; }
+    __device__ __forceinline__ double clk(int t) const { return clock[t - t0 + 1]; }
+};
 
 // ---------------------------------------------------------------------------
 // single-neuron traces
@@ -222,13 +243,27 @@ __device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool p
 // launch.  HVC-I blocks come first: with five lanes per neuron they carry the
 // longer per-step work, and the RA blocks run beside them.
 // ---------------------------------------------------------------------------
-template <int MODEL>
-__device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
+constexpr int kRALanes = 4;                      // lanes per HVC-RA neuron (RK4 form)
+constexpr int kRANeuronsPerWave = 64 / kRALanes;  // 16 neurons
+
+// HVC-RA neurons: one thread per neuron (LANES false, `slot` = neuron index) or,
+// RK4 form, four lanes per neuron (LANES true, `slot` = wave index; the quad's
+// first lane does the neuron's logging and stores)
+template <int MODEL, bool LANES>
+__device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0, int t1)
 {
+    static_assert(!LANES || MODEL == 1, "four lanes per neuron: RK4 form only");
+    const int lane = LANES ? __lane_id() : 0;
+    const int i = LANES ? slot * kRANeuronsPerWave + (lane >> 2) : slot;
+    const bool leader = !LANES || (lane & 3) == 0;
+    RAGateSlot gs[2];
+    if (LANES) {
+        gs[0] = kRAGates[2 * (lane & 3)];
+        gs[1] = kRAGates[2 * (lane & 3) + 1];
+    }
     const bool live = i < nd.n_ra;
     const int ii = live ? i : 0;
     const int growth = nd.net_growth[nd.ra_net[ii]];
-    const DecayRow* rows = (const DecayRow*)nd.decay;
     const RAUniform u = ra_uniform(nd);
     RAState s;
     HVC_LOAD(s, Vs, nd.ra[0][ii]) HVC_LOAD(s, Vd, nd.ra[1][ii]) HVC_LOAD(s, n, nd.ra[2][ii])
@@ -243,10 +278,10 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
     // post events detected in the previous window become visible now (NGS:9385-9386)
     int pcnt = nd.post_cnt[ii];
     double last_post = 0.0;
-    if (growth && live) {
+    if (growth && live && leader) {
         int pend = nd.post_pend[ii];
         if (pend > 0) {
-            double cs = nd.clock[t0 - 1];
+            double cs = win.clk(t0 - 1);
             for (int q = 0; q < pend; ++q) {
                 nd.post_ring[(size_t)ii * kPostRing + (pcnt % kPostRing)] = cs;
                 ++pcnt;
@@ -271,7 +306,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
     for (int t = t0; t <= t1; ++t) {
         if (kick == t) s.Ge_d += 3.0;  // raiseE(G_TRAINING_KICK), NGS:8933-8940
         int f = ra_state_check(s.Vs, s.Vd, fs, fd);
-        const DecayRow& row = rows[t];
+        const DecayRow& row = win.row(t);
         if (MODEL == 0) {
             U4 o = philox4x32((uint32_t)t, sid.local, sid.trial, kStreamRAWhite, sid.k0, sid.k1);
             ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
@@ -279,23 +314,23 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
                          row.e5[kOffDt], row.e5[kOffHalf]);
         } else {
             if (nd.ra_poisson) {
-                double time = nd.clock[t - 1];
+                double time = win.clk(t - 1);
                 ra_noise_check(ps, time, nd.dt, s.Gi_s, 0.035, nis);
                 ra_noise_check(ps, time, nd.dt, s.Gi_d, 0.045, nid);
                 ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
                 ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
             }
-            ra_step_runge4(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt]);
+            ra_step_runge4<LANES>(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt], gs);
         }
-        bool soma = live && (f & 1);
-        bool dend = live && (f & 2);
+        bool soma = live && leader && (f & 1);
+        bool dend = live && leader && (f & 2);
         if (soma) smask |= 1ull << (t - t0);
         if (dend) dmask |= 1ull << (t - t0);
         wave_append(nd.soma_log, &nd.log_cnt[0], nd.log_cap, soma, i, t, nd.err);
         wave_append(nd.dend_log, &nd.log_cnt[1], nd.log_cap, dend, i, t, nd.err);
         if (growth && soma) {
             // new postsynaptic event (NGS:8956-8967)
-            double it = nd.clock[t];
+            double it = win.clk(t);
             if (pcnt == 0 || it > last_post + 30.0) {
                 int pos = atomicAdd(nd.post_list_cnt, 1);
                 if (pos < kPostListCap)
@@ -306,7 +341,7 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
             }
         }
     }
-    if (live) {
+    if (live && leader) {
         nd.ra[0][i] = HVC_WIDE(s, Vs); nd.ra[1][i] = HVC_WIDE(s, Vd); nd.ra[2][i] = HVC_WIDE(s, n);
         nd.ra[3][i] = HVC_WIDE(s, l); nd.ra[4][i] = HVC_WIDE(s, h); nd.ra[5][i] = HVC_WIDE(s, r);
         nd.ra[6][i] = HVC_WIDE(s, c); nd.ra[7][i] = HVC_WIDE(s, Ca);
@@ -325,11 +360,10 @@ __device__ void integrate_ra_thread(const NetDev& nd, int i, int t0, int t1)
 
 // HVC-I, one lane per neuron (RK4 form)
 template <int MODEL>
-__device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
+__device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int t0, int t1)
 {
     const bool live = i < nd.n_i;
     const int ii = live ? i : 0;
-    const DecayRow* rows = (const DecayRow*)nd.decay;
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
@@ -342,11 +376,11 @@ __device__ void integrate_i_thread(const NetDev& nd, int i, int t0, int t1)
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
         if (nd.i_poisson) {
-            double time = nd.clock[t - 1];
+            double time = win.clk(t - 1);
             i_noise_check(ps, time, nd.dt, s.Ge, ne);
             i_noise_check(ps, time, nd.dt, s.Gi, ni);
         }
-        i_step<MODEL>(s, (real)nd.dt, rows[t]);
+        i_step<MODEL>(s, (real)nd.dt, win.row(t));
         bool fired = live && f;
         if (fired) mask |= 1ull << (t - t0);
         wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
@@ -366,7 +400,7 @@ constexpr int kILanes = 5;                    // lanes per HVC-I neuron (DP8)
 constexpr int kINeuronsPerWave = 64 / kILanes; // 12 neurons, 4 idle lanes
 
 // HVC-I, five lanes per neuron (DP8 form); one wave holds 12 neurons
-__device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
+__device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, int t0, int t1)
 {
     // the four spare lanes 60..63 shadow lanes 55..58 of the last group: same
     // neuron, same roles, results discarded -- their quotients then stay in the
@@ -379,7 +413,6 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
     const int i = wave * kINeuronsPerWave + group;
     const bool live = (i < nd.n_i) && !spare;
     const int ii = (i < nd.n_i) ? i : 0;
-    const DecayRow* rows = (const DecayRow*)nd.decay;
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
@@ -397,11 +430,11 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
         if (nd.i_poisson) {
-            double time = nd.clock[t - 1];
+            double time = win.clk(t - 1);
             i_noise_check(ps, time, nd.dt, s.Ge, ne);
             i_noise_check(ps, time, nd.dt, s.Gi, ni);
         }
-        i_step_lanes(s, xlo, role, base, (real)nd.dt, rows[t]);
+        i_step_lanes(s, xlo, role, base, (real)nd.dt, win.row(t));
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
         wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
@@ -424,22 +457,102 @@ __device__ void integrate_i_lanes(const NetDev& nd, int wave, int t0, int t1)
     }
 }
 
+constexpr int kINeuronsPerQuadWave = 16;       // four lanes per HVC-I neuron
+
+// HVC-I, four lanes per neuron (DP8 form); one wave holds 16 neurons
+__device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int t0, int t1)
+{
+    const int lane = __lane_id();
+    const int role = lane & 3;
+    const int i = wave * kINeuronsPerQuadWave + (lane >> 2);
+    const bool live = i < nd.n_i;
+    const int ii = live ? i : 0;
+    const GateSlot ga = kIQuadGates[role][0], gb = kIQuadGates[role][1];
+    IState s;
+    HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
+    HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
+    int flag = nd.i_flag[ii];
+    const NoiseId sid = i_noise_id(nd, ii);
+    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
+    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    // low parts of this lane's variable and of w (FP32 form; zero in FP64)
+    real xlo = 0, wlo = 0;
+#ifdef HVC_FP32
+    xlo = role == 0 ? s.V_lo : (role == 1 ? s.n_lo : (role == 2 ? s.m_lo : s.h_lo));
+    wlo = s.w_lo;
+#endif
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        if (nd.i_poisson) {
+            double time = win.clk(t - 1);
+            i_noise_check(ps, time, nd.dt, s.Ge, ne);
+            i_noise_check(ps, time, nd.dt, s.Gi, ni);
+        }
+        i_step_quad(s, xlo, wlo, role, ga, gb, (real)nd.dt, win.row(t));
+        bool fired = live && f && role == 0;
+        if (fired) mask |= 1ull << (t - t0);
+        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
+    }
+    if (live) {
+        // each lane writes its own variable, widened with its low part; lane 0 also w
+        const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
+#ifdef HVC_FP32
+        nd.in[role][i] = (double)xr + (double)xlo;
+        if (role == 0) nd.in[4][i] = (double)s.w + (double)wlo;
+#else
+        nd.in[role][i] = xr;
+        if (role == 0) nd.in[4][i] = s.w;
+#endif
+    }
+    if (live && role == 0) {
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
+        nd.i_draw[i] = ps.draw;
+        nd.i_mask[i] = mask;
+    }
+}
+
 // blocks [0, iblocks) integrate HVC-I (five lanes per neuron, 12 neurons per
-// wave, or one thread per neuron), the rest HVC-RA (one thread per neuron)
-// (the HVC-I layout is a template parameter: a kernel holds the register
+// wave; four lanes, 16 per wave; or one thread per neuron), the rest HVC-RA
+// (four lanes per neuron, 16 neurons per wave, or one thread per neuron).
+// LANES: bit 0 = HVC-I five lanes, bit 1 = HVC-RA four lanes, bit 2 = HVC-I
+// four lanes (the layouts are template parameters: a kernel holds the register
 // allocation of one layout only)
 template <int RA_MODEL, int I_MODEL, int LANES>
 __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks)
 {
+    // stage the window's decay rows and clock in LDS (dynamic shared memory of
+    // win_bytes(t1 - t0 + 1) bytes)
+    extern __shared__ double win_lds[];
+    const int nsteps = t1 - t0 + 1;
+    constexpr int kRowWords = (int)(sizeof(DecayRow) / sizeof(double));
+    const double* grow = (const double*)((const DecayRow*)nd.decay + t0);
+    for (int k = threadIdx.x; k < nsteps * kRowWords; k += blockDim.x) win_lds[k] = grow[k];
+    double* wclock = win_lds + nsteps * kRowWords;
+    for (int k = threadIdx.x; k <= nsteps; k += blockDim.x) wclock[k] = nd.clock[t0 - 1 + k];
+    __syncthreads();
+    const Win win{(const DecayRow*)win_lds, wclock, t0};
     if ((int)blockIdx.x < iblocks) {
-        if (LANES)
-            integrate_i_lanes(nd, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
+        if constexpr ((LANES & 4) != 0)
+            integrate_i_quad(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
+        else if constexpr ((LANES & 1) != 0)
+            integrate_i_lanes(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
         else
-            integrate_i_thread<I_MODEL>(nd, blockIdx.x * blockDim.x + threadIdx.x, t0, t1);
+            integrate_i_thread<I_MODEL>(nd, win, blockIdx.x * blockDim.x + threadIdx.x, t0, t1);
     } else {
-        integrate_ra_thread<RA_MODEL>(nd, (blockIdx.x - iblocks) * blockDim.x + threadIdx.x, t0, t1);
+        if constexpr ((LANES & 2) != 0)
+            integrate_ra<RA_MODEL, true>(nd, win, (blockIdx.x - iblocks) * (blockDim.x >> 6) + (threadIdx.x >> 6),
+                                         t0, t1);
+        else
+            integrate_ra<RA_MODEL, false>(nd, win, (blockIdx.x - iblocks) * blockDim.x + threadIdx.x, t0, t1);
     }
 }
+
+// LDS bytes of k_integrate for a window of n steps
+static inline size_t win_bytes(int n) { return (size_t)n * sizeof(DecayRow) + (size_t)(n + 1) * sizeof(double); }
 
 }  // namespace p32 / p64
 
@@ -465,26 +578,34 @@ void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n
 void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
     const int bs = tu.integrate_block;
-    const int lanes = (nd.i_model == 0 && tu.i_lanes) ? 1 : 0;
-    const int i_per_block = lanes ? (bs / 64) * kINeuronsPerWave : bs;
+    // HVC-I layout (DP8 only): 0 one thread, 1 five lanes, 2 four lanes per neuron
+    const int i_layout = nd.i_model == 0 ? tu.i_lanes : 0;
+    const int ra_lanes = (nd.ra_model == 1 && tu.ra_lanes) ? 1 : 0;
+    const int i_per_block = i_layout == 1 ? (bs / 64) * kINeuronsPerWave
+                            : (i_layout == 2 ? (bs / 64) * kINeuronsPerQuadWave : bs);
+    const int ra_per_block = ra_lanes ? (bs / 64) * kRANeuronsPerWave : bs;
     int iblocks = nd.n_i == 0 ? 0 : (nd.n_i + i_per_block - 1) / i_per_block;
-    int rblocks = blocks_for(nd.n_ra, bs);
+    int rblocks = blocks_for(nd.n_ra, ra_per_block);
     dim3 grid(iblocks + rblocks);
+    const size_t lds = win_bytes(t1 - t0 + 1);
+#define HVC_K(R, I, L) k_integrate<R, I, L><<<grid, bs, lds, st>>>(nd, t0, t1, iblocks)
     if (nd.ra_model == 0) {
-        if (nd.i_model == 1)
-            k_integrate<0, 1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
-        else if (lanes)
-            k_integrate<0, 0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
-        else
-            k_integrate<0, 0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        if (nd.i_model == 1) HVC_K(0, 1, 0);
+        else if (i_layout == 2) HVC_K(0, 0, 4);
+        else if (i_layout == 1) HVC_K(0, 0, 1);
+        else HVC_K(0, 0, 0);
+    } else if (ra_lanes) {
+        if (nd.i_model == 1) HVC_K(1, 1, 2);
+        else if (i_layout == 2) HVC_K(1, 0, 6);
+        else if (i_layout == 1) HVC_K(1, 0, 3);
+        else HVC_K(1, 0, 2);
     } else {
-        if (nd.i_model == 1)
-            k_integrate<1, 1, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
-        else if (lanes)
-            k_integrate<1, 0, 1><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
-        else
-            k_integrate<1, 0, 0><<<grid, bs, 0, st>>>(nd, t0, t1, iblocks);
+        if (nd.i_model == 1) HVC_K(1, 1, 0);
+        else if (i_layout == 2) HVC_K(1, 0, 4);
+        else if (i_layout == 1) HVC_K(1, 0, 1);
+        else HVC_K(1, 0, 0);
     }
+#undef HVC_K
 }
 
 #undef HVC_LAUNCHER

@@ -146,14 +146,34 @@ struct RAK {
     real Vs, Vd, n, l, h, r, c, Ca;
 };
 
-// all eight derivatives at one stage; E = decay factor of the stage time
+// gate values at one stage
+struct RAGates {
+    real m, nI, tN, hI, tH, lI, rI, cI;
+};
+
 template <int MODEL, bool EXACT>
-__device__ __forceinline__ RAK ra_rhs(Guard& bad, const RAState& s, real EL, real GCa, const RAUniform& u,
-                                      real vs, real vd, real n, real h, real l, real r,
-                                      real c, real ca, real E)
+__device__ __forceinline__ RAGates ra_gates(Guard& bad, real vs, real vd)
+{
+    RAGates g;
+    g.m = mInf<EXACT>(vs, bad);
+    g.nI = nInf<EXACT>(vs, bad);
+    g.tN = tauN<EXACT>(vs, bad);
+    g.hI = hInf<EXACT>(vs, bad);
+    g.tH = tauH<EXACT>(vs, bad);
+    g.lI = (MODEL == 0) ? lInf<EXACT>(vs, bad) : HR(0.0);
+    g.rI = rInf<EXACT>(vd, bad);
+    g.cI = cInf<EXACT>(vd, bad);
+    return g;
+}
+
+// all eight derivatives at one stage from its gate values; E = decay factor of the stage time
+template <int MODEL, bool EXACT>
+__device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, real GCa, const RAUniform& u,
+                                        const RAGates& g, real vs, real vd, real n, real h, real l, real r,
+                                        real c, real ca, real E)
 {
     RAK k;
-    real m = mInf<EXACT>(vs, bad);
+    real m = g.m;
     real m3 = m * m * m;
     real n4 = n * n * n * n;
     real acc = -kRA_GsL * (vs - EL) - kRA_GsNa * m3 * h * (vs - kRA_EsNa) - kRA_GsK * n4 * (vs - kRA_EsK);
@@ -165,13 +185,90 @@ __device__ __forceinline__ RAK ra_rhs(Guard& bad, const RAState& s, real EL, rea
     k.Vd = -kRA_GdL * (vd - EL) - GCa * r2 * (vd - kRA_EdCa) - DIVR(kRA_GdCaK * (vd - kRA_EdK) * c, 1 + DIVR(6, ca))
            - (s.Ge_d * E) * vd - (s.Gi_d * E) * (vd - kRA_Egaba) + u.mu_term_d
            + CD(100000 * (vs - vd), 550000);
-    k.n = DIVR(nInf<EXACT>(vs, bad) - n, tauN<EXACT>(vs, bad));
-    k.l = (MODEL == 0) ? CD(lInf<EXACT>(vs, bad) - l, 10) : HR(0.0);
-    k.h = DIVR(hInf<EXACT>(vs, bad) - h, tauH<EXACT>(vs, bad));
-    k.r = (rInf<EXACT>(vd, bad) - r);  // divided by tauR = 1 (exact)
-    k.c = (MODEL == 0) ? CD(cInf<EXACT>(vd, bad) - c, 15) : CD(cInf<EXACT>(vd, bad) - c, 10);
+    k.n = DIVR(g.nI - n, g.tN);
+    k.l = (MODEL == 0) ? CD(g.lI - l, 10) : HR(0.0);
+    k.h = DIVR(g.hI - h, g.tH);
+    k.r = (g.rI - r);  // divided by tauR = 1 (exact)
+    k.c = (MODEL == 0) ? CD(g.cI - c, 15) : CD(g.cI - c, 10);
     k.Ca = (-HR(0.1) * GCa * r * r * (vd - kRA_EdCa) - HR(0.02) * ca);
     return k;
+}
+
+template <int MODEL, bool EXACT>
+__device__ __forceinline__ RAK ra_rhs(Guard& bad, const RAState& s, real EL, real GCa, const RAUniform& u,
+                                      real vs, real vd, real n, real h, real l, real r,
+                                      real c, real ca, real E)
+{
+    return ra_rhs_g<MODEL, EXACT>(bad, s, EL, GCa, u, ra_gates<MODEL, EXACT>(bad, vs, vd), vs, vd, n, h, l, r, c,
+                                  ca, E);
+}
+
+// ---- HVC-RA, RK4 form, four lanes per neuron ----
+// The seven gates of HH2_buffer share one form, a + c / (1 + exp(+-(v + d) / e))
+// on Vs or Vd.  Lane q of a neuron's quad evaluates gate slots 2q and 2q + 1,
+// DPP quad broadcasts hand every lane all seven, and every lane then evaluates
+// the derivatives and stage values itself: the state stays replicated across
+// the quad and each value is the same expression tree as in ra_rhs.
+struct RAGateSlot {
+    real a, c, d, e, inv_e;   // inv_e = the form's rounding of 1 / e (as CD uses it)
+    int neg, dend;            // argument -(v + d) / (v + d); v = Vd / Vs
+};
+
+__constant__ RAGateSlot kRAGates[8] = {
+    {HR(0.0), HR(1.0), HR(30.0), HR(9.5), (real)(1.0 / 9.5), 1, 0},     // mInf
+    {HR(0.0), HR(1.0), HR(35.0), HR(10.0), (real)(1.0 / 10.0), 1, 0},   // nInf
+    {HR(0.1), HR(0.5), HR(27.0), HR(15.0), (real)(1.0 / 15.0), 0, 0},   // tauN
+    {HR(0.0), HR(1.0), HR(45.0), HR(7.0), (real)(1.0 / 7.0), 0, 0},     // hInf
+    {HR(0.1), HR(0.75), HR(40.5), HR(6.0), (real)(1.0 / 6.0), 0, 0},    // tauH
+    {HR(0.0), HR(1.0), HR(5.0), HR(10.0), (real)(1.0 / 10.0), 1, 1},    // rInf
+    {HR(0.0), HR(1.0), -HR(10.0), HR(7.0), (real)(1.0 / 7.0), 1, 1},    // cInf
+    {HR(0.0), HR(1.0), HR(0.0), HR(10.0), (real)(1.0 / 10.0), 1, 0},    // unused
+};
+
+template <bool EXACT>
+__device__ __forceinline__ real ra_gate(Guard& bad, const RAGateSlot& g, real vs, real vd)
+{
+    real x = (g.dend ? vd : vs) + g.d;
+    real q = DIVR(g.c, 1 + EXPR(CDV(g.neg ? -x : x, g.e, g.inv_e)));
+    return g.a + q;   // a = 0: 0 + q = q (q > 0)
+}
+
+// value of lane K of the quad, in every lane of the quad (DPP quad_perm)
+template <int K>
+__device__ __forceinline__ float quad_bcast(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, K * 0x55, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), K * 0x55, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <bool EXACT, bool LANES>
+__device__ __forceinline__ RAK ra_rhs4(Guard& bad, const RAState& s, real EL, real GCa, const RAUniform& u,
+                                       const RAGateSlot* sl, real vs, real vd, real n, real h, real r, real c,
+                                       real ca, real E)
+{
+    RAGates g;
+    if constexpr (LANES) {
+        const real g0 = ra_gate<EXACT>(bad, sl[0], vs, vd);
+        const real g1 = ra_gate<EXACT>(bad, sl[1], vs, vd);
+        g.m = quad_bcast<0>(g0);
+        g.nI = quad_bcast<0>(g1);
+        g.tN = quad_bcast<1>(g0);
+        g.hI = quad_bcast<1>(g1);
+        g.tH = quad_bcast<2>(g0);
+        g.rI = quad_bcast<2>(g1);
+        g.cI = quad_bcast<3>(g0);
+        g.lI = HR(0.0);
+    } else {
+        g = ra_gates<1, EXACT>(bad, vs, vd);
+    }
+    return ra_rhs_g<1, EXACT>(bad, s, EL, GCa, u, g, vs, vd, n, h, HR(0.0), r, c, ca, E);
 }
 
 // HH2_test::Debraband_step; J* are the point-distribution values of this step
@@ -222,23 +319,23 @@ __device__ __forceinline__ void ra_debraband(Guard& bad, RAState& s, real EL, re
     s.Gi_s = s.Gi_s * E_dt;
 }
 
-// HH2_buffer::Runge4_step (Kutta 3/8 rule)
-template <bool EXACT>
+// HH2_buffer::Runge4_step (Kutta 3/8 rule); LANES: four lanes per neuron, sl = this lane's two gate slots
+template <bool EXACT, bool LANES>
 __device__ __forceinline__ void ra_runge4(Guard& bad, RAState& s, real EL, real GCa, const RAUniform& u,
-                                          real E_third, real E_twothird, real E_dt)
+                                          const RAGateSlot* sl, real E_third, real E_twothird, real E_dt)
 {
     const real dt = u.dt;
-    RAK k1 = ra_rhs<1, EXACT>(bad, s, EL, GCa, u, s.Vs, s.Vd, s.n, s.h, HR(0.0), s.r, s.c, s.Ca, HR(1.0));
-    RAK k2 = ra_rhs<1, EXACT>(bad, s, EL, GCa, u, s.Vs + CD(dt * k1.Vs, 3), s.Vd + CD(dt * k1.Vd, 3), s.n + CD(dt * k1.n, 3),
-                       s.h + CD(dt * k1.h, 3), HR(0.0), s.r + CD(dt * k1.r, 3), s.c + CD(dt * k1.c, 3),
-                       s.Ca + CD(dt * k1.Ca, 3), E_third);
+    RAK k1 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs, s.Vd, s.n, s.h, s.r, s.c, s.Ca, HR(1.0));
+    RAK k2 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs + CD(dt * k1.Vs, 3), s.Vd + CD(dt * k1.Vd, 3),
+                                   s.n + CD(dt * k1.n, 3), s.h + CD(dt * k1.h, 3), s.r + CD(dt * k1.r, 3),
+                                   s.c + CD(dt * k1.c, 3), s.Ca + CD(dt * k1.Ca, 3), E_third);
 #define HVC_S3(x, k) (x + dt * (CD(-k1.k, 3) + k2.k))
-    RAK k3 = ra_rhs<1, EXACT>(bad, s, EL, GCa, u, HVC_S3(s.Vs, Vs), HVC_S3(s.Vd, Vd), HVC_S3(s.n, n), HVC_S3(s.h, h), HR(0.0),
-                       HVC_S3(s.r, r), HVC_S3(s.c, c), HVC_S3(s.Ca, Ca), E_twothird);
+    RAK k3 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, HVC_S3(s.Vs, Vs), HVC_S3(s.Vd, Vd), HVC_S3(s.n, n),
+                                   HVC_S3(s.h, h), HVC_S3(s.r, r), HVC_S3(s.c, c), HVC_S3(s.Ca, Ca), E_twothird);
 #undef HVC_S3
 #define HVC_S4(x, k) (x + dt * (k1.k - k2.k + k3.k))
-    RAK k4 = ra_rhs<1, EXACT>(bad, s, EL, GCa, u, HVC_S4(s.Vs, Vs), HVC_S4(s.Vd, Vd), HVC_S4(s.n, n), HVC_S4(s.h, h), HR(0.0),
-                       HVC_S4(s.r, r), HVC_S4(s.c, c), HVC_S4(s.Ca, Ca), E_dt);
+    RAK k4 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, HVC_S4(s.Vs, Vs), HVC_S4(s.Vd, Vd), HVC_S4(s.n, n),
+                                   HVC_S4(s.h, h), HVC_S4(s.r, r), HVC_S4(s.c, c), HVC_S4(s.Ca, Ca), E_dt);
 #undef HVC_S4
     s.Ge_d = s.Ge_d * E_dt;
     s.Ge_s = s.Ge_s * E_dt;
@@ -533,6 +630,39 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
     return role == 0 ? kV : (role == 4 ? kw : kg);
 }
 
+// DP8 stage values: the combination of stage derivatives P1.. of one
+// variable (HHI_buffer.cpp:426-691), shared by the five-lane and quad forms
+#define HVC_DP8_S2(P) CD(dt * P##1, 18)
+#define HVC_DP8_S3(P) CD(dt * (P##1 + 3 * P##2), 48)
+#define HVC_DP8_S4(P) dt * (P##1 + 3 * P##3) * HR(0.03125)
+#define HVC_DP8_S5(P) dt * (20 * P##1 - 75 * P##3 + 75 * P##4) * HR(0.015625)
+#define HVC_DP8_S6(P) CD(dt * (3 * P##1 + 15 * P##4 + 12 * P##5), 80)
+#define HVC_DP8_S7(P) dt * (CK(29443841, P##1, 614563906) + CK(77736538, P##4, 692538347) - \
+                            CK(28693883, P##5, 1125000000) + CK(23124283, P##6, 1800000000))
+#define HVC_DP8_S8(P) dt * (CK(16016141, P##1, 946692911) + CK(61564180, P##4, 158732637) + \
+                            CK(22789713, P##5, 633445777) + CK(545815736, P##6, 2771057229) - \
+                            CK(180193667, P##7, 1043307555))
+#define HVC_DP8_S9(P) dt * (CK(39632708, P##1, 573591083) - CK(433636366, P##4, 683701615) - \
+                            CK(421739975, P##5, 2616292301) + CK(100302831, P##6, 723423059) + \
+                            CK(790204164, P##7, 839813087) + CK(800635310, P##8, 3783071287))
+#define HVC_DP8_S10(P) dt * (CK(246121993, P##1, 1340847787) - CK(37695042795, P##4, 15268766246) - \
+                             CK(309121744, P##5, 1061227803) - CK(12992083, P##6, 490766935) + \
+                             CK(6005943493, P##7, 2108947869) + CK(393006217, P##8, 1396673457) + \
+                             CK(123872331, P##9, 1001029789))
+#define HVC_DP8_S11(P) dt * (CK(-1028468189, P##1, 846180014) + CK(8478235783, P##4, 508512852) + \
+                             CK(1311729495, P##5, 1432422823) - CK(10304129995, P##6, 1701304382) - \
+                             CK(48777925059, P##7, 3047939560) + CK(15336726248, P##8, 1032824649) - \
+                             CK(45442868181, P##9, 3398467696) + CK(3065993473, P##10, 597172653))
+#define HVC_DP8_S12(P) dt * (CK(185892177, P##1, 718116043) - CK(3185094517, P##4, 667107341) - \
+                             CK(477755414, P##5, 1098053517) - CK(703635378, P##6, 230739211) + \
+                             CK(5731566787, P##7, 1027545527) + CK(5232866602, P##8, 850066563) - \
+                             CK(4093664535, P##9, 808688257) + CK(3962137247, P##10, 1805957418) + \
+                             CK(65686358, P##11, 487910083))
+#define HVC_DP8_FIN(P) dt * (CK(13451932, P##1, 455176623) - CK(808719846, P##6, 976000145) + \
+                             CK(1757004468, P##7, 5645159321) + CK(656045339, P##8, 265891186) - \
+                             CK(3867574721, P##9, 1518517206) + CK(465885868, P##10, 322736535) + \
+                             CK(53011238, P##11, 667516719) + CK(2, P##12, 45))
+
 #define HVC_BCAST5(x)                       \
     v = __shfl(x, base + 0);                \
     n1 = __shfl(x, base + 1);               \
@@ -549,55 +679,42 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, in
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
     real x;
     real k1 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, HR(1.0), HR(1.0));
-    x = x0 + CD(dt * k1, 18);
+    x = x0 + HVC_DP8_S2(k);
     HVC_BCAST5(x)
     real k2 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp2], row.e5[kOffDp2]);
-    x = x0 + CD(dt * (k1 + 3 * k2), 48);
+    x = x0 + HVC_DP8_S3(k);
     HVC_BCAST5(x)
     real k3 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp3], row.e5[kOffDp3]);
-    x = x0 + dt * (k1 + 3 * k3) * HR(0.03125);
+    x = x0 + HVC_DP8_S4(k);
     HVC_BCAST5(x)
     real k4 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp4], row.e5[kOffDp4]);
-    x = x0 + dt * (20 * k1 - 75 * k3 + 75 * k4) * HR(0.015625);
+    x = x0 + HVC_DP8_S5(k);
     HVC_BCAST5(x)
     real k5 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp5], row.e5[kOffDp5]);
-    x = x0 + CD(dt * (3 * k1 + 15 * k4 + 12 * k5), 80);
+    x = x0 + HVC_DP8_S6(k);
     HVC_BCAST5(x)
     real k6 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp6], row.e5[kOffDp6]);
-    x = x0 + dt * (CK(29443841, k1, 614563906) + CK(77736538, k4, 692538347) - CK(28693883, k5, 1125000000) +
-                   CK(23124283, k6, 1800000000));
+    x = x0 + HVC_DP8_S7(k);
     HVC_BCAST5(x)
     real k7 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp7], row.e5[kOffDp7]);
-    x = x0 + dt * (CK(16016141, k1, 946692911) + CK(61564180, k4, 158732637) + CK(22789713, k5, 633445777) +
-                   CK(545815736, k6, 2771057229) - CK(180193667, k7, 1043307555));
+    x = x0 + HVC_DP8_S8(k);
     HVC_BCAST5(x)
     real k8 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp8], row.e5[kOffDp8]);
-    x = x0 + dt * (CK(39632708, k1, 573591083) - CK(433636366, k4, 683701615) - CK(421739975, k5, 2616292301) +
-                   CK(100302831, k6, 723423059) + CK(790204164, k7, 839813087) + CK(800635310, k8, 3783071287));
+    x = x0 + HVC_DP8_S9(k);
     HVC_BCAST5(x)
     real k9 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp9], row.e5[kOffDp9]);
-    x = x0 + dt * (CK(246121993, k1, 1340847787) - CK(37695042795, k4, 15268766246) -
-                   CK(309121744, k5, 1061227803) - CK(12992083, k6, 490766935) + CK(6005943493, k7, 2108947869) +
-                   CK(393006217, k8, 1396673457) + CK(123872331, k9, 1001029789));
+    x = x0 + HVC_DP8_S10(k);
     HVC_BCAST5(x)
     real k10 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp10], row.e5[kOffDp10]);
-    x = x0 + dt * (CK(-1028468189, k1, 846180014) + CK(8478235783, k4, 508512852) + CK(1311729495, k5, 1432422823) -
-                   CK(10304129995, k6, 1701304382) - CK(48777925059, k7, 3047939560) +
-                   CK(15336726248, k8, 1032824649) - CK(45442868181, k9, 3398467696) +
-                   CK(3065993473, k10, 597172653));
+    x = x0 + HVC_DP8_S11(k);
     HVC_BCAST5(x)
     real k11 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDp11], row.e5[kOffDp11]);
-    x = x0 + dt * (CK(185892177, k1, 718116043) - CK(3185094517, k4, 667107341) - CK(477755414, k5, 1098053517) -
-                   CK(703635378, k6, 230739211) + CK(5731566787, k7, 1027545527) + CK(5232866602, k8, 850066563) -
-                   CK(4093664535, k9, 808688257) + CK(3962137247, k10, 1805957418) + CK(65686358, k11, 487910083));
+    x = x0 + HVC_DP8_S12(k);
     HVC_BCAST5(x)
     real k12 = i_lane_rhs<EXACT>(bad, s, role, v, n1, m1, h1, w1, row.e2[kOffDt], row.e5[kOffDt]);
     {
         struct { real x, x_lo; } acc{x0, xlo};
-        HVC_ACC(acc, x, dt * (CK(13451932, k1, 455176623) - CK(808719846, k6, 976000145) +
-                              CK(1757004468, k7, 5645159321) + CK(656045339, k8, 265891186) -
-                              CK(3867574721, k9, 1518517206) + CK(465885868, k10, 322736535) +
-                              CK(53011238, k11, 667516719) + CK(2, k12, 45)))
+        HVC_ACC(acc, x, HVC_DP8_FIN(k))
         x = acc.x;
         xlo = acc.x_lo;
     }
@@ -607,6 +724,96 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, in
     s.Gi = s.Gi * row.e5[kOffDt];
 }
 #undef HVC_BCAST5
+
+// ---- HVC-I, DP8, four lanes per neuron (quad) ----
+// Lane q of a neuron's quad owns V (q = 0), n, m or h (q = 1..3) and evaluates
+// that variable's two gate rates; every lane also evaluates wInf and carries
+// w itself, so the four owned stage values travel by DPP quad broadcasts.
+// Every value is the expression tree of HHI_buffer, as in i_dp8 and the
+// five-lane form.
+__constant__ GateSlot kIQuadGates[4][2] = {
+    {{1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                  // V: no rates (unused)
+    {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1)}, {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0)}},        // n: an, bn
+    {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1)}, {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0)}},        // m: am, bm
+    {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0)}, {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1)}},         // h: ah, bh
+};
+
+// derivatives of the lane's variable (kx) and of w (kw) at stage state (v, n1, m1, h1, w1)
+template <bool EXACT>
+__device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role, const GateSlot& ga,
+                                           const GateSlot& gb, real v, real n1, real m1, real h1, real w1, real e2,
+                                           real e5, real& kx, real& kw)
+{
+    const GateSlot winf = {2, HR(1.0), HR(0.0), HR(5.0), HR(0.2)};   // wInf, as the five-lane table
+    real n4 = n1 * n1 * n1 * n1;
+    real m3 = m1 * m1 * m1;
+    real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
+                - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
+    real a = gate_rate<EXACT>(bad, ga, v);
+    real b = gate_rate<EXACT>(bad, gb, v);
+    real wi = gate_rate<EXACT>(bad, winf, v);
+    real x = role == 1 ? n1 : (role == 2 ? m1 : h1);
+    real kg = a * (1 - x) - b * x;
+    kx = role == 0 ? kV : kg;
+    kw = (wi - w1);
+}
+
+#define HVC_BCAST4(x)              \
+    v = quad_bcast<0>(x);          \
+    n1 = quad_bcast<1>(x);         \
+    m1 = quad_bcast<2>(x);         \
+    h1 = quad_bcast<3>(x);
+
+// HHI_buffer::DP8_step with four lanes per neuron; s is replicated in the quad;
+// xlo / wlo: low parts of the lane's variable and of w (FP32 form)
+template <bool EXACT>
+__device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, real& wlo, int role, const GateSlot& ga,
+                                           const GateSlot& gb, real dt, const DecayRow& row)
+{
+    const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
+    const real y0 = s.w;
+    real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
+    real x;
+#define HVC_QSTAGE(N, e2, e5, SX)                                                       \
+    real k_##N, q_##N;                                                                \
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, e2, e5, k_##N, q_##N); \
+    x = x0 + SX(k_);                                                                  \
+    w1 = y0 + SX(q_);                                                                 \
+    HVC_BCAST4(x)
+    real k_1, q_1;
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, HR(1.0), HR(1.0), k_1, q_1);
+    x = x0 + HVC_DP8_S2(k_);
+    w1 = y0 + HVC_DP8_S2(q_);
+    HVC_BCAST4(x)
+    HVC_QSTAGE(2, row.e2[kOffDp2], row.e5[kOffDp2], HVC_DP8_S3)
+    HVC_QSTAGE(3, row.e2[kOffDp3], row.e5[kOffDp3], HVC_DP8_S4)
+    HVC_QSTAGE(4, row.e2[kOffDp4], row.e5[kOffDp4], HVC_DP8_S5)
+    HVC_QSTAGE(5, row.e2[kOffDp5], row.e5[kOffDp5], HVC_DP8_S6)
+    HVC_QSTAGE(6, row.e2[kOffDp6], row.e5[kOffDp6], HVC_DP8_S7)
+    HVC_QSTAGE(7, row.e2[kOffDp7], row.e5[kOffDp7], HVC_DP8_S8)
+    HVC_QSTAGE(8, row.e2[kOffDp8], row.e5[kOffDp8], HVC_DP8_S9)
+    HVC_QSTAGE(9, row.e2[kOffDp9], row.e5[kOffDp9], HVC_DP8_S10)
+    HVC_QSTAGE(10, row.e2[kOffDp10], row.e5[kOffDp10], HVC_DP8_S11)
+    HVC_QSTAGE(11, row.e2[kOffDp11], row.e5[kOffDp11], HVC_DP8_S12)
+#undef HVC_QSTAGE
+    real k_12, q_12;
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, row.e2[kOffDt], row.e5[kOffDt], k_12, q_12);
+    {
+        struct { real x, x_lo; } acc{x0, xlo};
+        HVC_ACC(acc, x, HVC_DP8_FIN(k_))
+        x = acc.x;
+        xlo = acc.x_lo;
+        struct { real x, x_lo; } accw{y0, wlo};
+        HVC_ACC(accw, x, HVC_DP8_FIN(q_))
+        w1 = accw.x;
+        wlo = accw.x_lo;
+    }
+    HVC_BCAST4(x)
+    s.V = v; s.n = n1; s.m = m1; s.h = h1; s.w = w1;
+    s.Ge = s.Ge * row.e2[kOffDt];
+    s.Gi = s.Gi * row.e5[kOffDt];
+}
+#undef HVC_BCAST4
 
 }  // namespace p32 / p64
 }  // namespace hvc
