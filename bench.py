@@ -354,7 +354,8 @@ def main():
         avg, n = eng.kernel_timing(name)
         kern[name] = {"avg_ms": avg, "samples": n}
     eng.close()
-    per_trial = {k: v["avg_ms"] * (1 if k == "decay" else windows) for k, v in kern.items()}
+    # integrate and deliver run once per window; sync (the last window's) and decay once per trial
+    per_trial = {k: v["avg_ms"] * (1 if k in ("decay", "sync") else windows) for k, v in kern.items()}
     dom = max(per_trial, key=per_trial.get)
     ra_flops = opcount.flops_per_step("ra", args.ra_model)
     i_flops = opcount.flops_per_step("i", args.i_model)

@@ -224,8 +224,10 @@ hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32
  * every 16th window's launches (and the end-of-trial decay); hvc_kernel_timing
  * returns the average device duration (ms) per launch of a kernel class over
  * all trials since hvc_set_timing(ctx, 1) (which clears the averages).
- * names: "integrate" (HVC-RA and HVC-I neurons, one launch per window),
- * "deliver", "sync", "decay".
+ * names: "integrate" (HVC-RA and HVC-I neurons with the previous window's
+ * conductance updates, one launch per window), "deliver" (post-event sort and
+ * delivery, per window), "sync" (the last window's updates, once per trial),
+ * "decay" (once per growth trial).
  * No counterpart in the reference (it has no profiler). */
 hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches);
 hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled);

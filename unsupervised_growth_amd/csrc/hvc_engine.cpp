@@ -574,8 +574,10 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.post_ring = dalloc<double>(ctx, (size_t)nra * kPostRing, &st);
     nd.post_cnt = dalloc<int>(ctx, nra, &st);
     nd.post_pend = dalloc<int>(ctx, nra, &st);
-    nd.post_list = dalloc<uint64_t>(ctx, kPostListCap, &st);
-    nd.post_list_cnt = dalloc<int>(ctx, 1, &st);
+    // two buffers, alternating by window (window w fills and reads buffer w % 2)
+    nd.post_list = dalloc<uint64_t>(ctx, 2 * (size_t)kPostListCap, &st);
+    nd.post_list_cnt = dalloc<int>(ctx, 2, &st);
+    nd.post_list_next_cnt = nd.post_list_cnt ? nd.post_list_cnt + 1 : nullptr;
     ctx->d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
     ctx->d_rr_tgt = dalloc<int>(ctx, E, &st);
     ctx->d_rr_d = dalloc<double>(ctx, E, &st);
@@ -1006,23 +1008,31 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     launch_reset(s, nd, ctx->E);
     if (any_chain) launch_chain_edges(s, nd);
     const int nw = (int)ctx->windows.size();
+    // the window's view: post-event buffer w % 2.  The sync of window w (the
+    // inbox sums into the conductances) runs at the start of window w + 1's
+    // integration; after the last window it runs on its own.
+    NetDev ndw[2] = {nd, nd};
+    ndw[1].post_list = nd.post_list + kPostListCap;
+    ndw[1].post_list_cnt = nd.post_list_cnt + 1;
+    ndw[1].post_list_next_cnt = nd.post_list_cnt;
     for (int w = 0; w < nw; ++w) {
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
+        const NetDev& nv = ndw[w & 1];
         bool tm = ctx->timing && (w % 16 == 7);
         tstart(ctx, kTInt, tm);
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1, ctx->tuning);
+            launch_integrate_f32(s, nv, t0, t1, ctx->tuning);
         else
-            launch_integrate(s, nd, t0, t1, ctx->tuning);
+            launch_integrate(s, nv, t0, t1, ctx->tuning);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
-        if (any_growth) launch_sort_posts(s, nd);
-        launch_deliver(s, nd, t0, t1, ctx->tuning);
+        if (any_growth) launch_sort_posts(s, nv);
+        launch_deliver(s, nv, t0, t1, ctx->tuning);
         tstop(ctx, kTDeliver, tm);
-        tstart(ctx, kTSync, tm);
-        launch_sync(s, nd, ctx->tuning);
-        tstop(ctx, kTSync, tm);
     }
+    tstart(ctx, kTSync, ctx->timing);
+    launch_sync(s, nd, ctx->tuning);
+    tstop(ctx, kTSync, ctx->timing);
     if (any_growth) {
         tstart(ctx, kTDecay, ctx->timing);
         launch_decay(s, nd);

@@ -16,6 +16,42 @@ namespace hvc {
 
 static inline int blocks_for(int n, int bs) { return n > 0 ? (n + bs - 1) / bs : 1; }
 
+#ifndef HVC_INBOX_DEFINED
+#define HVC_INBOX_DEFINED
+// ordered sum of one target's inbox: records sorted by (arrival step, presynaptic id)
+__device__ __forceinline__ double inbox_sum(const Inbox& ib, int tgt, int n)
+{
+    Record* r = ib.rec + (size_t)tgt * ib.cap;
+    for (int a = 1; a < n; ++a) {
+        Record x = r[a];
+        int b = a - 1;
+        while (b >= 0 && r[b].key > x.key) {
+            r[b + 1] = r[b];
+            --b;
+        }
+        r[b + 1] = x;
+    }
+    double sum = 0.0;
+    for (int a = 0; a < n; ++a) sum += r[a].w;
+    return sum;
+}
+
+// the previous window's sync for one target (NGS:9309-9366): the ordered sum
+// of its inbox, if any, and the inbox emptied
+__device__ __forceinline__ bool inbox_take(const NetDev& nd, const Inbox& ib, int tgt, double& sum)
+{
+    int n = ib.cnt[tgt];
+    if (n <= 0) return false;
+    if (n > ib.cap) {
+        atomicOr(nd.err, (unsigned)kErrInbox);
+        n = ib.cap;
+    }
+    sum = inbox_sum(ib, tgt, n);
+    ib.cnt[tgt] = 0;
+    return true;
+}
+#endif
+
 HVC_PREC_BEGIN
 
 // One neuron step.  FP64: with the branch-free constant divisions; if any lane
@@ -269,7 +305,18 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     HVC_LOAD(s, Vs, nd.ra[0][ii]) HVC_LOAD(s, Vd, nd.ra[1][ii]) HVC_LOAD(s, n, nd.ra[2][ii])
     HVC_LOAD(s, l, nd.ra[3][ii]) HVC_LOAD(s, h, nd.ra[4][ii]) HVC_LOAD(s, r, nd.ra[5][ii])
     HVC_LOAD(s, c, nd.ra[6][ii]) HVC_LOAD(s, Ca, nd.ra[7][ii])
-    s.Ge_s = nd.ra[8][ii]; s.Gi_s = nd.ra[9][ii]; s.Ge_d = nd.ra[10][ii]; s.Gi_d = nd.ra[11][ii];
+    // the previous window's sync: raiseI (Gi_d += G), raiseExcWeight (Ge_d += w / Ad)
+    double ge_d = nd.ra[10][ii], gi_d = nd.ra[11][ii];
+    if (live && leader) {
+        double sum;
+        if (inbox_take(nd, nd.in_gi_ra, i, sum)) gi_d += sum;
+        if (inbox_take(nd, nd.in_ge_ra, i, sum)) ge_d += HVC_CDIV(sum, 10000);
+    }
+    if constexpr (LANES) {
+        ge_d = quad_bcast<0>(ge_d);
+        gi_d = quad_bcast<0>(gi_d);
+    }
+    s.Ge_s = nd.ra[8][ii]; s.Gi_s = nd.ra[9][ii]; s.Ge_d = ge_d; s.Gi_d = gi_d;
     int flags = nd.ra_flags[ii];
     int fs = flags & 1, fd = (flags >> 1) & 1;
     const real el = (real)nd.EL[ii], gca = (real)nd.GCa[ii];
@@ -367,7 +414,13 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
-    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
+    // the previous window's sync: HHI raiseE (Ge += G)
+    double ge = nd.in[5][ii];
+    if (live) {
+        double sum;
+        if (inbox_take(nd, nd.in_ge_i, i, sum)) ge += sum;
+    }
+    s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     const NoiseId sid = i_noise_id(nd, ii);
     PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
@@ -416,7 +469,14 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
-    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
+    // the previous window's sync: HHI raiseE (Ge += G), by the group's first lane
+    double ge = nd.in[5][ii];
+    if (live && role == 0) {
+        double sum;
+        if (inbox_take(nd, nd.in_ge_i, i, sum)) ge += sum;
+    }
+    ge = __shfl(ge, base);
+    s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     const NoiseId sid = i_noise_id(nd, ii);
     PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
@@ -471,7 +531,14 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
-    s.Ge = nd.in[5][ii]; s.Gi = nd.in[6][ii];
+    // the previous window's sync: HHI raiseE (Ge += G), by the quad's first lane
+    double ge = nd.in[5][ii];
+    if (live && role == 0) {
+        double sum;
+        if (inbox_take(nd, nd.in_ge_i, i, sum)) ge += sum;
+    }
+    ge = quad_bcast<0>(ge);
+    s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     const NoiseId sid = i_noise_id(nd, ii);
     PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};

@@ -125,8 +125,9 @@ struct NetDev {
     double* post_ring;           // [n_ra][kPostRing]
     int* post_cnt;
     int* post_pend;
-    uint64_t* post_list;         // post_key(network, step, neuron), sorted
-    int* post_list_cnt;
+    uint64_t* post_list;         // post_key(network, step, neuron), sorted; the window's buffer
+    int* post_list_cnt;          // its count
+    int* post_list_next_cnt;     // count of the other buffer (the next window's), reset by k_deliver
 
     // RA->RA candidates
     const int* rr_ptr;

@@ -87,7 +87,8 @@ __global__ void k_reset(NetDev nd)
     }
     if (i == 0) {
         nd.log_cnt[0] = 0; nd.log_cnt[1] = 0; nd.log_cnt[2] = 0;
-        *nd.post_list_cnt = 0;
+        nd.post_list_cnt[0] = 0;
+        nd.post_list_cnt[1] = 0;
         *nd.err = 0;
     }
     if (i < 3 * nd.n_nets) nd.warn[i] = 0;
@@ -525,6 +526,9 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
 __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int blocks_ra)
 {
     int b = blockIdx.x;
+    // the other post-event buffer was read by the previous window's delivery;
+    // the next window's integration fills it
+    if (b == 0 && threadIdx.x == 0) *nd.post_list_next_cnt = 0;
     int role = b / blocks_ra;
     int local = (b % blocks_ra) * blockDim.x + threadIdx.x;
     if (role == 0) {
@@ -545,29 +549,12 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
     }
 }
 
-// ordered sum of one target's inbox: records sorted by (arrival step, presynaptic id)
-__device__ __forceinline__ double inbox_sum(const Inbox& ib, int tgt, int n)
-{
-    Record* r = ib.rec + (size_t)tgt * ib.cap;
-    for (int a = 1; a < n; ++a) {
-        Record x = r[a];
-        int b = a - 1;
-        while (b >= 0 && r[b].key > x.key) {
-            r[b + 1] = r[b];
-            --b;
-        }
-        r[b + 1] = x;
-    }
-    double sum = 0.0;
-    for (int a = 0; a < n; ++a) sum += r[a].w;
-    return sum;
-}
-
-// sync: apply the window's conductance updates (NGS:9309-9366)
+// sync: apply the window's conductance updates (NGS:9309-9366).  Within a
+// trial k_integrate applies each window's updates as it loads the neurons for
+// the next window; this kernel applies the last window's.
 __global__ __launch_bounds__(512) void k_sync(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *nd.post_list_cnt = 0;  // the window's post-event list has been consumed
     if (i < nd.n_ra) {
         int n = nd.in_gi_ra.cnt[i];
         if (n > 0) {
