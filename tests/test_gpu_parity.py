@@ -70,18 +70,20 @@ def _compare_synapses(syn, osyn):
         np.testing.assert_array_equal(getattr(syn, k), osyn[k], err_msg=k)
 
 
-@pytest.mark.parametrize("rr,ra_model,ra_lanes", [("dense", 0, 1), (12, 0, 1), ("dense", 1, 1), (12, 1, 0)])
-def test_growth_trials_bit_exact(rr, ra_model, ra_lanes):
-    """Full growth trials; the RK4 form both with four lanes per HVC-RA neuron
-    and with one thread per neuron."""
+@pytest.mark.parametrize("rr,ra_model,ra_lanes,i_lanes", [("dense", 0, 0, 1), (12, 0, 0, 2), ("dense", 1, 1, 2),
+                                                             (12, 1, 0, 1), (12, 1, 1, 0)])
+def test_growth_trials_bit_exact(rr, ra_model, ra_lanes, i_lanes):
+    """Full growth trials in every integration layout: HVC-RA (RK4 form) with
+    four lanes or one thread per neuron, HVC-I (DP8) with four or five lanes or
+    one thread per neuron."""
     net = _small_net(rr=rr)
     trial_ms = 300.0
-    print("growth rr=%s ra_model=%d ra_lanes=%d n_ra=%d n_i=%d trial_ms=%g" % (rr, ra_model, ra_lanes, net.n_ra,
-                                                                            net.n_i, trial_ms), flush=True)
+    print("growth rr=%s ra_model=%d ra_lanes=%d i_lanes=%d n_ra=%d n_i=%d trial_ms=%g" % (
+        rr, ra_model, ra_lanes, i_lanes, net.n_ra, net.n_i, trial_ms), flush=True)
     ora = OracleSim(net, trial_ms=trial_ms, ra_model=ra_model)
     with _engine(trial_ms=trial_ms, ra_model=ra_model) as eng:
         eng.upload(net)
-        eng.set_tuning(ra_lanes=ra_lanes)
+        eng.set_tuning(ra_lanes=ra_lanes, i_lanes=i_lanes)
         for tr in range(3):
             kw = dict(mode=0, training=True, trial_index=tr, first_trial=(tr == 0),
                       mean_injection_time=100.0 + 10 * tr)

@@ -209,6 +209,11 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     *out = ctx;
     HVC_TRY(hipSetDevice(device));
     HVC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // default launch shape for one network: HVC-I in quads in the FP32 form
+    // (the DPP exchange shortens DP8's dependent chain), five lanes in FP64
+    // (there the extra gate per lane costs more than the exchange saves)
+    ctx->tuning.i_lanes = prm->precision == 1 ? 2 : 1;
+    ctx->tuning.ra_lanes = 0;
 
     // trial clock and synchronisation points (NGS:8906-8925, 9291, 9563)
     const double dt = prm->dt;

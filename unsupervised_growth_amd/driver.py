@@ -192,7 +192,7 @@ class GrowthDriver:
         d["num_test_trials"] = num_trials
         return d
 
-    def set_tuning(self, integrate_block, i_lanes, deliver_block, sync_block, ra_lanes=1):
+    def set_tuning(self, integrate_block, i_lanes, deliver_block, sync_block, ra_lanes=0):
         """Launch shapes of the driver's engine (kept across rebuilds)."""
         t = _abi.Tuning(integrate_block, i_lanes, deliver_block, sync_block, ra_lanes)
         st = self.lib.hvc_set_tuning(self.engine_ctx(), ctypes.byref(t))
