@@ -45,6 +45,11 @@ struct Guard {
 #define CK(n, k, d) ((k) * (real)((double)(n) / (double)(d)))
 #define DIVR(a, b) ((real)(a) * __builtin_amdgcn_rcpf((real)(b)))
 #define EXPR(x) ::hvc::p32::fexp(x)
+// the neuron arithmetic of this form may fuse a * b + c into one rounding
+#define HVC_FP_FAST _Pragma("clang fp contract(fast)")
+// keeps a value computed in every lane (the form's quotients are one
+// reciprocal: cheaper than a divergent branch around them)
+#define HVC_KEEP(x) asm volatile("" : "+v"(x))
 // exp(x) = 2^(x log2 e): the product x log2 e is carried as p + e (fma gives
 // the rounding error of x L_hi exactly, x L_lo adds the tail of log2 e), the
 // hardware exp2 evaluates 2^p and 2^e ~ 1 + e ln 2 corrects it; about 2 ulp.
@@ -83,6 +88,8 @@ struct Guard : DivGuard {
 #define CK(n, k, d) CD((n) * (k), d)
 #define DIVR(a, b) ((a) / (b))
 #define EXPR(x) ::hvc::dexp(x)
+#define HVC_FP_FAST
+#define HVC_KEEP(x)
 #define HVC_ACC(st, f, inc) st.f = st.f + inc;
 #endif
 
@@ -126,21 +133,21 @@ constexpr real kRA_GsL = HR(0.1), kRA_GdL = HR(0.1), kRA_EdCa = HR(120.0), kRA_E
 constexpr real kRA_EsNa = HR(55.0), kRA_EsK = -HR(90.0), kRA_Egaba = -HR(80.0);
 
 template <bool EXACT>
-__device__ __forceinline__ real nInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-(v + 35), 10))); }
+__device__ __forceinline__ real nInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 35), 10))); }
 template <bool EXACT>
-__device__ __forceinline__ real tauN(real v, Guard& bad) { return HR(0.1) + DIVR(HR(0.5), 1 + EXPR(CD(v + 27, 15))); }
+__device__ __forceinline__ real tauN(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.5), 1 + EXPR(CD(v + 27, 15))); }
 template <bool EXACT>
-__device__ __forceinline__ real hInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(v + 45, 7))); }
+__device__ __forceinline__ real hInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(v + 45, 7))); }
 template <bool EXACT>
-__device__ __forceinline__ real tauH(real v, Guard& bad) { return HR(0.1) + DIVR(HR(0.75), 1 + EXPR(CD(v + HR(40.5), 6))); }
+__device__ __forceinline__ real tauH(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.75), 1 + EXPR(CD(v + HR(40.5), 6))); }
 template <bool EXACT>
-__device__ __forceinline__ real mInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-(v + 30), HR(9.5)))); }
+__device__ __forceinline__ real mInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 30), HR(9.5)))); }
 template <bool EXACT>
-__device__ __forceinline__ real lInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-(v + 40), 5))); }
+__device__ __forceinline__ real lInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 40), 5))); }
 template <bool EXACT>
-__device__ __forceinline__ real rInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-(v + 5), 10))); }
+__device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 5), 10))); }
 template <bool EXACT>
-__device__ __forceinline__ real cInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-(v - 10), 7))); }
+__device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v - 10), 7))); }
 
 struct RAK {
     real Vs, Vd, n, l, h, r, c, Ca;
@@ -172,6 +179,7 @@ __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, r
                                         const RAGates& g, real vs, real vd, real n, real h, real l, real r,
                                         real c, real ca, real E)
 {
+    HVC_FP_FAST
     RAK k;
     real m = g.m;
     real m3 = m * m * m;
@@ -228,6 +236,7 @@ __constant__ RAGateSlot kRAGates[8] = {
 template <bool EXACT>
 __device__ __forceinline__ real ra_gate(Guard& bad, const RAGateSlot& g, real vs, real vd)
 {
+    HVC_FP_FAST
     real x = (g.dend ? vd : vs) + g.d;
     real q = DIVR(g.c, 1 + EXPR(CDV(g.neg ? -x : x, g.e, g.inv_e)));
     return g.a + q;   // a = 0: 0 + q = q (q > 0)
@@ -237,14 +246,14 @@ __device__ __forceinline__ real ra_gate(Guard& bad, const RAGateSlot& g, real vs
 template <int K>
 __device__ __forceinline__ float quad_bcast(float x)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), K * 0x55, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), K * 0x55, 0xF, 0xF, true));
 }
 template <int K>
 __device__ __forceinline__ double quad_bcast(double x)
 {
     const long long b = __double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, K * 0x55, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), K * 0x55, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, K * 0x55, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), K * 0x55, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
@@ -277,6 +286,7 @@ __device__ __forceinline__ void ra_debraband(Guard& bad, RAState& s, real EL, re
                                              real J1s, real J2s, real J1d, real J2d,
                                              real E_dt, real E_half)
 {
+    HVC_FP_FAST
     const real dt = u.dt;
     real H1Vs = s.Vs + CD(u.sq_sig_s * (-HR(0.01844540496323970) * J1s - HR(0.1866426386543421) * J2s), 5000);
     real H1Vd = s.Vd + CD(u.sq_sig_d * (-HR(0.01844540496323970) * J1d - HR(0.1866426386543421) * J2d), 10000);
@@ -324,6 +334,7 @@ template <bool EXACT, bool LANES>
 __device__ __forceinline__ void ra_runge4(Guard& bad, RAState& s, real EL, real GCa, const RAUniform& u,
                                           const RAGateSlot* sl, real E_third, real E_twothird, real E_dt)
 {
+    HVC_FP_FAST
     const real dt = u.dt;
     RAK k1 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs, s.Vd, s.n, s.h, s.r, s.c, s.Ca, HR(1.0));
     RAK k2 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs + CD(dt * k1.Vs, 3), s.Vd + CD(dt * k1.Vd, 3),
@@ -383,19 +394,19 @@ constexpr real kI_gL = HR(0.1), kI_El = -HR(65.0), kI_gNa = HR(100.0), kI_Ena = 
 constexpr real kI_Ek = -HR(80.0), kI_gKHT = HR(500.0), kI_Ei = -HR(75.0), kI_A = HR(6000.0);
 
 template <bool EXACT>
-__device__ __forceinline__ real i_an(real v, Guard& bad) { return DIVR(HR(0.15) * (v + 15), 1 - EXPR(CD(-(v + 15), 10))); }
+__device__ __forceinline__ real i_an(real v, Guard& bad) { HVC_FP_FAST return DIVR(HR(0.15) * (v + 15), 1 - EXPR(CD(-(v + 15), 10))); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bn(real v, Guard& bad) { return HR(0.2) * EXPR(CD(-(v + 25), 80)); }
+__device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * EXPR(CD(-(v + 25), 80)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_am(real v, Guard& bad) { return DIVR(v + 22, 1 - EXPR(CD(-(v + 22), 10))); }
+__device__ __forceinline__ real i_am(real v, Guard& bad) { HVC_FP_FAST return DIVR(v + 22, 1 - EXPR(CD(-(v + 22), 10))); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bm(real v, Guard& bad) { return 40 * EXPR(CD(-(v + 47), 18)); }
+__device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * EXPR(CD(-(v + 47), 18)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_ah(real v, Guard& bad) { return HR(0.7) * EXPR(CD(-(v + 34), 20)); }
+__device__ __forceinline__ real i_ah(real v, Guard& bad) { HVC_FP_FAST return HR(0.7) * EXPR(CD(-(v + 34), 20)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bh(real v, Guard& bad) { return DIVR(10, 1 + EXPR(CD(-(v + 4), 10))); }
+__device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return DIVR(10, 1 + EXPR(CD(-(v + 4), 10))); }
 template <bool EXACT>
-__device__ __forceinline__ real i_wInf(real v, Guard& bad) { return DIVR(1, 1 + EXPR(CD(-v, 5))); }
+__device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-v, 5))); }
 
 struct IK {
     real V, n, m, h, w;
@@ -406,6 +417,7 @@ template <bool EXACT>
 __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1, real m1, real h1, real w1,
                                     real e2, real e5)
 {
+    HVC_FP_FAST
     IK k;
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
@@ -422,6 +434,7 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
 template <bool EXACT>
 __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const DecayRow& row)
 {
+    HVC_FP_FAST
     IK k1 = i_rhs<EXACT>(bad, s, s.V, s.n, s.m, s.h, s.w, HR(1.0), HR(1.0));
     IK k2 = i_rhs<EXACT>(bad, s, s.V + CD(dt * k1.V, 18), s.n + CD(dt * k1.n, 18), s.m + CD(dt * k1.m, 18), s.h + CD(dt * k1.h, 18),
                   s.w + CD(dt * k1.w, 18), row.e2[kOffDp2], row.e5[kOffDp2]);
@@ -489,6 +502,7 @@ __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Deca
 template <bool EXACT>
 __device__ __forceinline__ void i_runge4(Guard& bad, IState& s, real dt, const DecayRow& row)
 {
+    HVC_FP_FAST
     IK k1 = i_rhs<EXACT>(bad, s, s.V, s.n, s.m, s.h, s.w, HR(1.0), HR(1.0));
     IK k2 = i_rhs<EXACT>(bad, s, s.V + CD(dt * k1.V, 3), s.n + CD(dt * k1.n, 3), s.m + CD(dt * k1.m, 3), s.h + CD(dt * k1.h, 3),
                   s.w + CD(dt * k1.w, 3), row.e2[kOffThird], row.e5[kOffThird]);
@@ -597,11 +611,13 @@ struct GateSlot {
 template <bool EXACT>
 __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
 {
+    HVC_FP_FAST
     real vd = v + g.d;
     real E = EXPR(CDV(-vd, g.e, g.inv_e));
     real num = (g.type == 0) ? g.c * vd : g.c;
     real den = (g.type == 0) ? (1 - E) : (1 + E);
     real q = DIVR(num, den);
+    HVC_KEEP(q);
     return (g.type == 1) ? g.c * E : q;
 }
 
@@ -618,6 +634,7 @@ template <bool EXACT>
 __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role, real v, real n1, real m1, real h1,
                                              real w1, real e2, real e5)
 {
+    HVC_FP_FAST
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
     real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
@@ -675,6 +692,7 @@ template <bool EXACT>
 __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, int role, int base, real dt,
                                             const DecayRow& row)
 {
+    HVC_FP_FAST
     const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
     real x;
@@ -726,36 +744,37 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, in
 #undef HVC_BCAST5
 
 // ---- HVC-I, DP8, four lanes per neuron (quad) ----
-// Lane q of a neuron's quad owns V (q = 0), n, m or h (q = 1..3) and evaluates
-// that variable's two gate rates; every lane also evaluates wInf and carries
-// w itself, so the four owned stage values travel by DPP quad broadcasts.
-// Every value is the expression tree of HHI_buffer, as in i_dp8 and the
-// five-lane form.
+// Lane q of a neuron's quad owns n, m or h (q = 1..3) and evaluates that
+// variable's two gate rates; lane 0 owns V and w and evaluates wInf (V has no
+// rates).  Only V and the three gates are needed by other lanes, so four
+// stage values travel by DPP quad broadcasts and w stays in lane 0.  Every
+// value is the expression tree of HHI_buffer, as in i_dp8 and the five-lane
+// form.
 __constant__ GateSlot kIQuadGates[4][2] = {
-    {{1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                  // V: no rates (unused)
+    {{2, HR(1.0), HR(0.0), HR(5.0), HR(0.2)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                   // V, w: wInf (second slot unused)
     {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1)}, {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0)}},        // n: an, bn
     {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1)}, {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0)}},        // m: am, bm
     {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0)}, {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1)}},         // h: ah, bh
 };
 
-// derivatives of the lane's variable (kx) and of w (kw) at stage state (v, n1, m1, h1, w1)
+// derivatives of the lane's variable (kx) and, in lane 0, of w (kw; 0 in the
+// other lanes) at stage state (v, n1, m1, h1, w1); w1 is valid in lane 0 only
 template <bool EXACT>
 __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role, const GateSlot& ga,
                                            const GateSlot& gb, real v, real n1, real m1, real h1, real w1, real e2,
                                            real e5, real& kx, real& kw)
 {
-    const GateSlot winf = {2, HR(1.0), HR(0.0), HR(5.0), HR(0.2)};   // wInf, as the five-lane table
+    HVC_FP_FAST
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
     real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
                 - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
     real a = gate_rate<EXACT>(bad, ga, v);
     real b = gate_rate<EXACT>(bad, gb, v);
-    real wi = gate_rate<EXACT>(bad, winf, v);
     real x = role == 1 ? n1 : (role == 2 ? m1 : h1);
     real kg = a * (1 - x) - b * x;
     kx = role == 0 ? kV : kg;
-    kw = (wi - w1);
+    kw = role == 0 ? (a - w1) : HR(0.0);   // a = wInf in lane 0
 }
 
 #define HVC_BCAST4(x)              \
@@ -764,12 +783,13 @@ __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role
     m1 = quad_bcast<2>(x);         \
     h1 = quad_bcast<3>(x);
 
-// HHI_buffer::DP8_step with four lanes per neuron; s is replicated in the quad;
-// xlo / wlo: low parts of the lane's variable and of w (FP32 form)
+// HHI_buffer::DP8_step with four lanes per neuron; s is replicated in the quad
+// except w (lane 0); xlo / wlo: low parts of the lane's variable and of w (FP32 form)
 template <bool EXACT>
 __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, real& wlo, int role, const GateSlot& ga,
                                            const GateSlot& gb, real dt, const DecayRow& row)
 {
+    HVC_FP_FAST
     const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
     const real y0 = s.w;
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
