@@ -37,6 +37,7 @@ void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, cons
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
 void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu);
 void launch_sort_posts(hipStream_t st, const NetDev& nd);
+void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int gen_ra, int gen_i);
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
@@ -527,6 +528,18 @@ struct Caps {
     long long log_cap = 0;
 };
 
+// entries per neuron of a chunk's noise-event list: the chunk spans at most
+// kNoiseChunkWindows windows of (sync_ms / dt) + 2 steps; HVC-RA checks fire at
+// 4 x 100 Hz and HVC-I checks at 2 x 250 Hz, so a list holds at most
+// 0.5 events per ms on average -- sized for mean + 12 standard deviations plus
+// slack and the two end marks (an overflow is reported as a capacity error)
+static int noise_cap(const hvc_params* prm)
+{
+    const int steps = std::min(64, (int)(prm->sync_ms / prm->dt) + 2);
+    const double mean = 0.5 * kNoiseChunkWindows * steps * prm->dt;
+    return (int)std::ceil(mean + 12.0 * std::sqrt(mean)) + 18;
+}
+
 // allocates every array of ctx->nd at capacity c (contents undefined)
 static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
 {
@@ -564,6 +577,11 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.i_flag = dalloc<int>(ctx, ni, &st);
     nd.i_noise = dalloc<double>(ctx, 2 * (size_t)ni, &st);
     nd.i_draw = dalloc<uint32_t>(ctx, ni, &st);
+    nd.ev_cap = noise_cap(&ctx->prm);
+    nd.ra_ev = dalloc<NoiseEvent>(ctx, (size_t)nra * nd.ev_cap, &st);
+    nd.i_ev = dalloc<NoiseEvent>(ctx, (size_t)ni * nd.ev_cap, &st);
+    nd.ra_ev_pos = dalloc<int>(ctx, nra, &st);
+    nd.i_ev_pos = dalloc<int>(ctx, ni, &st);
     nd.soma_mask = dalloc<uint64_t>(ctx, nra, &st);
     nd.dend_mask = dalloc<uint64_t>(ctx, nra, &st);
     nd.i_mask = dalloc<uint64_t>(ctx, ni, &st);
@@ -972,6 +990,15 @@ extern "C" hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, doub
 
 // One trial of every network in the context (trials[b] for network b); the
 // networks advance together, one launch per kernel and coupling window.
+// noise events of the chunk that starts at window w (k_noise_events)
+static void noise_chunk(hvc_ctx* ctx, const NetDev& nd, int w)
+{
+    const int gen_ra = ctx->prm.ra_model == 1 && ctx->prm.ra_poisson, gen_i = ctx->prm.i_poisson != 0;
+    if (w % kNoiseChunkWindows != 0 || !(gen_ra || gen_i)) return;
+    const int last = std::min(w + kNoiseChunkWindows, (int)ctx->windows.size()) - 1;
+    launch_noise_events(ctx->stream, nd, ctx->windows[w].first, ctx->windows[last].second, gen_ra, gen_i);
+}
+
 static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs)
 {
     const int B = ctx->n_nets;
@@ -1024,6 +1051,7 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
         const NetDev& nv = ndw[w & 1];
         bool tm = ctx->timing && (w % 16 == 7);
+        noise_chunk(ctx, nd, w);
         tstart(ctx, kTInt, tm);
         if (ctx->prm.precision == 1)
             launch_integrate_f32(s, nv, t0, t1, ctx->tuning);
@@ -1191,6 +1219,7 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
     hvc_status r = HVC_OK;
     for (int w = 0; w < nw; ++w) {
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
+        noise_chunk(ctx, nd, w);
         hipEventRecord(ev[6 * w + 0], s);
         if (ctx->prm.precision == 1)
             launch_integrate_f32(s, nd, t0, t1, tu);
@@ -1286,9 +1315,10 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     const double logs = 3.0 * spikes_per_neuron * sizeof(int2);
     const double inflight = 2.0 * kMaxInflight * sizeof(int) + 2 * sizeof(int);
     PoolCost c;
+    const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
     c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
-           4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs;
-    c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs;
+           4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise;
+    c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise;
     c.rr = 4 + 8 + 4 + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;
     c.a2i = 4 + 8 + 8;
     c.i2ra = 4 + 8 + 8;

@@ -145,6 +145,40 @@ struct Win {
     __device__ __forceinline__ double clk(int t) const { return clock[t - t0 + 1]; }
 };
 
+// The neuron's next two noise events of the chunk (lists are generated by
+// k_noise_events).  take() applies every event of step t, in list order; the
+// following entry is loaded a step or more before it is looked at, except
+// when two events fall on one step.
+struct NoiseCursor {
+    const NoiseEvent* ev;
+    int pos;
+    NoiseEvent cur, nxt;
+    __device__ __forceinline__ void open(const NoiseEvent* list, int p)
+    {
+        ev = list;
+        pos = p;
+        cur = ev[p];
+        nxt = ev[p + 1];
+    }
+    __device__ __forceinline__ void none()
+    {
+        ev = nullptr;
+        pos = 0;
+        cur.step = kNoNoiseStep;
+        nxt.step = kNoNoiseStep;
+    }
+    template <class Apply>
+    __device__ __forceinline__ void take(int t, Apply apply)
+    {
+        while (cur.step == t) {
+            apply(cur.target, cur.inc);
+            cur = nxt;
+            ++pos;
+            nxt = ev[pos + 1];
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // single-neuron traces
 // ---------------------------------------------------------------------------
@@ -342,12 +376,11 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     int new_posts = 0;
 
     const NoiseId sid = ra_noise_id(nd, ii);
-    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamRAPoisson, sid.local, nd.ra_draw[ii]};
-    double nes = 0, nis = 0, ned = 0, nid = 0;
-    if (MODEL == 1 && nd.ra_poisson) {
-        nes = nd.ra_noise[ii]; nis = nd.ra_noise[(size_t)nd.n_ra + ii];
-        ned = nd.ra_noise[2 * (size_t)nd.n_ra + ii]; nid = nd.ra_noise[3 * (size_t)nd.n_ra + ii];
-    }
+    NoiseCursor nc;
+    if (MODEL == 1 && nd.ra_poisson)
+        nc.open(nd.ra_ev + (size_t)ii * nd.ev_cap, nd.ra_ev_pos[ii]);
+    else
+        nc.none();
 
     uint64_t smask = 0, dmask = 0;
     for (int t = t0; t <= t1; ++t) {
@@ -360,13 +393,14 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
                          kPointDistribution[point_index(o.z)], kPointDistribution[point_index(o.w)],
                          row.e5[kOffDt], row.e5[kOffHalf]);
         } else {
-            if (nd.ra_poisson) {
-                double time = win.clk(t - 1);
-                ra_noise_check(ps, time, nd.dt, s.Gi_s, 0.035, nis);
-                ra_noise_check(ps, time, nd.dt, s.Gi_d, 0.045, nid);
-                ra_noise_check(ps, time, nd.dt, s.Ge_s, 0.035, nes);
-                ra_noise_check(ps, time, nd.dt, s.Ge_d, 0.045, ned);
-            }
+            // HH2_buffer noise checks (Gi_s, Gi_d, Ge_s, Ge_d), as listed
+            // (selects, not stores through an index: the state stays in registers)
+            nc.take(t, [&](int g, double inc) {
+                s.Gi_s = g == 0 ? (real)(s.Gi_s + inc) : s.Gi_s;
+                s.Gi_d = g == 1 ? (real)(s.Gi_d + inc) : s.Gi_d;
+                s.Ge_s = g == 2 ? (real)(s.Ge_s + inc) : s.Ge_s;
+                s.Ge_d = g == 3 ? (real)(s.Ge_d + inc) : s.Ge_d;
+            });
             ra_step_runge4<LANES>(s, el, gca, u, row.e5[kOffThird], row.e5[kOffTwoThird], row.e5[kOffDt], gs);
         }
         bool soma = live && leader && (f & 1);
@@ -397,11 +431,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
         nd.soma_mask[i] = smask;
         nd.dend_mask[i] = dmask;
         if (growth) nd.post_pend[i] = new_posts;
-        if (MODEL == 1 && nd.ra_poisson) {
-            nd.ra_noise[i] = nes; nd.ra_noise[(size_t)nd.n_ra + i] = nis;
-            nd.ra_noise[2 * (size_t)nd.n_ra + i] = ned; nd.ra_noise[3 * (size_t)nd.n_ra + i] = nid;
-            nd.ra_draw[i] = ps.draw;
-        }
+        if (MODEL == 1 && nd.ra_poisson) nd.ra_ev_pos[i] = nc.pos;
     }
 }
 
@@ -422,17 +452,19 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
     }
     s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
-    const NoiseId sid = i_noise_id(nd, ii);
-    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
-    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    NoiseCursor nc;
+    if (nd.i_poisson)
+        nc.open(nd.i_ev + (size_t)ii * nd.ev_cap, nd.i_ev_pos[ii]);
+    else
+        nc.none();
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
-        if (nd.i_poisson) {
-            double time = win.clk(t - 1);
-            i_noise_check(ps, time, nd.dt, s.Ge, ne);
-            i_noise_check(ps, time, nd.dt, s.Gi, ni);
-        }
+        // HHI_buffer noise checks (Ge, Gi), as listed
+        nc.take(t, [&](int g, double inc) {
+            s.Ge = g == 0 ? (real)(s.Ge + inc) : s.Ge;
+            s.Gi = g == 1 ? (real)(s.Gi + inc) : s.Gi;
+        });
         i_step<MODEL>(s, (real)nd.dt, win.row(t));
         bool fired = live && f;
         if (fired) mask |= 1ull << (t - t0);
@@ -443,8 +475,7 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
         nd.in[3][i] = HVC_WIDE(s, h); nd.in[4][i] = HVC_WIDE(s, w);
         nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
         nd.i_flag[i] = flag;
-        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
-        nd.i_draw[i] = ps.draw;
+        if (nd.i_poisson) nd.i_ev_pos[i] = nc.pos;
         nd.i_mask[i] = mask;
     }
 }
@@ -478,9 +509,11 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     ge = __shfl(ge, base);
     s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
-    const NoiseId sid = i_noise_id(nd, ii);
-    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
-    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    NoiseCursor nc;
+    if (nd.i_poisson)
+        nc.open(nd.i_ev + (size_t)ii * nd.ev_cap, nd.i_ev_pos[ii]);
+    else
+        nc.none();
     // low part of this lane's own variable (FP32 form; zero in FP64)
     real xlo = 0;
 #ifdef HVC_FP32
@@ -489,11 +522,11 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
-        if (nd.i_poisson) {
-            double time = win.clk(t - 1);
-            i_noise_check(ps, time, nd.dt, s.Ge, ne);
-            i_noise_check(ps, time, nd.dt, s.Gi, ni);
-        }
+        // HHI_buffer noise checks (Ge, Gi), as listed
+        nc.take(t, [&](int g, double inc) {
+            s.Ge = g == 0 ? (real)(s.Ge + inc) : s.Ge;
+            s.Gi = g == 1 ? (real)(s.Gi + inc) : s.Gi;
+        });
         i_step_lanes(s, xlo, role, base, (real)nd.dt, win.row(t));
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
@@ -511,8 +544,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     if (live && role == 0) {
         nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
         nd.i_flag[i] = flag;
-        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
-        nd.i_draw[i] = ps.draw;
+        if (nd.i_poisson) nd.i_ev_pos[i] = nc.pos;
         nd.i_mask[i] = mask;
     }
 }
@@ -540,9 +572,11 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     ge = quad_bcast<0>(ge);
     s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
-    const NoiseId sid = i_noise_id(nd, ii);
-    PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[ii]};
-    double ne = nd.i_noise[ii], ni = nd.i_noise[(size_t)nd.n_i + ii];
+    NoiseCursor nc;
+    if (nd.i_poisson)
+        nc.open(nd.i_ev + (size_t)ii * nd.ev_cap, nd.i_ev_pos[ii]);
+    else
+        nc.none();
     // low parts of this lane's variable and of w (FP32 form; zero in FP64)
     real xlo = 0, wlo = 0;
 #ifdef HVC_FP32
@@ -552,11 +586,11 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     uint64_t mask = 0;
     for (int t = t0; t <= t1; ++t) {
         int f = i_state_check(s.V, flag);
-        if (nd.i_poisson) {
-            double time = win.clk(t - 1);
-            i_noise_check(ps, time, nd.dt, s.Ge, ne);
-            i_noise_check(ps, time, nd.dt, s.Gi, ni);
-        }
+        // HHI_buffer noise checks (Ge, Gi), as listed
+        nc.take(t, [&](int g, double inc) {
+            s.Ge = g == 0 ? (real)(s.Ge + inc) : s.Ge;
+            s.Gi = g == 1 ? (real)(s.Gi + inc) : s.Gi;
+        });
         i_step_quad(s, xlo, wlo, role, ga, gb, (real)nd.dt, win.row(t));
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
@@ -576,8 +610,7 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     if (live && role == 0) {
         nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
         nd.i_flag[i] = flag;
-        nd.i_noise[i] = ne; nd.i_noise[(size_t)nd.n_i + i] = ni;
-        nd.i_draw[i] = ps.draw;
+        if (nd.i_poisson) nd.i_ev_pos[i] = nc.pos;
         nd.i_mask[i] = mask;
     }
 }

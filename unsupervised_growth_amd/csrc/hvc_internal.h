@@ -22,7 +22,21 @@ enum ErrFlag : uint32_t {
     kErrInflight = 8u,
     kErrInbox = 16u,
     kErrPostRing = 32u,
+    kErrNoise = 64u,
 };
+
+// Poisson noise events of one neuron, generated ahead of the integration for a
+// chunk of coupling windows (the draws depend on the stream and the clock
+// only): at step `step` add `inc` to conductance `target` (HVC-RA 0 Gi_s,
+// 1 Gi_d, 2 Ge_s, 3 Ge_d; HVC-I 0 Ge, 1 Gi), in list order.  A list ends with
+// two entries of step kNoNoiseStep.
+struct NoiseEvent {
+    int step;
+    int target;
+    double inc;
+};
+constexpr int kNoNoiseStep = 0x7fffffff;
+constexpr int kNoiseChunkWindows = 64;   // windows per generated chunk
 
 // post-event key: network (12 bits), step (24 bits), neuron (28 bits); the
 // sorted keys list each network's events by (step, neuron)
@@ -112,6 +126,13 @@ struct NetDev {
     int* i_flag;
     double* i_noise;             // [2][n_i]: exc inh
     uint32_t* i_draw;
+    // noise events of the current chunk: [n][ev_cap] per neuron, and the index
+    // of each neuron's next unapplied event
+    NoiseEvent* ra_ev;
+    NoiseEvent* i_ev;
+    int* ra_ev_pos;
+    int* i_ev_pos;
+    int ev_cap;
 
     // per-window outputs
     uint64_t* soma_mask;

@@ -94,6 +94,130 @@ __global__ void k_reset(NetDev nd)
     if (i < 3 * nd.n_nets) nd.warn[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Poisson noise events of a chunk of steps [tb, te].  The noise checks of
+// HH2_buffer (HH2_buffer.cpp:592-607) and HHI_buffer (HHI_buffer.cpp:278-294)
+// fire at the first step whose neuron time clock[t - 1] exceeds the channel's
+// noise time, draw from one stream per neuron in the order of the checks, and
+// never look at the neuron's state; so one lane per neuron replays them here,
+// channel by channel in step order, with the same draws, and the integration
+// only adds the listed increments.  The stream position and noise times carry
+// over to the next chunk.
+// ---------------------------------------------------------------------------
+
+// first step t in [tmin, te] with clock[t - 1] > nt; te + 1 if there is none
+__device__ int noise_step(const double* clock, double dt, int tmin, int te, double nt)
+{
+    if (tmin > te) return te + 1;
+    const double c0 = clock[tmin - 1];
+    if (c0 > nt) return tmin;
+    const double steps = (nt - c0) / dt;        // the clock advances by dt per step
+    if (steps > (double)(te - tmin + 2)) return te + 1;
+    int t = tmin + (int)steps;
+    if (t > te + 1) t = te + 1;
+    while (t > tmin && clock[t - 2] > nt) --t;  // now t - 1 does not fire (or t = tmin)
+    while (t <= te && !(clock[t - 1] > nt)) ++t;
+    return t;
+}
+
+struct NoiseOut {
+    NoiseEvent* ev;
+    int n, cap;
+    unsigned* err;
+    __device__ __forceinline__ void put(int step, int target, double inc)
+    {
+        if (n < cap - 2)
+            ev[n++] = NoiseEvent{step, target, inc};
+        else
+            atomicOr(err, (unsigned)kErrNoise);
+    }
+    __device__ __forceinline__ void close()
+    {
+        ev[n] = NoiseEvent{kNoNoiseStep, 0, 0.0};
+        ev[n + 1] = NoiseEvent{kNoNoiseStep, 0, 0.0};
+    }
+};
+
+// one noise check that fired at step t: G += random(G_noise), then the next
+// interval, redrawn (with another increment each time) while shorter than dt
+__device__ __forceinline__ double noise_fire(PoissonStream& ps, NoiseOut& o, int t, int target, double G_noise,
+                                             double lambda, double dt, double noise_time)
+{
+    o.put(t, target, ps.random(G_noise));
+    double random = 1000.0 * ps.spike_time(lambda);
+    while (random < dt) {
+        random = 1000.0 * ps.spike_time(lambda);
+        o.put(t, target, ps.random(G_noise));
+    }
+    return noise_time + random;
+}
+
+// blocks [0, blocks_ra): HVC-RA (RK4 form, channels Gi_s, Gi_d, Ge_s, Ge_d with
+// G_noise 0.035, 0.045, 0.035, 0.045 at 100 Hz); the rest HVC-I (Ge, Gi with
+// 0.45 at 250 Hz)
+__global__ __launch_bounds__(64) void k_noise_events(NetDev nd, int tb, int te, int blocks_ra, int gen_ra, int gen_i)
+{
+    const double dt = nd.dt;
+    if ((int)blockIdx.x < blocks_ra) {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= nd.n_ra) return;
+        NoiseOut o{nd.ra_ev + (size_t)i * nd.ev_cap, 0, nd.ev_cap, nd.err};
+        nd.ra_ev_pos[i] = 0;
+        if (gen_ra) {
+            const NoiseId sid = ra_noise_id(nd, i);
+            PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamRAPoisson, sid.local, nd.ra_draw[i]};
+            const size_t n = (size_t)nd.n_ra;
+            double nt0 = nd.ra_noise[n + i], nt1 = nd.ra_noise[3 * n + i];    // inh soma, inh dend
+            double nt2 = nd.ra_noise[i], nt3 = nd.ra_noise[2 * n + i];        // exc soma, exc dend
+            int s0 = noise_step(nd.clock, dt, tb, te, nt0), s1 = noise_step(nd.clock, dt, tb, te, nt1);
+            int s2 = noise_step(nd.clock, dt, tb, te, nt2), s3 = noise_step(nd.clock, dt, tb, te, nt3);
+            for (;;) {
+                // earliest check; at one step the channels fire in the order of the checks
+                int c = 0, t = s0;
+                if (s1 < t) { c = 1; t = s1; }
+                if (s2 < t) { c = 2; t = s2; }
+                if (s3 < t) { c = 3; t = s3; }
+                if (t > te) break;
+                const double nt = c == 0 ? nt0 : (c == 1 ? nt1 : (c == 2 ? nt2 : nt3));
+                const double nn = noise_fire(ps, o, t, c, (c & 1) ? 0.045 : 0.035, 100.0, dt, nt);
+                const int sn = noise_step(nd.clock, dt, t + 1, te, nn);
+                if (c == 0) { nt0 = nn; s0 = sn; }
+                else if (c == 1) { nt1 = nn; s1 = sn; }
+                else if (c == 2) { nt2 = nn; s2 = sn; }
+                else { nt3 = nn; s3 = sn; }
+            }
+            nd.ra_noise[n + i] = nt0; nd.ra_noise[3 * n + i] = nt1;
+            nd.ra_noise[i] = nt2; nd.ra_noise[2 * n + i] = nt3;
+            nd.ra_draw[i] = ps.draw;
+        }
+        o.close();
+    } else {
+        const int i = (blockIdx.x - blocks_ra) * blockDim.x + threadIdx.x;
+        if (i >= nd.n_i) return;
+        NoiseOut o{nd.i_ev + (size_t)i * nd.ev_cap, 0, nd.ev_cap, nd.err};
+        nd.i_ev_pos[i] = 0;
+        if (gen_i) {
+            const NoiseId sid = i_noise_id(nd, i);
+            PoissonStream ps{sid.k0, sid.k1, sid.trial, kStreamIPoisson, sid.local, nd.i_draw[i]};
+            const size_t n = (size_t)nd.n_i;
+            double nt0 = nd.i_noise[i], nt1 = nd.i_noise[n + i];               // exc, inh
+            int s0 = noise_step(nd.clock, dt, tb, te, nt0), s1 = noise_step(nd.clock, dt, tb, te, nt1);
+            for (;;) {
+                int c = 0, t = s0;
+                if (s1 < t) { c = 1; t = s1; }
+                if (t > te) break;
+                const double nn = noise_fire(ps, o, t, c, 0.45, 250.0, dt, c == 0 ? nt0 : nt1);
+                const int sn = noise_step(nd.clock, dt, t + 1, te, nn);
+                if (c == 0) { nt0 = nn; s0 = sn; }
+                else { nt1 = nn; s1 = sn; }
+            }
+            nd.i_noise[i] = nt0; nd.i_noise[n + i] = nt1;
+            nd.i_draw[i] = ps.draw;
+        }
+        o.close();
+    }
+}
+
 __global__ void k_clear_pre(int* pre_cnt, int E)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -671,6 +795,12 @@ __global__ void k_add_i32(int* a, long long n, int delta)
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta)
 {
     if (n > 0) k_add_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, n, delta);
+}
+
+void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int gen_ra, int gen_i)
+{
+    const int bra = blocks_for(nd.n_ra, 64), bi = blocks_for(nd.n_i, 64);
+    k_noise_events<<<bra + bi, 64, 0, st>>>(nd, tb, te, bra, gen_ra, gen_i);
 }
 
 void launch_sort_posts(hipStream_t st, const NetDev& nd)
