@@ -27,16 +27,18 @@ void launch_trace_ra(hipStream_t st, const NetDev& nd, int n, int n_steps, uint3
 void launch_trace_i(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                     const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
 void launch_reset(hipStream_t st, const NetDev& nd, int E);
-void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
+void launch_integrate(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu, const StdpArgs& sa,
+                      int stdp);
 void launch_trace_ra_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, uint32_t trial, const double* EL,
                          const double* GCa, const double* kick, const double* exc_w, const double* inh_g, double* out,
                          uint8_t* fired);
 void launch_trace_i_f32(hipStream_t st, const NetDev& nd, int n, int n_steps, int first_trial, uint32_t trial,
                         const double* exc_g, const double* inh_g, double* out, uint8_t* fired);
-void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
+void launch_integrate_f32(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu, const StdpArgs& sa,
+                          int stdp);
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu);
 void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu);
-void launch_sort_posts(hipStream_t st, const NetDev& nd);
+void launch_stdp(hipStream_t st, const NetDev& nd, const StdpArgs& sa);
 void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int gen_ra, int gen_i);
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta);
 void launch_decay(hipStream_t st, const NetDev& nd);
@@ -601,6 +603,7 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.post_list = dalloc<uint64_t>(ctx, 2 * (size_t)kPostListCap, &st);
     nd.post_list_cnt = dalloc<int>(ctx, 2, &st);
     nd.post_list_next_cnt = nd.post_list_cnt ? nd.post_list_cnt + 1 : nullptr;
+    nd.last_pre = dalloc<double>(ctx, nra, &st);
     ctx->d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
     ctx->d_rr_tgt = dalloc<int>(ctx, E, &st);
     ctx->d_rr_d = dalloc<double>(ctx, E, &st);
@@ -1053,15 +1056,22 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
         bool tm = ctx->timing && (w % 16 == 7);
         noise_chunk(ctx, nd, w);
         tstart(ctx, kTInt, tm);
+        // the previous window's sync-point STDP runs in blocks of this window's integration
+        const NetDev& pv = ndw[(w + 1) & 1];
+        const StdpArgs sa{pv.post_list, pv.post_list_cnt, w > 0 ? ctx->windows[w - 1].second : 0};
+        const int stdp = any_growth && w > 0;
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nv, t0, t1, ctx->tuning);
+            launch_integrate_f32(s, nv, t0, t1, ctx->tuning, sa, stdp);
         else
-            launch_integrate(s, nv, t0, t1, ctx->tuning);
+            launch_integrate(s, nv, t0, t1, ctx->tuning, sa, stdp);
         tstop(ctx, kTInt, tm);
         tstart(ctx, kTDeliver, tm);
-        if (any_growth) launch_sort_posts(s, nv);
         launch_deliver(s, nv, t0, t1, ctx->tuning);
         tstop(ctx, kTDeliver, tm);
+    }
+    if (any_growth && nw > 0) {
+        const NetDev& lv = ndw[(nw - 1) & 1];
+        launch_stdp(s, nd, StdpArgs{lv.post_list, lv.post_list_cnt, ctx->windows[nw - 1].second});
     }
     tstart(ctx, kTSync, ctx->timing);
     launch_sync(s, nd, ctx->tuning);
@@ -1221,10 +1231,11 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
         noise_chunk(ctx, nd, w);
         hipEventRecord(ev[6 * w + 0], s);
+        const StdpArgs none{nd.post_list, nd.post_list_cnt, 0};
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1, tu);
+            launch_integrate_f32(s, nd, t0, t1, tu, none, 0);
         else
-            launch_integrate(s, nd, t0, t1, tu);
+            launch_integrate(s, nd, t0, t1, tu, none, 0);
         hipEventRecord(ev[6 * w + 1], s);
         launch_deliver(s, nd, t0, t1, tu);
         hipEventRecord(ev[6 * w + 2], s);
@@ -1316,7 +1327,7 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     const double inflight = 2.0 * kMaxInflight * sizeof(int) + 2 * sizeof(int);
     PoolCost c;
     const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
-    c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
+    c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 8 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
            4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise;
     c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise;
     c.rr = 4 + 8 + 4 + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;

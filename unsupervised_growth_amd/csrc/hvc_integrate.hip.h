@@ -11,6 +11,7 @@
 
 #include "hvc_internal.h"
 #include "hvc_neuron.hip.h"
+#include "hvc_stdp.hip.h"
 
 namespace hvc {
 
@@ -616,14 +617,22 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
 }
 
 // blocks [0, iblocks) integrate HVC-I (five lanes per neuron, 12 neurons per
-// wave; four lanes, 16 per wave; or one thread per neuron), the rest HVC-RA
-// (four lanes per neuron, 16 neurons per wave, or one thread per neuron).
+// wave; four lanes, 16 per wave; or one thread per neuron), the next rblocks
+// HVC-RA (four lanes per neuron, 16 neurons per wave, or one thread per
+// neuron); any blocks after them run the previous window's sync-point STDP
+// (one row per thread, sa).
 // LANES: bit 0 = HVC-I five lanes, bit 1 = HVC-RA four lanes, bit 2 = HVC-I
 // four lanes (the layouts are template parameters: a kernel holds the register
 // allocation of one layout only)
 template <int RA_MODEL, int I_MODEL, int LANES>
-__global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks)
+__global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks, int rblocks, StdpArgs sa)
 {
+    if ((int)blockIdx.x >= iblocks + rblocks) {
+        __shared__ uint64_t stdp_slice[kStdpSliceCap];
+        __shared__ int stdp_count;
+        stdp_block(nd, sa, ((int)blockIdx.x - iblocks - rblocks) * blockDim.x, stdp_slice, &stdp_count);
+        return;
+    }
     // stage the window's decay rows and clock in LDS (dynamic shared memory of
     // win_bytes(t1 - t0 + 1) bytes)
     extern __shared__ double win_lds[];
@@ -675,7 +684,8 @@ void HVC_LAUNCHER(launch_trace_i)(hipStream_t st, const NetDev& nd, int n, int n
     k_trace_i<<<blocks_for(n, 64), 64, 0, st>>>(nd, n, n_steps, first_trial, trial, exc_g, inh_g, out, fired);
 }
 
-void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
+void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu,
+                                     const StdpArgs& sa, int stdp)
 {
     const int bs = tu.integrate_block;
     // HVC-I layout (DP8 only): 0 one thread, 1 five lanes, 2 four lanes per neuron
@@ -686,9 +696,10 @@ void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, in
     const int ra_per_block = ra_lanes ? (bs / 64) * kRANeuronsPerWave : bs;
     int iblocks = nd.n_i == 0 ? 0 : (nd.n_i + i_per_block - 1) / i_per_block;
     int rblocks = blocks_for(nd.n_ra, ra_per_block);
-    dim3 grid(iblocks + rblocks);
+    const int sblocks = stdp ? blocks_for(nd.n_ra, bs) : 0;
+    dim3 grid(iblocks + rblocks + sblocks);
     const size_t lds = win_bytes(t1 - t0 + 1);
-#define HVC_K(R, I, L) k_integrate<R, I, L><<<grid, bs, lds, st>>>(nd, t0, t1, iblocks)
+#define HVC_K(R, I, L) k_integrate<R, I, L><<<grid, bs, lds, st>>>(nd, t0, t1, iblocks, rblocks, sa)
     if (nd.ra_model == 0) {
         if (nd.i_model == 1) HVC_K(0, 1, 0);
         else if (i_layout == 2) HVC_K(0, 0, 4);

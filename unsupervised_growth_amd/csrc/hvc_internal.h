@@ -36,6 +36,14 @@ struct NoiseEvent {
     double inc;
 };
 constexpr int kNoNoiseStep = 0x7fffffff;
+
+// the sync-point STDP of one window (hvc_stdp.hip.h)
+struct StdpArgs {
+    const uint64_t* posts;   // the window's post events (unsorted)
+    const int* count;
+    int t1;                  // the window's sync step
+};
+constexpr int kStdpSliceCap = 1024;        // post events per window of one block's networks
 constexpr int kNoiseChunkWindows = 64;   // windows per generated chunk
 
 // post-event key: network (12 bits), step (24 bits), neuron (28 bits); the
@@ -146,9 +154,10 @@ struct NetDev {
     double* post_ring;           // [n_ra][kPostRing]
     int* post_cnt;
     int* post_pend;
-    uint64_t* post_list;         // post_key(network, step, neuron), sorted; the window's buffer
+    uint64_t* post_list;         // post_key(network, step, neuron) in any order; the window's buffer
     int* post_list_cnt;          // its count
     int* post_list_next_cnt;     // count of the other buffer (the next window's), reset by k_deliver
+    double* last_pre;            // [n_ra] latest pre-event time of the row in this trial
 
     // RA->RA candidates
     const int* rr_ptr;

@@ -9,9 +9,11 @@
 //   deliver                    : per presynaptic neuron, delay-queue pops whose
 //                                arrival falls in the window become records in
 //                                the target's inbox; RA->RA rows also run the
-//                                order-dependent LTD / update_synapse scan and,
-//                                at the window's sync point, LTP and axon
-//                                remodeling (NGS:9375-9554).
+//                                order-dependent LTD / update_synapse scan.
+//   stdp                       : at the window's sync point, LTP / LTD against
+//                                the window's post events and axon remodeling
+//                                (NGS:9375-9554), in blocks of the next
+//                                window's k_integrate (hvc_stdp.hip.h).
 //   sync                       : per target, records are ordered by (arrival
 //                                step, presynaptic id) -- the reference's
 //                                accumulation order -- summed and applied
@@ -22,6 +24,7 @@
 
 #include "hvc_internal.h"
 #include "hvc_integrate.hip.h"
+#include "hvc_stdp.hip.h"
 
 namespace hvc {
 
@@ -64,6 +67,7 @@ __global__ void k_reset(NetDev nd)
         }
         nd.post_cnt[i] = 0;
         nd.post_pend[i] = 0;
+        nd.last_pre[i] = -1.0e300;
         nd.st_count[i] = 0;
         nd.fl_ra2i.head[i] = 0; nd.fl_ra2i.count[i] = 0;
         nd.fl_rr.head[i] = 0; nd.fl_rr.count[i] = 0;
@@ -225,37 +229,6 @@ __global__ void k_clear_pre(int* pre_cnt, int E)
 }
 
 // ---------------------------------------------------------------------------
-// the window's post-event list, appended in any order by k_integrate, sorted by
-// (step, id): the order in which the sync block meets new post events
-// (NGS:9375-9386).  One block, bitonic network in LDS.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_sort_posts(NetDev nd)
-{
-    __shared__ uint64_t keys[kPostListCap];
-    int n = *nd.post_list_cnt;
-    if (n > kPostListCap) n = kPostListCap;
-    if (n <= 1) return;
-    int np2 = 1;
-    while (np2 < n) np2 <<= 1;
-    for (int k = threadIdx.x; k < np2; k += blockDim.x) keys[k] = (k < n) ? nd.post_list[k] : ~0ull;
-    __syncthreads();
-    for (int size = 2; size <= np2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int k = threadIdx.x; k < np2; k += blockDim.x) {
-                int p = k ^ stride;
-                if (p > k) {
-                    bool up = ((k & size) == 0);
-                    uint64_t a = keys[k], b = keys[p];
-                    if ((a > b) == up) { keys[k] = b; keys[p] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = threadIdx.x; k < n; k += blockDim.x) nd.post_list[k] = keys[k];
-}
-
-// ---------------------------------------------------------------------------
 // delivery helpers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void inbox_put(const Inbox& ib, int tgt, int step, int pre, double w, unsigned int* err)
@@ -329,116 +302,6 @@ __device__ void push_fixed(const NetDev& nd, int pre, int row_begin, int row_end
     }
     fl.head[pre] = head;
     fl.count[pre] = count;
-}
-
-// ---------------------------------------------------------------------------
-// STDP and synapse-state machine (NGS:16546-16652), row-owned by one thread
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int edge_of(const NetDev& nd, int i, int j)
-{
-    int lo = nd.rr_ptr[i], hi = nd.rr_ptr[i + 1] - 1;
-    if (nd.dense) return lo + j;
-    while (lo <= hi) {
-        int mid = (lo + hi) >> 1;
-        int v = nd.rr_tgt[mid];
-        if (v == j) return mid;
-        if (v < j) lo = mid + 1; else hi = mid - 1;
-    }
-    return -1;
-}
-
-__device__ __forceinline__ double ltp(const hvc_synaptic_params& p, double w, double dt, int* warn)
-{
-    if (dt <= p.T_0) { atomicAdd(&warn[0], 1); return w; }
-    if (dt <= p.T_0 + p.T_P)
-        w = w + p.A_P * (dt - p.T_0) / p.T_P;
-    else
-        w = w + p.A_P * dexp(-(dt - p.T_0 - p.T_P) / p.TAU_P);
-    if (w > p.WEIGHT_MAX) w = p.WEIGHT_MAX;
-    if (w < 0) w = 0;
-    return w;
-}
-
-__device__ __forceinline__ double ltd(const hvc_synaptic_params& p, double w, double dt, int* warn)
-{
-    if (dt > p.T_0) { atomicAdd(&warn[1], 1); return w; }
-    if (dt >= p.T_0 - p.T_D)
-        w = w + w * p.A_D * (dt - p.T_0) / p.T_D;
-    else
-        w = w - w * p.A_D * dexp((dt - p.T_0 + p.T_D) / p.TAU_D);
-    if (w < 0) w = 0;
-    return w;
-}
-
-__device__ __forceinline__ bool list_erase(int* list, int* cnt, int j)
-{
-    int n = *cnt;
-    for (int k = 0; k < n; ++k) {
-        if (list[k] == j) {
-            for (int q = k; q < n - 1; ++q) list[q] = list[q + 1];
-            *cnt = n - 1;
-            return true;
-        }
-    }
-    return false;
-}
-
-// warning counters of the network that owns HVC-RA row i
-__device__ __forceinline__ int* row_warn(const NetDev& nd, int i) { return nd.warn + 3 * nd.ra_net[i]; }
-
-// first index of the sorted post-event keys that is >= key
-__device__ __forceinline__ int post_lower_bound(const uint64_t* keys, int n, uint64_t key)
-{
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (keys[mid] < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-// NGS:16546-16592
-__device__ void update_synapse(const NetDev& nd, int i, int e)
-{
-    const hvc_synaptic_params& p = nd.syn;
-    int j = nd.rr_tgt[e];
-    double w = nd.w[e];
-    int* alist = nd.act_list + nd.rr_ptr[i];
-    int* slist = nd.sup_list + (size_t)i * p.Nss;
-    if ((w >= p.ACTIVATION_THRESHOLD) && (nd.act[e] == 0) && (nd.remodeled[i] == 0)) {
-        nd.act[e] = 1;
-        alist[nd.act_cnt[i]++] = j;
-    }
-    if ((w >= p.SUPERSYNAPSE_THRESHOLD) && (nd.sup[e] == 0) && (nd.sup_cnt[i] < p.Nss)) {
-        nd.sup[e] = 1;
-        slist[nd.sup_cnt[i]++] = j;
-    }
-    if ((w < p.SUPERSYNAPSE_THRESHOLD) && (nd.sup[e] == 1)) {
-        nd.sup[e] = 0;
-        nd.remodeled[i] = 0;
-        if (!list_erase(slist, &nd.sup_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
-    }
-    if ((w < p.ACTIVATION_THRESHOLD) && (nd.act[e] == 1)) {
-        nd.act[e] = 0;
-        if (!list_erase(alist, &nd.act_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
-    }
-}
-
-__device__ void update_row(const NetDev& nd, int i)
-{
-    for (int e = nd.rr_ptr[i]; e < nd.rr_ptr[i + 1]; ++e) update_synapse(nd, i, e);
-}
-
-// NGS:16363-16415
-__device__ void axon_remodeling(const NetDev& nd, int i)
-{
-    int base = nd.rr_ptr[i];
-    int* slist = nd.sup_list + (size_t)i * nd.syn.Nss;
-    for (int k = 0; k < nd.act_cnt[i]; ++k) nd.act[edge_of(nd, i, nd.act_list[base + k])] = 0;
-    for (int k = 0; k < nd.sup_cnt[i]; ++k) nd.act[edge_of(nd, i, slist[k])] = 1;
-    for (int k = 0; k < nd.sup_cnt[i]; ++k) nd.act_list[base + k] = slist[k];
-    nd.act_cnt[i] = nd.sup_cnt[i];
-    nd.remodeled[i] = 1;
 }
 
 // LTD against the target's post events not older than STDP_WINDOW (NGS:9111-9180)
@@ -567,6 +430,7 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
             if (pc == 0 || bT > ring[(pc - 1) % kPreRing] + 30.0) {
                 ring[pc % kPreRing] = bT;
                 nd.pre_cnt[be] = pc + 1;
+                if (bT > nd.last_pre[i]) nd.last_pre[i] = bT;
                 new_pre = true;
             }
             if (new_pre) {
@@ -599,47 +463,6 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
         nd.st_count[i] = sc;
     }
 
-    // sync at step t1: LTP / LTD against this window's new post events of the
-    // row's own network (NGS:9375-9542)
-    int np = *nd.post_list_cnt;
-    if (np > kPostListCap) np = kPostListCap;
-    if (np == 0) return;
-    const int net = nd.ra_net[i];
-    const int k_lo = post_lower_bound(nd.post_list, np, post_key(net, 0, 0));
-    const int k_hi = post_lower_bound(nd.post_list, np, post_key(net + 1, 0, 0));
-    if (k_lo == k_hi) return;
-    const double now = nd.clock[t1];
-    const bool saturated = (nd.sup_cnt[i] == Nss);
-    const int* slist = nd.sup_list + (size_t)i * Nss;
-    for (int k = k_lo; k < k_hi; ++k) {
-        int post = post_key_id(nd.post_list[k]);
-        if (post == i) continue;
-        if (saturated) {
-            bool found = false;
-            for (int q = 0; q < nd.sup_cnt[i]; ++q)
-                if (slist[q] == post) { found = true; break; }
-            if (!found) continue;
-        }
-        int e = edge_of(nd, i, post);
-        if (e < 0) continue;
-        int pc = nd.pre_cnt[e];
-        if (pc == 0) continue;
-        const double* ring = nd.pre_ring + (size_t)e * kPreRing;
-        int first = pc > kPreRing ? pc - kPreRing : 0;
-        double lim = now - 100.0;
-        int q = first;
-        while (q < pc && ring[q % kPreRing] < lim) ++q;
-        for (; q < pc; ++q) {
-            double dt = now - ring[q % kPreRing];
-            if (dt <= nd.syn.T_0)
-                nd.w[e] = ltd(nd.syn, nd.w[e], dt, row_warn(nd, i));
-            else
-                nd.w[e] = ltp(nd.syn, nd.w[e], dt, row_warn(nd, i));
-            update_synapse(nd, i, e);
-        }
-    }
-    // NGS:9548-9554
-    if ((nd.sup_cnt[i] == Nss) && (nd.remodeled[i] == 0)) axon_remodeling(nd, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -803,9 +626,18 @@ void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int g
     k_noise_events<<<bra + bi, 64, 0, st>>>(nd, tb, te, bra, gen_ra, gen_i);
 }
 
-void launch_sort_posts(hipStream_t st, const NetDev& nd)
+// the last window's sync-point STDP (earlier windows run it inside the next
+// window's k_integrate)
+__global__ __launch_bounds__(256) void k_stdp(NetDev nd, StdpArgs sa)
 {
-    k_sort_posts<<<1, 1024, 0, st>>>(nd);
+    __shared__ uint64_t slice[kStdpSliceCap];
+    __shared__ int scount;
+    stdp_block(nd, sa, blockIdx.x * blockDim.x, slice, &scount);
+}
+
+void launch_stdp(hipStream_t st, const NetDev& nd, const StdpArgs& sa)
+{
+    k_stdp<<<blocks_for(nd.n_ra, 256), 256, 0, st>>>(nd, sa);
 }
 
 void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu)
