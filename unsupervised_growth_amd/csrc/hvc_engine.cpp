@@ -229,7 +229,8 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
         internal_time += dt;
         ctx->clock[t] = internal_time;
         if ((internal_time > network_time) || (t == ctx->num_steps - 1)) {
-            if (t - t0 + 1 > 64) return fail(ctx, HVC_E_ARG, "more than 64 steps between synchronisation points");
+            if (t - t0 + 1 > kMaxWindowSteps)
+                return fail(ctx, HVC_E_ARG, "more than 64 steps between synchronisation points");
             ctx->windows.emplace_back(t0, t);
             t0 = t + 1;
             network_time += prm->sync_ms;
@@ -604,6 +605,7 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.post_list_cnt = dalloc<int>(ctx, 2, &st);
     nd.post_list_next_cnt = nd.post_list_cnt ? nd.post_list_cnt + 1 : nullptr;
     nd.last_pre = dalloc<double>(ctx, nra, &st);
+    nd.st_next = dalloc<double>(ctx, nra, &st);
     ctx->d_rr_ptr = dalloc<int>(ctx, nra + 1, &st);
     ctx->d_rr_tgt = dalloc<int>(ctx, E, &st);
     ctx->d_rr_d = dalloc<double>(ctx, E, &st);
@@ -1327,7 +1329,7 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     const double inflight = 2.0 * kMaxInflight * sizeof(int) + 2 * sizeof(int);
     PoolCost c;
     const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
-    c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 8 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
+    c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 16 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
            4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise;
     c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise;
     c.rr = 4 + 8 + 4 + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;

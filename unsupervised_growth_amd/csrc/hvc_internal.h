@@ -11,6 +11,7 @@ constexpr int kPostRing = 8;     // post-event times kept per HVC-RA neuron
 constexpr int kPreRing = 4;      // delivered pre-event times kept per candidate edge
 constexpr int kMaxStreams = 16;  // in-flight RA->RA spikes per presynaptic row
 constexpr int kMaxInflight = 16; // in-flight spikes per neuron for fixed-weight pushes
+constexpr int kMaxWindowSteps = 64;  // steps between two synchronisation points
 constexpr int kPostListCap = 8192;  // new post events per window, all networks together
 constexpr int kMaxNets = 4096;      // networks per context (12 bits of a post-event key)
 
@@ -158,6 +159,7 @@ struct NetDev {
     int* post_list_cnt;          // its count
     int* post_list_next_cnt;     // count of the other buffer (the next window's), reset by k_deliver
     double* last_pre;            // [n_ra] latest pre-event time of the row in this trial
+    double* st_next;             // [n_ra] earliest delivery time of the row's in-flight spikes
 
     // RA->RA candidates
     const int* rr_ptr;

@@ -244,20 +244,22 @@ __device__ __forceinline__ void inbox_put(const Inbox& ib, int tgt, int step, in
     }
 }
 
-// first step a >= lo with clock[a] >= T (the drain test internal_time >= delivery_time)
-__device__ __forceinline__ int arrival_step(const double* clock, int lo, double T)
+// The window's clock in LDS: wclk[t - t0 + 1] = clock[t] for t0 - 1 <= t <= t1.
+// First step a >= lo (lo >= t0) with clock[a] >= T (the drain test
+// internal_time >= delivery_time); T <= clock[t1], so a <= t1.
+__device__ __forceinline__ int arrival_step(const double* wclk, int t0, int lo, double T)
 {
     int a = lo;
-    while (clock[a] < T) ++a;
+    while (wclk[a - t0 + 1] < T) ++a;
     return a;
 }
 
 // Fixed-weight delay queue of one presynaptic neuron (RA->I NGS:8971-8986 +
 // 9039-9062, I->RA NGS:9235-9284, chain-test RA->RA NGS:13222-13250): each
 // in-flight spike walks its delay-sorted edges; pops arriving by t1 go to inboxes.
-__device__ void push_fixed(const NetDev& nd, int pre, int row_begin, int row_end, const int* edge_ids,
-                           const int* tgt_arr, const double* w_arr, const double* d_arr, const InFlight& fl,
-                           uint64_t new_mask, int t0, int t1, const Inbox& ib)
+__device__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int row_begin, int row_end,
+                           const int* edge_ids, const int* tgt_arr, const double* w_arr, const double* d_arr,
+                           const InFlight& fl, uint64_t new_mask, int t0, int t1, const Inbox& ib)
 {
     int head = fl.head[pre], count = fl.count[pre];
     int* fstep = fl.step + (size_t)pre * kMaxInflight;
@@ -277,28 +279,39 @@ __device__ void push_fixed(const NetDev& nd, int pre, int row_begin, int row_end
         fcur[slot] = 0;
         ++count;
     }
-    const double tend = nd.clock[t1];
+    const double tend = wclk[t1 - t0 + 1];
+    int head_cur = deg;    // cursor of the head spike after this window
     for (int q = 0; q < count; ++q) {
         int slot = (head + q) % kMaxInflight;
         int s = fstep[slot];
         int c = fcur[slot];
         const double cs = nd.clock[s];
         int lo = s > t0 ? s : t0;
+        // the next edge's delay is loaded while this edge's record is written
+        int e = c < deg ? (edge_ids ? edge_ids[row_begin + c] : row_begin + c) : 0;
+        double d = c < deg ? d_arr[e] : 0.0;
         while (c < deg) {
-            int e = edge_ids ? edge_ids[row_begin + c] : row_begin + c;
-            double T = cs + d_arr[e];
+            double T = cs + d;
             if (T > tend) break;
-            int a = arrival_step(nd.clock, lo, T);
+            const int tg = tgt_arr[e];
+            const double w = w_arr[e];
+            const int en = c + 1 < deg ? (edge_ids ? edge_ids[row_begin + c + 1] : row_begin + c + 1) : e;
+            const double dn = d_arr[en];
+            int a = arrival_step(wclk, t0, lo, T);
             lo = a;
-            inbox_put(ib, tgt_arr[e], a, pre, w_arr[e], nd.err);
+            inbox_put(ib, tg, a, pre, w, nd.err);
             ++c;
+            e = en;
+            d = dn;
         }
         fcur[slot] = c;
+        if (q == 0) head_cur = c;
     }
     // retire finished spikes from the head (all share the same edges, so FIFO)
-    while (count > 0 && fcur[head] >= deg) {
+    while (count > 0 && (head_cur >= deg)) {
         head = (head + 1) % kMaxInflight;
         --count;
+        head_cur = count > 0 ? fcur[head] : deg;
     }
     fl.head[pre] = head;
     fl.count[pre] = count;
@@ -355,17 +368,24 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
 
 // Growth-mode RA->RA row of presynaptic neuron i for the window t0..t1
 // (NGS:8991-9030 enqueue, 9069-9191 drain with LTD, 9375-9554 at the sync).
-__device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
+__device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int t1)
 {
     const int Nss = nd.syn.Nss;
     uint64_t mask = nd.soma_mask[i];
     int sc = nd.st_count[i];
+    // earliest delivery time among the in-flight spikes (their stream heads):
+    // at a step before it, with no spike to enqueue, the drain and the
+    // exhausted-stream pass change nothing, so the step is skipped
+    double next_T = sc > 0 ? nd.st_next[i] : HUGE_VAL;
     int* st_step = nd.st_step + (size_t)i * kMaxStreams;
     int* st_mode = nd.st_mode + (size_t)i * kMaxStreams;
     int* st_cur = nd.st_cursor + (size_t)i * kMaxStreams;
     int* st_n = nd.st_n + (size_t)i * kMaxStreams;
     for (int t = t0; t <= t1; ++t) {
-        if ((mask >> (t - t0)) & 1ull) {
+        const bool spike = (mask >> (t - t0)) & 1ull;
+        const double now = wclk[t - t0 + 1];
+        if (!spike && !(now >= next_T)) continue;
+        if (spike) {
             if (sc == kMaxStreams) {
                 atomicOr(nd.err, (unsigned)kErrStreams);
             } else {
@@ -404,7 +424,6 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
             }
         }
         nd.st_count[i] = sc;
-        const double now = nd.clock[t];
         // drain in (delivery_time, target) order across all in-flight spikes
         for (;;) {
             int best = -1;
@@ -444,10 +463,12 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
         }
         // drop exhausted streams, keeping order
         int w = 0;
+        next_T = HUGE_VAL;
         for (int q = 0; q < sc; ++q) {
             double T;
             int tg, e;
             if (stream_peek(nd, i, q, T, tg, e)) {
+                if (T < next_T) next_T = T;
                 if (w != q) {
                     st_step[w] = st_step[q]; st_mode[w] = st_mode[q]; st_cur[w] = st_cur[q]; st_n[w] = st_n[q];
                     if (st_mode[q] == 1) {
@@ -462,7 +483,7 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
         sc = w;
         nd.st_count[i] = sc;
     }
-
+    nd.st_next[i] = next_T;
 }
 
 // ---------------------------------------------------------------------------
@@ -473,25 +494,28 @@ __device__ void rr_row(const NetDev& nd, int i, int t0, int t1)
 __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int blocks_ra)
 {
     int b = blockIdx.x;
-    // the other post-event buffer was read by the previous window's delivery;
-    // the next window's integration fills it
+    __shared__ double wclk[kMaxWindowSteps + 1];
+    for (int k = threadIdx.x; k <= t1 - t0 + 1; k += blockDim.x) wclk[k] = nd.clock[t0 - 1 + k];
+    __syncthreads();
+    // the other post-event buffer was read by the previous window's STDP (in
+    // this window's integration); the next window's integration fills it
     if (b == 0 && threadIdx.x == 0) *nd.post_list_next_cnt = 0;
     int role = b / blocks_ra;
     int local = (b % blocks_ra) * blockDim.x + threadIdx.x;
     if (role == 0) {
         if (local >= nd.n_ra) return;
         if (nd.net_growth[nd.ra_net[local]])
-            rr_row(nd, local, t0, t1);
+            rr_row(nd, wclk, local, t0, t1);
         else
-            push_fixed(nd, local, nd.rr_ptr[local], nd.rr_ptr[local] + nd.act_cnt[local], nd.chain_e, nd.rr_tgt,
+            push_fixed(nd, wclk, local, nd.rr_ptr[local], nd.rr_ptr[local] + nd.act_cnt[local], nd.chain_e, nd.rr_tgt,
                        nd.w, nd.rr_d, nd.fl_rr, nd.soma_mask[local], t0, t1, nd.in_ge_ra);
     } else if (role == 1) {
         if (local >= nd.n_ra) return;
-        push_fixed(nd, local, nd.ra2i.ptr[local], nd.ra2i.ptr[local + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
+        push_fixed(nd, wclk, local, nd.ra2i.ptr[local], nd.ra2i.ptr[local + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
                    nd.ra2i.d, nd.fl_ra2i, nd.soma_mask[local], t0, t1, nd.in_ge_i);
     } else {
         if (local >= nd.n_i) return;
-        push_fixed(nd, local, nd.i2ra.ptr[local], nd.i2ra.ptr[local + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
+        push_fixed(nd, wclk, local, nd.i2ra.ptr[local], nd.i2ra.ptr[local + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
                    nd.i2ra.d, nd.fl_i2ra, nd.i_mask[local], t0, t1, nd.in_gi_ra);
     }
 }
