@@ -757,24 +757,55 @@ __constant__ GateSlot kIQuadGates[4][2] = {
     {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0)}, {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1)}},         // h: ah, bh
 };
 
-// derivatives of the lane's variable (kx) and, in lane 0, of w (kw; 0 in the
-// other lanes) at stage state (v, n1, m1, h1, w1); w1 is valid in lane 0 only
+// FP32 form of a gate slot: with per-lane constants the three rate forms are
+// one expression, E = exp(-(v + d) / e), rate = cB E + (cA (v + d) + cC) / (1 + sg E)
+// (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects
+struct GateK {
+    real cA, cB, cC, sg, d, inv_e;
+};
+__device__ __forceinline__ GateK gate_k(const GateSlot& g)
+{
+    return GateK{g.type == 0 ? g.c : HR(0.0), g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : HR(0.0),
+                 g.type == 0 ? -HR(1.0) : HR(1.0), g.d, g.inv_e};
+}
+#ifdef HVC_FP32
+__device__ __forceinline__ real gate_rate_k(const GateK& g, real v)
+{
+    HVC_FP_FAST
+    const real vd = v + g.d;
+    const real E = EXPR(-vd * g.inv_e);
+    const real q = __builtin_fmaf(g.cA, vd, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
+    return __builtin_fmaf(g.cB, E, q);
+}
+#endif
+
+// derivatives of the lane's variable (kx) and, in lane 0, of w (kw) at stage
+// state (v, n1, m1, h1, w1); xo = the lane's own variable at the stage; w1 is
+// valid in lane 0 only (kw of the other lanes is not used)
 template <bool EXACT>
 __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role, const GateSlot& ga,
-                                           const GateSlot& gb, real v, real n1, real m1, real h1, real w1, real e2,
-                                           real e5, real& kx, real& kw)
+                                           const GateSlot& gb, real v, real n1, real m1, real h1, real w1, real xo,
+                                           real e2, real e5, real& kx, real& kw)
 {
     HVC_FP_FAST
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
+#ifdef HVC_FP32
+    // (the reference's closing "+ 100000 * 0 / A" only turns -0 into +0)
+    real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
+                - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei);
+    real a = gate_rate_k(gate_k(ga), v);
+    real b = gate_rate_k(gate_k(gb), v);
+    kw = a - w1;   // a = wInf in lane 0
+#else
     real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
                 - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
     real a = gate_rate<EXACT>(bad, ga, v);
     real b = gate_rate<EXACT>(bad, gb, v);
-    real x = role == 1 ? n1 : (role == 2 ? m1 : h1);
-    real kg = a * (1 - x) - b * x;
-    kx = role == 0 ? kV : kg;
     kw = role == 0 ? (a - w1) : HR(0.0);   // a = wInf in lane 0
+#endif
+    real kg = a * (1 - xo) - b * xo;
+    kx = role == 0 ? kV : kg;
 }
 
 #define HVC_BCAST4(x)              \
@@ -794,14 +825,14 @@ __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, rea
     const real y0 = s.w;
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
     real x;
-#define HVC_QSTAGE(N, e2, e5, SX)                                                       \
-    real k_##N, q_##N;                                                                \
-    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, e2, e5, k_##N, q_##N); \
-    x = x0 + SX(k_);                                                                  \
-    w1 = y0 + SX(q_);                                                                 \
+#define HVC_QSTAGE(N, e2, e5, SX)                                                          \
+    real k_##N, q_##N;                                                                   \
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, x, e2, e5, k_##N, q_##N); \
+    x = x0 + SX(k_);                                                                     \
+    w1 = y0 + SX(q_);                                                                    \
     HVC_BCAST4(x)
     real k_1, q_1;
-    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, HR(1.0), HR(1.0), k_1, q_1);
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, x0, HR(1.0), HR(1.0), k_1, q_1);
     x = x0 + HVC_DP8_S2(k_);
     w1 = y0 + HVC_DP8_S2(q_);
     HVC_BCAST4(x)
@@ -817,7 +848,7 @@ __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, rea
     HVC_QSTAGE(11, row.e2[kOffDp11], row.e5[kOffDp11], HVC_DP8_S12)
 #undef HVC_QSTAGE
     real k_12, q_12;
-    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, row.e2[kOffDt], row.e5[kOffDt], k_12, q_12);
+    i_quad_rhs<EXACT>(bad, s, role, ga, gb, v, n1, m1, h1, w1, x, row.e2[kOffDt], row.e5[kOffDt], k_12, q_12);
     {
         struct { real x, x_lo; } acc{x0, xlo};
         HVC_ACC(acc, x, HVC_DP8_FIN(k_))
