@@ -44,25 +44,16 @@ struct Guard {
 #define CDV(a, B, invB) ((a) * (invB))
 #define CK(n, k, d) ((k) * (real)((double)(n) / (double)(d)))
 #define DIVR(a, b) ((real)(a) * __builtin_amdgcn_rcpf((real)(b)))
-#define EXPR(x) ::hvc::p32::fexp(x)
+// exp(a / B) as 2^(a * RN(log2(e) / B)): the argument a / B is already only
+// known to |a / B| * 2^-24 in this form, and folding log2(e) into the constant
+// keeps the error of that order with one multiplication and the hardware exp2
+#define EXPCD(a, B) __builtin_amdgcn_exp2f((a) * (real)(1.4426950408889634 / (double)(B)))
+#define EXPCDV(a, B, invB, inv2B) __builtin_amdgcn_exp2f((a) * (inv2B))
 // the neuron arithmetic of this form may fuse a * b + c into one rounding
 #define HVC_FP_FAST _Pragma("clang fp contract(fast)")
 // keeps a value computed in every lane (the form's quotients are one
 // reciprocal: cheaper than a divergent branch around them)
 #define HVC_KEEP(x) asm volatile("" : "+v"(x))
-// exp(x) = 2^(x log2 e): the product x log2 e is carried as p + e (fma gives
-// the rounding error of x L_hi exactly, x L_lo adds the tail of log2 e), the
-// hardware exp2 evaluates 2^p and 2^e ~ 1 + e ln 2 corrects it; about 2 ulp.
-__device__ __forceinline__ float fexp(float x)
-{
-    const float L_hi = 1.44269502162933349609375f;      // RN(log2 e)
-    const float L_lo = 1.925963033500011079e-8f;        // log2 e - L_hi
-    float p = x * L_hi;
-    float e = __builtin_fmaf(x, L_hi, -p);
-    e = __builtin_fmaf(x, L_lo, e);
-    float r = __builtin_amdgcn_exp2f(p);
-    return __builtin_fmaf(r, e * 0.693147180559945309f, r);
-}
 // state update with compensated (Kahan) summation: st.f + st.f_lo carries the
 // value, so increments far below half an ulp of st.f still accumulate
 #define HVC_ACC(st, f, inc)                  \
@@ -88,6 +79,8 @@ struct Guard : DivGuard {
 #define CK(n, k, d) CD((n) * (k), d)
 #define DIVR(a, b) ((a) / (b))
 #define EXPR(x) ::hvc::dexp(x)
+#define EXPCD(a, B) EXPR(CD(a, B))
+#define EXPCDV(a, B, invB, inv2B) EXPR(CDV(a, B, invB))
 #define HVC_FP_FAST
 #define HVC_KEEP(x)
 #define HVC_ACC(st, f, inc) st.f = st.f + inc;
@@ -133,21 +126,21 @@ constexpr real kRA_GsL = HR(0.1), kRA_GdL = HR(0.1), kRA_EdCa = HR(120.0), kRA_E
 constexpr real kRA_EsNa = HR(55.0), kRA_EsK = -HR(90.0), kRA_Egaba = -HR(80.0);
 
 template <bool EXACT>
-__device__ __forceinline__ real nInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 35), 10))); }
+__device__ __forceinline__ real nInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 35), 10)); }
 template <bool EXACT>
-__device__ __forceinline__ real tauN(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.5), 1 + EXPR(CD(v + 27, 15))); }
+__device__ __forceinline__ real tauN(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.5), 1 + EXPCD(v + 27, 15)); }
 template <bool EXACT>
-__device__ __forceinline__ real hInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(v + 45, 7))); }
+__device__ __forceinline__ real hInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(v + 45, 7)); }
 template <bool EXACT>
-__device__ __forceinline__ real tauH(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.75), 1 + EXPR(CD(v + HR(40.5), 6))); }
+__device__ __forceinline__ real tauH(real v, Guard& bad) { HVC_FP_FAST return HR(0.1) + DIVR(HR(0.75), 1 + EXPCD(v + HR(40.5), 6)); }
 template <bool EXACT>
-__device__ __forceinline__ real mInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 30), HR(9.5)))); }
+__device__ __forceinline__ real mInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 30), HR(9.5))); }
 template <bool EXACT>
-__device__ __forceinline__ real lInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 40), 5))); }
+__device__ __forceinline__ real lInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 40), 5)); }
 template <bool EXACT>
-__device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v + 5), 10))); }
+__device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 5), 10)); }
 template <bool EXACT>
-__device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-(v - 10), 7))); }
+__device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v - 10), 7)); }
 
 struct RAK {
     real Vs, Vd, n, l, h, r, c, Ca;
@@ -220,16 +213,18 @@ __device__ __forceinline__ RAK ra_rhs(Guard& bad, const RAState& s, real EL, rea
 struct RAGateSlot {
     real a, c, d, e, inv_e;   // inv_e = the form's rounding of 1 / e (as CD uses it)
     int neg, dend;            // argument -(v + d) / (v + d); v = Vd / Vs
+    real inv2_e;              // RN(log2(e) / e) (FP32 form, EXPCDV)
 };
 
 __constant__ RAGateSlot kRAGates[8] = {
-    {HR(0.0), HR(1.0), HR(30.0), HR(9.5), (real)(1.0 / 9.5), 1, 0},     // mInf
-    {HR(0.0), HR(1.0), HR(35.0), HR(10.0), (real)(1.0 / 10.0), 1, 0},   // nInf
-    {HR(0.1), HR(0.5), HR(27.0), HR(15.0), (real)(1.0 / 15.0), 0, 0},   // tauN
-    {HR(0.0), HR(1.0), HR(45.0), HR(7.0), (real)(1.0 / 7.0), 0, 0},     // hInf
-    {HR(0.1), HR(0.75), HR(40.5), HR(6.0), (real)(1.0 / 6.0), 0, 0},    // tauH
-    {HR(0.0), HR(1.0), HR(5.0), HR(10.0), (real)(1.0 / 10.0), 1, 1},    // rInf
-    {HR(0.0), HR(1.0), -HR(10.0), HR(7.0), (real)(1.0 / 7.0), 1, 1},    // cInf
+#define HVC_L2(e) (real)(1.4426950408889634 / (e))
+    {HR(0.0), HR(1.0), HR(30.0), HR(9.5), (real)(1.0 / 9.5), 1, 0, HVC_L2(9.5)},     // mInf
+    {HR(0.0), HR(1.0), HR(35.0), HR(10.0), (real)(1.0 / 10.0), 1, 0, HVC_L2(10.0)},  // nInf
+    {HR(0.1), HR(0.5), HR(27.0), HR(15.0), (real)(1.0 / 15.0), 0, 0, HVC_L2(15.0)},  // tauN
+    {HR(0.0), HR(1.0), HR(45.0), HR(7.0), (real)(1.0 / 7.0), 0, 0, HVC_L2(7.0)},     // hInf
+    {HR(0.1), HR(0.75), HR(40.5), HR(6.0), (real)(1.0 / 6.0), 0, 0, HVC_L2(6.0)},    // tauH
+    {HR(0.0), HR(1.0), HR(5.0), HR(10.0), (real)(1.0 / 10.0), 1, 1, HVC_L2(10.0)},   // rInf
+    {HR(0.0), HR(1.0), -HR(10.0), HR(7.0), (real)(1.0 / 7.0), 1, 1, HVC_L2(7.0)},    // cInf
     {HR(0.0), HR(1.0), HR(0.0), HR(10.0), (real)(1.0 / 10.0), 1, 0},    // unused
 };
 
@@ -238,7 +233,7 @@ __device__ __forceinline__ real ra_gate(Guard& bad, const RAGateSlot& g, real vs
 {
     HVC_FP_FAST
     real x = (g.dend ? vd : vs) + g.d;
-    real q = DIVR(g.c, 1 + EXPR(CDV(g.neg ? -x : x, g.e, g.inv_e)));
+    real q = DIVR(g.c, 1 + EXPCDV(g.neg ? -x : x, g.e, g.inv_e, g.inv2_e));
     return g.a + q;   // a = 0: 0 + q = q (q > 0)
 }
 
@@ -394,19 +389,19 @@ constexpr real kI_gL = HR(0.1), kI_El = -HR(65.0), kI_gNa = HR(100.0), kI_Ena = 
 constexpr real kI_Ek = -HR(80.0), kI_gKHT = HR(500.0), kI_Ei = -HR(75.0), kI_A = HR(6000.0);
 
 template <bool EXACT>
-__device__ __forceinline__ real i_an(real v, Guard& bad) { HVC_FP_FAST return DIVR(HR(0.15) * (v + 15), 1 - EXPR(CD(-(v + 15), 10))); }
+__device__ __forceinline__ real i_an(real v, Guard& bad) { HVC_FP_FAST return DIVR(HR(0.15) * (v + 15), 1 - EXPCD(-(v + 15), 10)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * EXPR(CD(-(v + 25), 80)); }
+__device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * EXPCD(-(v + 25), 80); }
 template <bool EXACT>
-__device__ __forceinline__ real i_am(real v, Guard& bad) { HVC_FP_FAST return DIVR(v + 22, 1 - EXPR(CD(-(v + 22), 10))); }
+__device__ __forceinline__ real i_am(real v, Guard& bad) { HVC_FP_FAST return DIVR(v + 22, 1 - EXPCD(-(v + 22), 10)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * EXPR(CD(-(v + 47), 18)); }
+__device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * EXPCD(-(v + 47), 18); }
 template <bool EXACT>
-__device__ __forceinline__ real i_ah(real v, Guard& bad) { HVC_FP_FAST return HR(0.7) * EXPR(CD(-(v + 34), 20)); }
+__device__ __forceinline__ real i_ah(real v, Guard& bad) { HVC_FP_FAST return HR(0.7) * EXPCD(-(v + 34), 20); }
 template <bool EXACT>
-__device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return DIVR(10, 1 + EXPR(CD(-(v + 4), 10))); }
+__device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return DIVR(10, 1 + EXPCD(-(v + 4), 10)); }
 template <bool EXACT>
-__device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPR(CD(-v, 5))); }
+__device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-v, 5)); }
 
 struct IK {
     real V, n, m, h, w;
@@ -606,6 +601,7 @@ __constant__ real kPointDistribution[30] = {
 struct GateSlot {
     int type;        // 0: A, 1: B, 2: C
     real c, d, e, inv_e;
+    real inv2_e;     // RN(log2(e) / e) (FP32 form, EXPCDV)
 };
 
 template <bool EXACT>
@@ -613,7 +609,7 @@ __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
 {
     HVC_FP_FAST
     real vd = v + g.d;
-    real E = EXPR(CDV(-vd, g.e, g.inv_e));
+    real E = EXPCDV(-vd, g.e, g.inv_e, g.inv2_e);
     real num = (g.type == 0) ? g.c * vd : g.c;
     real den = (g.type == 0) ? (1 - E) : (1 + E);
     real q = DIVR(num, den);
@@ -622,11 +618,14 @@ __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
 }
 
 __constant__ GateSlot kIGates[5][2] = {
-    {{1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                    // V: no rates (unused)
-    {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1)}, {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0)}},          // n: an, bn
-    {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1)}, {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0)}},          // m: am, bm
-    {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0)}, {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1)}},           // h: ah, bh
-    {{2, HR(1.0), HR(0.0), HR(5.0), HR(0.2)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                     // w: wInf (unused)
+    {{1, HR(0.0), HR(0.0), HR(10.0), HR(0.1), HVC_L2(10.0)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1), HVC_L2(10.0)}},   // V: no rates (unused)
+    {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1), HVC_L2(10.0)},
+     {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0), HVC_L2(80.0)}},                                                  // n: an, bn
+    {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1), HVC_L2(10.0)},
+     {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0), HVC_L2(18.0)}},                                                 // m: am, bm
+    {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0), HVC_L2(20.0)},
+     {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1), HVC_L2(10.0)}},                                                            // h: ah, bh
+    {{2, HR(1.0), HR(0.0), HR(5.0), HR(0.2), HVC_L2(5.0)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1), HVC_L2(10.0)}},     // w: wInf (unused)
 };
 
 // derivative of the lane's variable at stage state (v, n1, m1, h1, w1)
@@ -751,29 +750,32 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, in
 // value is the expression tree of HHI_buffer, as in i_dp8 and the five-lane
 // form.
 __constant__ GateSlot kIQuadGates[4][2] = {
-    {{2, HR(1.0), HR(0.0), HR(5.0), HR(0.2)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1)}},                   // V, w: wInf (second slot unused)
-    {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1)}, {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0)}},        // n: an, bn
-    {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1)}, {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0)}},        // m: am, bm
-    {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0)}, {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1)}},         // h: ah, bh
+    {{2, HR(1.0), HR(0.0), HR(5.0), HR(0.2), HVC_L2(5.0)}, {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1), HVC_L2(10.0)}},   // V, w: wInf (second slot unused)
+    {{0, HR(0.15), HR(15.0), HR(10.0), HR(0.1), HVC_L2(10.0)},
+     {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0), HVC_L2(80.0)}},                                                // n: an, bn
+    {{0, HR(1.0), HR(22.0), HR(10.0), HR(0.1), HVC_L2(10.0)},
+     {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0), HVC_L2(18.0)}},                                               // m: am, bm
+    {{1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0), HVC_L2(20.0)},
+     {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1), HVC_L2(10.0)}},                                                          // h: ah, bh
 };
 
 // FP32 form of a gate slot: with per-lane constants the three rate forms are
 // one expression, E = exp(-(v + d) / e), rate = cB E + (cA (v + d) + cC) / (1 + sg E)
 // (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects
 struct GateK {
-    real cA, cB, cC, sg, d, inv_e;
+    real cA, cB, cC, sg, d, inv2_e;
 };
 __device__ __forceinline__ GateK gate_k(const GateSlot& g)
 {
     return GateK{g.type == 0 ? g.c : HR(0.0), g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : HR(0.0),
-                 g.type == 0 ? -HR(1.0) : HR(1.0), g.d, g.inv_e};
+                 g.type == 0 ? -HR(1.0) : HR(1.0), g.d, g.inv2_e};
 }
 #ifdef HVC_FP32
 __device__ __forceinline__ real gate_rate_k(const GateK& g, real v)
 {
     HVC_FP_FAST
     const real vd = v + g.d;
-    const real E = EXPR(-vd * g.inv_e);
+    const real E = EXPCDV(-vd, 0, 0, g.inv2_e);
     const real q = __builtin_fmaf(g.cA, vd, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
     return __builtin_fmaf(g.cB, E, q);
 }
