@@ -92,10 +92,9 @@ struct Inbox {
 };
 
 struct InFlight {
-    int* step;      // [n][kMaxInflight]
+    int* step;      // [n][kMaxInflight] spike step of a held slot, -1 free
     int* cursor;    // [n][kMaxInflight]
-    int* head;      // [n]
-    int* count;     // [n]
+    int* count;     // [n] held slots (0: the slot table is not valid)
 };
 
 struct NetDev {

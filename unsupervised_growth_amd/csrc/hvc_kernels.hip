@@ -69,8 +69,8 @@ __global__ void k_reset(NetDev nd)
         nd.post_pend[i] = 0;
         nd.last_pre[i] = -1.0e300;
         nd.st_count[i] = 0;
-        nd.fl_ra2i.head[i] = 0; nd.fl_ra2i.count[i] = 0;
-        nd.fl_rr.head[i] = 0; nd.fl_rr.count[i] = 0;
+        nd.fl_ra2i.count[i] = 0;
+        nd.fl_rr.count[i] = 0;
         nd.in_gi_ra.cnt[i] = 0;
         nd.in_ge_ra.cnt[i] = 0;
     }
@@ -86,7 +86,7 @@ __global__ void k_reset(NetDev nd)
             nd.i_noise[(size_t)nd.n_i + i] = i_init_noise(ps, init_dt);
         }
         nd.i_draw[i] = ps.draw;
-        nd.fl_i2ra.head[i] = 0; nd.fl_i2ra.count[i] = 0;
+        nd.fl_i2ra.count[i] = 0;
         nd.in_ge_i.cnt[i] = 0;
     }
     if (i == 0) {
@@ -254,67 +254,104 @@ __device__ __forceinline__ int arrival_step(const double* wclk, int t0, int lo, 
     return a;
 }
 
-// Fixed-weight delay queue of one presynaptic neuron (RA->I NGS:8971-8986 +
-// 9039-9062, I->RA NGS:9235-9284, chain-test RA->RA NGS:13222-13250): each
-// in-flight spike walks its delay-sorted edges; pops arriving by t1 go to inboxes.
-__device__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int row_begin, int row_end,
+// value of lane K of the quad, in every lane of the quad (DPP quad_perm)
+template <int K>
+__device__ __forceinline__ unsigned quad_lane(unsigned x)
+{
+    return (unsigned)__builtin_amdgcn_mov_dpp((int)x, K * 0x55, 0xF, 0xF, true);
+}
+template <int G>
+__device__ __forceinline__ unsigned group_or(unsigned x)
+{
+    if constexpr (G == 1) return x;
+    else return quad_lane<0>(x) | quad_lane<1>(x) | quad_lane<2>(x) | quad_lane<3>(x);
+}
+template <int G>
+__device__ __forceinline__ unsigned group_sum(unsigned x)
+{
+    if constexpr (G == 1) return x;
+    else return quad_lane<0>(x) + quad_lane<1>(x) + quad_lane<2>(x) + quad_lane<3>(x);
+}
+
+// Fixed-weight delay queues of one presynaptic neuron (RA->I NGS:8971-8986 +
+// 9039-9062, I->RA NGS:9235-9284, chain-test RA->RA NGS:13222-13250), G lanes
+// per neuron (G = 1 or a quad).  Each in-flight spike holds one of the
+// neuron's kMaxInflight slots (spike step, edge cursor; step -1 = free; the
+// table is valid while fl.count, the number of held slots, is non-zero) and
+// walks the delay-sorted edges; the pops arriving by t1 become inbox records.
+// A window's records are summed in the reference's order by the sync whatever
+// order they are written in, so lane r walks the spikes of slots r, r + G, ...
+// and new spikes take the free slots in slot order.
+template <int G>
+__device__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin, int row_end,
                            const int* edge_ids, const int* tgt_arr, const double* w_arr, const double* d_arr,
                            const InFlight& fl, uint64_t new_mask, int t0, int t1, const Inbox& ib)
 {
-    int head = fl.head[pre], count = fl.count[pre];
+    constexpr int K = kMaxInflight / G;
+    const int deg = row_end - row_begin;
+    const int held = fl.count[pre];
+    if (deg == 0 || (held == 0 && new_mask == 0)) return;
     int* fstep = fl.step + (size_t)pre * kMaxInflight;
     int* fcur = fl.cursor + (size_t)pre * kMaxInflight;
-    int deg = row_end - row_begin;
-    // enqueue this window's spikes (in step order)
-    while (new_mask) {
-        int b = __ffsll((unsigned long long)new_mask) - 1;
-        new_mask &= new_mask - 1;
-        if (deg == 0) continue;
-        if (count == kMaxInflight) {
-            atomicOr(nd.err, (unsigned)kErrInflight);
-            break;
-        }
-        int slot = (head + count) % kMaxInflight;
-        fstep[slot] = t0 + b;
-        fcur[slot] = 0;
-        ++count;
+    int st[K];
+    unsigned own_free = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        st[k] = held > 0 ? fstep[r + G * k] : -1;
+        if (st[k] < 0) own_free |= 1u << (r + G * k);
     }
+    const unsigned free_mask = group_or<G>(own_free);
+    const int nnew = __popcll(new_mask);
+    if (r == 0 && nnew > __popc(free_mask)) atomicOr(nd.err, (unsigned)kErrInflight);
     const double tend = wclk[t1 - t0 + 1];
-    int head_cur = deg;    // cursor of the head spike after this window
-    for (int q = 0; q < count; ++q) {
-        int slot = (head + q) % kMaxInflight;
-        int s = fstep[slot];
-        int c = fcur[slot];
+    unsigned now_held = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int slot = r + G * k;
+        int s = st[k], c = 0;
+        bool fresh = false;
+        if (s < 0) {
+            const int rank = __popc(free_mask & ((1u << slot) - 1u));
+            if (rank >= nnew) {
+                if (held == 0) fstep[slot] = -1;   // the table was not valid: mark the slot free
+                continue;
+            }
+            uint64_t m = new_mask;
+            for (int j = 0; j < rank; ++j) m &= m - 1;
+            s = t0 + __ffsll((unsigned long long)m) - 1;
+            fresh = true;
+        } else {
+            c = fcur[slot];
+        }
         const double cs = nd.clock[s];
         int lo = s > t0 ? s : t0;
         // the next edge's delay is loaded while this edge's record is written
         int e = c < deg ? (edge_ids ? edge_ids[row_begin + c] : row_begin + c) : 0;
         double d = c < deg ? d_arr[e] : 0.0;
         while (c < deg) {
-            double T = cs + d;
+            const double T = cs + d;
             if (T > tend) break;
             const int tg = tgt_arr[e];
             const double w = w_arr[e];
             const int en = c + 1 < deg ? (edge_ids ? edge_ids[row_begin + c + 1] : row_begin + c + 1) : e;
             const double dn = d_arr[en];
-            int a = arrival_step(wclk, t0, lo, T);
+            const int a = arrival_step(wclk, t0, lo, T);
             lo = a;
             inbox_put(ib, tg, a, pre, w, nd.err);
             ++c;
             e = en;
             d = dn;
         }
-        fcur[slot] = c;
-        if (q == 0) head_cur = c;
+        if (c >= deg) {
+            fstep[slot] = -1;
+        } else {
+            if (fresh) fstep[slot] = s;
+            fcur[slot] = c;
+            ++now_held;
+        }
     }
-    // retire finished spikes from the head (all share the same edges, so FIFO)
-    while (count > 0 && (head_cur >= deg)) {
-        head = (head + 1) % kMaxInflight;
-        --count;
-        head_cur = count > 0 ? fcur[head] : deg;
-    }
-    fl.head[pre] = head;
-    fl.count[pre] = count;
+    const unsigned total = group_sum<G>(now_held);
+    if (r == 0) fl.count[pre] = (int)total;
 }
 
 // LTD against the target's post events not older than STDP_WINDOW (NGS:9111-9180)
@@ -491,7 +528,7 @@ __device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int 
 //   role 0: RA->RA rows (growth) or chain-test RA->RA push
 //   role 1: RA->I push        role 2: I->RA push
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int blocks_ra)
+__global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int b0, int b1)
 {
     int b = blockIdx.x;
     __shared__ double wclk[kMaxWindowSteps + 1];
@@ -500,23 +537,29 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
     // the other post-event buffer was read by the previous window's STDP (in
     // this window's integration); the next window's integration fills it
     if (b == 0 && threadIdx.x == 0) *nd.post_list_next_cnt = 0;
-    int role = b / blocks_ra;
-    int local = (b % blocks_ra) * blockDim.x + threadIdx.x;
-    if (role == 0) {
-        if (local >= nd.n_ra) return;
-        if (nd.net_growth[nd.ra_net[local]])
-            rr_row(nd, wclk, local, t0, t1);
+    if (b < b0) {
+        // RA->RA: one lane per row
+        const int i = b * blockDim.x + threadIdx.x;
+        if (i >= nd.n_ra) return;
+        if (nd.net_growth[nd.ra_net[i]])
+            rr_row(nd, wclk, i, t0, t1);
         else
-            push_fixed(nd, wclk, local, nd.rr_ptr[local], nd.rr_ptr[local] + nd.act_cnt[local], nd.chain_e, nd.rr_tgt,
-                       nd.w, nd.rr_d, nd.fl_rr, nd.soma_mask[local], t0, t1, nd.in_ge_ra);
-    } else if (role == 1) {
-        if (local >= nd.n_ra) return;
-        push_fixed(nd, wclk, local, nd.ra2i.ptr[local], nd.ra2i.ptr[local + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
-                   nd.ra2i.d, nd.fl_ra2i, nd.soma_mask[local], t0, t1, nd.in_ge_i);
+            push_fixed<1>(nd, wclk, i, 0, nd.rr_ptr[i], nd.rr_ptr[i] + nd.act_cnt[i], nd.chain_e, nd.rr_tgt, nd.w,
+                          nd.rr_d, nd.fl_rr, nd.soma_mask[i], t0, t1, nd.in_ge_ra);
+        return;
+    }
+    // fixed-weight pushes: a quad per presynaptic neuron
+    const bool ra = b < b0 + b1;
+    const int q = (ra ? b - b0 : b - b0 - b1) * blockDim.x + threadIdx.x;
+    const int pre = q >> 2, r = q & 3;
+    if (ra) {
+        if (pre >= nd.n_ra) return;
+        push_fixed<4>(nd, wclk, pre, r, nd.ra2i.ptr[pre], nd.ra2i.ptr[pre + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
+                      nd.ra2i.d, nd.fl_ra2i, nd.soma_mask[pre], t0, t1, nd.in_ge_i);
     } else {
-        if (local >= nd.n_i) return;
-        push_fixed(nd, wclk, local, nd.i2ra.ptr[local], nd.i2ra.ptr[local + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
-                   nd.i2ra.d, nd.fl_i2ra, nd.i_mask[local], t0, t1, nd.in_gi_ra);
+        if (pre >= nd.n_i) return;
+        push_fixed<4>(nd, wclk, pre, r, nd.i2ra.ptr[pre], nd.i2ra.ptr[pre + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
+                      nd.i2ra.d, nd.fl_i2ra, nd.i_mask[pre], t0, t1, nd.in_gi_ra);
     }
 }
 
@@ -628,9 +671,9 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
 
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
-    int nmax = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
-    int blocks_ra = blocks_for(nmax, tu.deliver_block);
-    k_deliver<<<3 * blocks_ra, tu.deliver_block, 0, st>>>(nd, t0, t1, blocks_ra);
+    const int bs = tu.deliver_block;
+    const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(4 * nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
+    k_deliver<<<b0 + b1 + b2, bs, 0, st>>>(nd, t0, t1, b0, b1);
 }
 
 __global__ void k_add_i32(int* a, long long n, int delta)
