@@ -283,7 +283,7 @@ __device__ __forceinline__ unsigned group_sum(unsigned x)
 // order they are written in, so lane r walks the spikes of slots r, r + G, ...
 // and new spikes take the free slots in slot order.
 template <int G>
-__device__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin, int row_end,
+__device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin, int row_end,
                            const int* edge_ids, const int* tgt_arr, const double* w_arr, const double* d_arr,
                            const InFlight& fl, uint64_t new_mask, int t0, int t1, const Inbox& ib)
 {
@@ -403,9 +403,54 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
     return true;
 }
 
+// heads (T, target, edge) of a row's first kRowHeads streams, one column of
+// the block's LDS per thread
+constexpr int kRowHeads = 8;
+struct StreamHeads {
+    double* t_;
+    int* tg_;
+    int* e_;
+    int stride;
+    __device__ __forceinline__ double T(int q) const { return t_[q * stride]; }
+    __device__ __forceinline__ int tg(int q) const { return tg_[q * stride]; }
+    __device__ __forceinline__ int e(int q) const { return e_[q * stride]; }
+    __device__ __forceinline__ void set(int q, double T, int tg, int e)
+    {
+        t_[q * stride] = T;
+        tg_[q * stride] = tg;
+        e_[q * stride] = e;
+    }
+};
+
+// one drained item of row i: edge e to tgt, delivery time T, at step t
+// (NGS:9080-9191)
+__device__ __forceinline__ void rr_item(const NetDev& nd, int i, int e, int tgt, double T, int t, double now, int Nss)
+{
+    // conductance of active synapses (NGS:9080-9084)
+    if (nd.act[e] == 1) inbox_put(nd.in_ge_ra, tgt, t, i, nd.w[e], nd.err);
+    // new presynaptic event (NGS:9091-9102)
+    int pc = nd.pre_cnt[e];
+    double* ring = nd.pre_ring + (size_t)e * kPreRing;
+    bool new_pre = false;
+    if (pc == 0 || T > ring[(pc - 1) % kPreRing] + 30.0) {
+        ring[pc % kPreRing] = T;
+        nd.pre_cnt[e] = pc + 1;
+        if (T > nd.last_pre[i]) nd.last_pre[i] = T;
+        new_pre = true;
+    }
+    if (new_pre) {
+        if (nd.sup_cnt[i] == Nss) {
+            if (nd.sup[e] == 1) ltd_on_delivery(nd, i, e, tgt, T, now);
+            if (nd.sup_cnt[i] < Nss) update_row(nd, i);
+        } else {
+            ltd_on_delivery(nd, i, e, tgt, T, now);
+        }
+    }
+}
+
 // Growth-mode RA->RA row of presynaptic neuron i for the window t0..t1
 // (NGS:8991-9030 enqueue, 9069-9191 drain with LTD, 9375-9554 at the sync).
-__device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int t1)
+__device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int t1, StreamHeads hd)
 {
     const int Nss = nd.syn.Nss;
     uint64_t mask = nd.soma_mask[i];
@@ -418,10 +463,15 @@ __device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int 
     int* st_mode = nd.st_mode + (size_t)i * kMaxStreams;
     int* st_cur = nd.st_cursor + (size_t)i * kMaxStreams;
     int* st_n = nd.st_n + (size_t)i * kMaxStreams;
+    // heads of the first kRowHeads streams kept in LDS (hd; T = +inf: exhausted):
+    // a drain step then peeks only the stream it consumed.  Rows with more
+    // streams take the uncached path.
+    bool cached = false;
     for (int t = t0; t <= t1; ++t) {
         const bool spike = (mask >> (t - t0)) & 1ull;
         const double now = wclk[t - t0 + 1];
         if (!spike && !(now >= next_T)) continue;
+        bool exhausted = false;   // a stream ran out in this step (the pass below then drops it)
         if (spike) {
             if (sc == kMaxStreams) {
                 atomicOr(nd.err, (unsigned)kErrStreams);
@@ -458,11 +508,51 @@ __device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int 
                     st_n[sc] = nd.rr_ptr[i + 1] - nd.rr_ptr[i];
                 }
                 ++sc;
+                if (cached) {
+                    if (sc > kRowHeads) {
+                        cached = false;
+                    } else {
+                        double T;
+                        int tg, e;
+                        if (!stream_peek(nd, i, sc - 1, T, tg, e)) { T = HUGE_VAL; exhausted = true; }
+                        hd.set(sc - 1, T, tg, e);
+                    }
+                }
             }
         }
         nd.st_count[i] = sc;
+        if (!cached && sc <= kRowHeads) {
+            for (int q = 0; q < sc; ++q) {
+                double T;
+                int tg, e;
+                if (!stream_peek(nd, i, q, T, tg, e)) { T = HUGE_VAL; exhausted = true; }
+                hd.set(q, T, tg, e);
+            }
+            cached = true;
+        }
         // drain in (delivery_time, target) order across all in-flight spikes
         for (;;) {
+            if (cached) {
+                int best = -1;
+                double bT = 0;
+                int btg = 0, be = 0;
+                for (int q = 0; q < sc; ++q) {
+                    const double T = hd.T(q);
+                    const int tg = hd.tg(q);
+                    if (now >= T && (best < 0 || T < bT || (T == bT && tg < btg))) {
+                        best = q; bT = T; btg = tg; be = hd.e(q);
+                    }
+                }
+                if (best < 0) break;
+                st_cur[best] += 1;
+                // the consumed stream's next head, loaded while the item is processed
+                double nT;
+                int ntg, ne;
+                if (!stream_peek(nd, i, best, nT, ntg, ne)) { nT = HUGE_VAL; exhausted = true; }
+                hd.set(best, nT, ntg, ne);
+                rr_item(nd, i, be, btg, bT, t, now, Nss);
+                continue;
+            }
             int best = -1;
             double bT = 0;
             int btg = 0, be = 0;
@@ -477,26 +567,16 @@ __device__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int 
             }
             if (best < 0) break;
             st_cur[best] += 1;
-            // conductance of active synapses (NGS:9080-9084)
-            if (nd.act[be] == 1) inbox_put(nd.in_ge_ra, btg, t, i, nd.w[be], nd.err);
-            // new presynaptic event (NGS:9091-9102)
-            int pc = nd.pre_cnt[be];
-            double* ring = nd.pre_ring + (size_t)be * kPreRing;
-            bool new_pre = false;
-            if (pc == 0 || bT > ring[(pc - 1) % kPreRing] + 30.0) {
-                ring[pc % kPreRing] = bT;
-                nd.pre_cnt[be] = pc + 1;
-                if (bT > nd.last_pre[i]) nd.last_pre[i] = bT;
-                new_pre = true;
+            rr_item(nd, i, be, btg, bT, t, now, Nss);
+        }
+        if (cached) {
+            // no stream ran out: the table is unchanged, next_T from the heads
+            if (!exhausted) {
+                next_T = HUGE_VAL;
+                for (int q = 0; q < sc; ++q) next_T = hd.T(q) < next_T ? hd.T(q) : next_T;
+                continue;
             }
-            if (new_pre) {
-                if (nd.sup_cnt[i] == Nss) {
-                    if (nd.sup[be] == 1) ltd_on_delivery(nd, i, be, btg, bT, now);
-                    if (nd.sup_cnt[i] < Nss) update_row(nd, i);
-                } else {
-                    ltd_on_delivery(nd, i, be, btg, bT, now);
-                }
-            }
+            cached = false;   // the pass below renumbers the streams
         }
         // drop exhausted streams, keeping order
         int w = 0;
@@ -532,6 +612,7 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
 {
     int b = blockIdx.x;
     __shared__ double wclk[kMaxWindowSteps + 1];
+    extern __shared__ double heads_lds[];   // kRowHeads x blockDim: T, then target and edge ids
     for (int k = threadIdx.x; k <= t1 - t0 + 1; k += blockDim.x) wclk[k] = nd.clock[t0 - 1 + k];
     __syncthreads();
     // the other post-event buffer was read by the previous window's STDP (in
@@ -541,8 +622,11 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
         // RA->RA: one lane per row
         const int i = b * blockDim.x + threadIdx.x;
         if (i >= nd.n_ra) return;
+        const int bs = blockDim.x;
+        const StreamHeads hd{heads_lds + threadIdx.x, (int*)(heads_lds + kRowHeads * bs) + threadIdx.x,
+                             (int*)(heads_lds + kRowHeads * bs) + kRowHeads * bs + threadIdx.x, bs};
         if (nd.net_growth[nd.ra_net[i]])
-            rr_row(nd, wclk, i, t0, t1);
+            rr_row(nd, wclk, i, t0, t1, hd);
         else
             push_fixed<1>(nd, wclk, i, 0, nd.rr_ptr[i], nd.rr_ptr[i] + nd.act_cnt[i], nd.chain_e, nd.rr_tgt, nd.w,
                           nd.rr_d, nd.fl_rr, nd.soma_mask[i], t0, t1, nd.in_ge_ra);
@@ -673,7 +757,8 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuni
 {
     const int bs = tu.deliver_block;
     const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(4 * nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
-    k_deliver<<<b0 + b1 + b2, bs, 0, st>>>(nd, t0, t1, b0, b1);
+    const size_t lds = (size_t)kRowHeads * bs * (sizeof(double) + 2 * sizeof(int));
+    k_deliver<<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
 }
 
 __global__ void k_add_i32(int* a, long long n, int delta)
