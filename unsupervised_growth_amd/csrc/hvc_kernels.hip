@@ -256,21 +256,19 @@ __device__ __forceinline__ int arrival_step(const double* wclk, int t0, int lo, 
 
 // value of lane K of the quad, in every lane of the quad (DPP quad_perm)
 template <int K>
-__device__ __forceinline__ unsigned quad_lane(unsigned x)
+__device__ __forceinline__ int quad_lane(int x)
 {
-    return (unsigned)__builtin_amdgcn_mov_dpp((int)x, K * 0x55, 0xF, 0xF, true);
+    return __builtin_amdgcn_mov_dpp(x, K * 0x55, 0xF, 0xF, true);
 }
 template <int G>
-__device__ __forceinline__ unsigned group_or(unsigned x)
+__device__ __forceinline__ int group_min(int x)
 {
-    if constexpr (G == 1) return x;
-    else return quad_lane<0>(x) | quad_lane<1>(x) | quad_lane<2>(x) | quad_lane<3>(x);
-}
-template <int G>
-__device__ __forceinline__ unsigned group_sum(unsigned x)
-{
-    if constexpr (G == 1) return x;
-    else return quad_lane<0>(x) + quad_lane<1>(x) + quad_lane<2>(x) + quad_lane<3>(x);
+    if constexpr (G == 1) {
+        return x;
+    } else {
+        const int a = min(quad_lane<0>(x), quad_lane<1>(x)), b = min(quad_lane<2>(x), quad_lane<3>(x));
+        return min(a, b);
+    }
 }
 
 // Fixed-weight delay queues of one presynaptic neuron (RA->I NGS:8971-8986 +
@@ -278,80 +276,97 @@ __device__ __forceinline__ unsigned group_sum(unsigned x)
 // per neuron (G = 1 or a quad).  Each in-flight spike holds one of the
 // neuron's kMaxInflight slots (spike step, edge cursor; step -1 = free; the
 // table is valid while fl.count, the number of held slots, is non-zero) and
-// walks the delay-sorted edges; the pops arriving by t1 become inbox records.
-// A window's records are summed in the reference's order by the sync whatever
-// order they are written in, so lane r walks the spikes of slots r, r + G, ...
-// and new spikes take the free slots in slot order.
+// walks the row's edges, sorted by delay: the edges due by t1 are a prefix
+// from the cursor, and their pops become inbox records.  The sync sums a
+// window's records in the reference's order whatever order they are written
+// in, so lane r of the group takes edges cursor + r, cursor + r + G, ... (the
+// records' atomics of a spike then overlap) and the group agrees on the new
+// cursor, the first edge not due.  New spikes take the free slots in slot order.
 template <int G>
-__device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin, int row_end,
-                           const int* edge_ids, const int* tgt_arr, const double* w_arr, const double* d_arr,
-                           const InFlight& fl, uint64_t new_mask, int t0, int t1, const Inbox& ib)
+__device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin,
+                                           int row_end, const int* edge_ids, const int* tgt_arr, const double* w_arr,
+                                           const double* d_arr, const InFlight& fl, uint64_t new_mask, int t0, int t1,
+                                           const Inbox& ib)
 {
-    constexpr int K = kMaxInflight / G;
     const int deg = row_end - row_begin;
     const int held = fl.count[pre];
     if (deg == 0 || (held == 0 && new_mask == 0)) return;
     int* fstep = fl.step + (size_t)pre * kMaxInflight;
     int* fcur = fl.cursor + (size_t)pre * kMaxInflight;
-    int st[K];
-    unsigned own_free = 0;
+    unsigned held_mask = 0;
+    if (held > 0) {
+        const int4* q4 = (const int4*)fstep;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        st[k] = held > 0 ? fstep[r + G * k] : -1;
-        if (st[k] < 0) own_free |= 1u << (r + G * k);
+        for (int k = 0; k < kMaxInflight / 4; ++k) {
+            const int4 v = q4[k];
+            held_mask |= (v.x >= 0 ? 1u : 0u) << (4 * k) | (v.y >= 0 ? 2u : 0u) << (4 * k) |
+                         (v.z >= 0 ? 4u : 0u) << (4 * k) | (v.w >= 0 ? 8u : 0u) << (4 * k);
+        }
     }
-    const unsigned free_mask = group_or<G>(own_free);
+    // the first nnew free slots take the window's new spikes (in step order)
     const int nnew = __popcll(new_mask);
-    if (r == 0 && nnew > __popc(free_mask)) atomicOr(nd.err, (unsigned)kErrInflight);
+    unsigned free_mask = ~held_mask & ((1u << kMaxInflight) - 1u), assigned = 0;
+    for (int j = 0; j < nnew && free_mask; ++j) {
+        const unsigned low = free_mask & (0u - free_mask);
+        assigned |= low;
+        free_mask &= ~low;
+    }
+    if (r == 0 && nnew > __popc(assigned)) atomicOr(nd.err, (unsigned)kErrInflight);
+    if (held == 0 && r == 0) {
+        // the table was not valid: mark the other slots free
+        for (unsigned m = free_mask; m; m &= m - 1) fstep[__ffs(m) - 1] = -1;
+    }
     const double tend = wclk[t1 - t0 + 1];
-    unsigned now_held = 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int slot = r + G * k;
-        int s = st[k], c = 0;
-        bool fresh = false;
-        if (s < 0) {
-            const int rank = __popc(free_mask & ((1u << slot) - 1u));
-            if (rank >= nnew) {
-                if (held == 0) fstep[slot] = -1;   // the table was not valid: mark the slot free
-                continue;
-            }
-            uint64_t m = new_mask;
-            for (int j = 0; j < rank; ++j) m &= m - 1;
-            s = t0 + __ffsll((unsigned long long)m) - 1;
-            fresh = true;
+    int now_held = 0;
+    for (unsigned m = held_mask | assigned; m; m &= m - 1) {
+        const int k = __ffs(m) - 1;
+        int s, c;
+        const bool fresh = (assigned >> k) & 1u;
+        if (fresh) {
+            uint64_t nm = new_mask;
+            for (int j = __popc(assigned & ((1u << k) - 1u)); j > 0; --j) nm &= nm - 1;
+            s = t0 + __ffsll((unsigned long long)nm) - 1;
+            c = 0;
         } else {
-            c = fcur[slot];
+            s = fstep[k];
+            c = fcur[k];
         }
         const double cs = nd.clock[s];
         int lo = s > t0 ? s : t0;
+        int cc = c + r, stop = deg;
         // the next edge's delay is loaded while this edge's record is written
-        int e = c < deg ? (edge_ids ? edge_ids[row_begin + c] : row_begin + c) : 0;
-        double d = c < deg ? d_arr[e] : 0.0;
-        while (c < deg) {
+        int e = cc < deg ? (edge_ids ? edge_ids[row_begin + cc] : row_begin + cc) : 0;
+        double d = cc < deg ? d_arr[e] : 0.0;
+        while (cc < deg) {
             const double T = cs + d;
-            if (T > tend) break;
+            if (T > tend) {
+                stop = cc;
+                break;
+            }
             const int tg = tgt_arr[e];
             const double w = w_arr[e];
-            const int en = c + 1 < deg ? (edge_ids ? edge_ids[row_begin + c + 1] : row_begin + c + 1) : e;
+            const int cn = cc + G;
+            const int en = cn < deg ? (edge_ids ? edge_ids[row_begin + cn] : row_begin + cn) : e;
             const double dn = d_arr[en];
             const int a = arrival_step(wclk, t0, lo, T);
             lo = a;
             inbox_put(ib, tg, a, pre, w, nd.err);
-            ++c;
+            cc = cn;
             e = en;
             d = dn;
         }
-        if (c >= deg) {
-            fstep[slot] = -1;
-        } else {
-            if (fresh) fstep[slot] = s;
-            fcur[slot] = c;
-            ++now_held;
+        const int cnew = group_min<G>(stop);   // first edge not due (deg: the spike is done)
+        if (cnew < deg) ++now_held;
+        if (r == 0) {
+            if (cnew >= deg) {
+                if (!fresh || held == 0) fstep[k] = -1;   // (a fresh slot of a valid table is still free)
+            } else {
+                if (fresh) fstep[k] = s;
+                fcur[k] = cnew;
+            }
         }
     }
-    const unsigned total = group_sum<G>(now_held);
-    if (r == 0) fl.count[pre] = (int)total;
+    if (r == 0) fl.count[pre] = now_held;
 }
 
 // LTD against the target's post events not older than STDP_WINDOW (NGS:9111-9180)
