@@ -308,6 +308,21 @@ __device__ __forceinline__ void wave_append(int2* log, int* cnt, int cap, bool p
     }
 }
 
+// the window's spikes of the lanes (bit k of mask: step t0 + k) appended after
+// the steps, one atomic per wave and round (a lane spikes at most once per
+// window in practice, so one round); the host sorts the log
+__device__ __forceinline__ void window_append(int2* log, int* cnt, int cap, uint64_t mask, int neuron, int t0,
+                                              unsigned int* err)
+{
+    for (;;) {
+        const bool pred = mask != 0;
+        if (__ballot(pred) == 0) return;
+        const int step = pred ? t0 + __ffsll((unsigned long long)mask) - 1 : 0;
+        wave_append(log, cnt, cap, pred, neuron, step, err);
+        mask &= mask - 1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // integrate: HVC-RA neurons (NGS:8931-9216 minus delivery) and HVC-I neurons
 // (NGS:9221-9286 minus delivery) through the steps t0..t1 of one window, in one
@@ -408,20 +423,37 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
         bool dend = live && leader && (f & 2);
         if (soma) smask |= 1ull << (t - t0);
         if (dend) dmask |= 1ull << (t - t0);
-        wave_append(nd.soma_log, &nd.log_cnt[0], nd.log_cap, soma, i, t, nd.err);
-        wave_append(nd.dend_log, &nd.log_cnt[1], nd.log_cap, dend, i, t, nd.err);
-        if (growth && soma) {
-            // new postsynaptic event (NGS:8956-8967)
-            double it = win.clk(t);
-            if (pcnt == 0 || it > last_post + 30.0) {
-                int pos = atomicAdd(nd.post_list_cnt, 1);
-                if (pos < kPostListCap)
-                    nd.post_list[pos] = post_key(nd.ra_net[i], t, i);
-                else
-                    atomicOr(nd.err, (unsigned)kErrPostList);
-                ++new_posts;
-            }
+    }
+    // after the steps: the spike logs, and the new postsynaptic events (NGS:8956-8967;
+    // a window's events become visible in the next window, so each spike is
+    // tested against the events of earlier windows only, as in the step loop)
+    window_append(nd.soma_log, &nd.log_cnt[0], nd.log_cap, smask, i, t0, nd.err);
+    window_append(nd.dend_log, &nd.log_cnt[1], nd.log_cap, dmask, i, t0, nd.err);
+    uint64_t pmask = 0;
+    if (growth) {
+        for (uint64_t m = smask; m; m &= m - 1) {
+            const int t = t0 + __ffsll((unsigned long long)m) - 1;
+            if (pcnt == 0 || win.clk(t) > last_post + 30.0) pmask |= 1ull << (t - t0);
         }
+    }
+    for (;;) {
+        const bool pred = pmask != 0;
+        const uint64_t b = __ballot(pred);
+        if (b == 0) break;
+        const int lane = __lane_id(), lead = __ffsll((unsigned long long)b) - 1;
+        int base = 0;
+        if (lane == lead) base = atomicAdd(nd.post_list_cnt, __popcll(b));
+        base = __shfl(base, lead);
+        if (pred) {
+            const int t = t0 + __ffsll((unsigned long long)pmask) - 1;
+            const int pos = base + __popcll(b & ((1ull << lane) - 1ull));
+            if (pos < kPostListCap)
+                nd.post_list[pos] = post_key(nd.ra_net[i], t, i);
+            else
+                atomicOr(nd.err, (unsigned)kErrPostList);
+            ++new_posts;
+        }
+        pmask &= pmask - 1;
     }
     if (live && leader) {
         nd.ra[0][i] = HVC_WIDE(s, Vs); nd.ra[1][i] = HVC_WIDE(s, Vd); nd.ra[2][i] = HVC_WIDE(s, n);
@@ -469,8 +501,8 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
         i_step<MODEL>(s, (real)nd.dt, win.row(t));
         bool fired = live && f;
         if (fired) mask |= 1ull << (t - t0);
-        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
     }
+    window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
     if (live) {
         nd.in[0][i] = HVC_WIDE(s, V); nd.in[1][i] = HVC_WIDE(s, n); nd.in[2][i] = HVC_WIDE(s, m);
         nd.in[3][i] = HVC_WIDE(s, h); nd.in[4][i] = HVC_WIDE(s, w);
@@ -531,8 +563,8 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
         i_step_lanes(s, xlo, role, base, (real)nd.dt, win.row(t));
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
-        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
     }
+    window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
     if (live) {
         // each lane writes its own variable, widened with its low part
         const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
@@ -595,8 +627,8 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
         i_step_quad(s, xlo, wlo, role, ga, gb, (real)nd.dt, win.row(t));
         bool fired = live && f && role == 0;
         if (fired) mask |= 1ull << (t - t0);
-        wave_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, fired, i, t, nd.err);
     }
+    window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
     if (live) {
         // each lane writes its own variable, widened with its low part; lane 0 also w
         const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
