@@ -142,8 +142,21 @@ struct Win {
     const DecayRow* rows;   // rows[t - t0]
     const double* clock;    // clock[t - t0 + 1] = clock at step t
     int t0;
+    double* lds;            // the block's copy: nsteps rows, then the clock
+    int nsteps;
     __device__ __forceinline__ const DecayRow& row(int t) const { return rows[t - t0]; }
     __device__ __forceinline__ double clk(int t) const { return clock[t - t0 + 1]; }
+    // copy the tables into LDS (every thread of the block, after it has issued
+    // its state loads, so their latencies overlap)
+    __device__ __forceinline__ void stage(const NetDev& nd) const
+    {
+        constexpr int kRowWords = (int)(sizeof(DecayRow) / sizeof(double));
+        const double* grow = (const double*)((const DecayRow*)nd.decay + t0);
+        for (int k = threadIdx.x; k < nsteps * kRowWords; k += blockDim.x) lds[k] = grow[k];
+        double* wc = lds + nsteps * kRowWords;
+        for (int k = threadIdx.x; k <= nsteps; k += blockDim.x) wc[k] = nd.clock[t0 - 1 + k];
+        __syncthreads();
+    }
 };
 
 // The neuron's next two noise events of the chunk (lists are generated by
@@ -355,6 +368,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     HVC_LOAD(s, Vs, nd.ra[0][ii]) HVC_LOAD(s, Vd, nd.ra[1][ii]) HVC_LOAD(s, n, nd.ra[2][ii])
     HVC_LOAD(s, l, nd.ra[3][ii]) HVC_LOAD(s, h, nd.ra[4][ii]) HVC_LOAD(s, r, nd.ra[5][ii])
     HVC_LOAD(s, c, nd.ra[6][ii]) HVC_LOAD(s, Ca, nd.ra[7][ii])
+    win.stage(nd);
     // the previous window's sync: raiseI (Gi_d += G), raiseExcWeight (Ge_d += w / Ad)
     double ge_d = nd.ra[10][ii], gi_d = nd.ra[11][ii];
     if (live && leader) {
@@ -477,6 +491,7 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G)
     double ge = nd.in[5][ii];
     if (live) {
@@ -533,6 +548,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G), by the group's first lane
     double ge = nd.in[5][ii];
     if (live && role == 0) {
@@ -596,6 +612,7 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G), by the quad's first lane
     double ge = nd.in[5][ii];
     if (live && role == 0) {
@@ -665,17 +682,12 @@ __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, in
         stdp_block(nd, sa, ((int)blockIdx.x - iblocks - rblocks) * blockDim.x, stdp_slice, &stdp_count);
         return;
     }
-    // stage the window's decay rows and clock in LDS (dynamic shared memory of
-    // win_bytes(t1 - t0 + 1) bytes)
+    // the window's decay rows and clock, staged in LDS (dynamic shared memory of
+    // win_bytes(t1 - t0 + 1) bytes) by Win::stage in the neuron functions
     extern __shared__ double win_lds[];
     const int nsteps = t1 - t0 + 1;
     constexpr int kRowWords = (int)(sizeof(DecayRow) / sizeof(double));
-    const double* grow = (const double*)((const DecayRow*)nd.decay + t0);
-    for (int k = threadIdx.x; k < nsteps * kRowWords; k += blockDim.x) win_lds[k] = grow[k];
-    double* wclock = win_lds + nsteps * kRowWords;
-    for (int k = threadIdx.x; k <= nsteps; k += blockDim.x) wclock[k] = nd.clock[t0 - 1 + k];
-    __syncthreads();
-    const Win win{(const DecayRow*)win_lds, wclock, t0};
+    const Win win{(const DecayRow*)win_lds, win_lds + nsteps * kRowWords, t0, win_lds, nsteps};
     if ((int)blockIdx.x < iblocks) {
         if constexpr ((LANES & 4) != 0)
             integrate_i_quad(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
