@@ -22,6 +22,8 @@
 // so the FP64 form reproduces the CPU results bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hvc_internal.h"
 #include "hvc_integrate.hip.h"
 #include "hvc_stdp.hip.h"
@@ -270,18 +272,36 @@ __device__ __forceinline__ int group_min(int x)
         return min(a, b);
     }
 }
+template <int G>
+__device__ __forceinline__ double group_min(double x)
+{
+    if constexpr (G == 1) {
+        return x;
+    } else {
+        auto lane = [](double v, auto k) {
+            const long long b = __double_as_longlong(v);
+            constexpr int K = decltype(k)::value;
+            const int lo = quad_lane<K>((int)b), hi = quad_lane<K>((int)(b >> 32));
+            return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+        };
+        const double a = fmin(lane(x, std::integral_constant<int, 0>{}), lane(x, std::integral_constant<int, 1>{}));
+        const double c = fmin(lane(x, std::integral_constant<int, 2>{}), lane(x, std::integral_constant<int, 3>{}));
+        return fmin(a, c);
+    }
+}
 
 // Fixed-weight delay queues of one presynaptic neuron (RA->I NGS:8971-8986 +
 // 9039-9062, I->RA NGS:9235-9284, chain-test RA->RA NGS:13222-13250), G lanes
 // per neuron (G = 1 or a quad).  Each in-flight spike holds one of the
-// neuron's kMaxInflight slots (spike step, edge cursor; step -1 = free; the
-// table is valid while fl.count, the number of held slots, is non-zero) and
-// walks the row's edges, sorted by delay: the edges due by t1 are a prefix
-// from the cursor, and their pops become inbox records.  The sync sums a
-// window's records in the reference's order whatever order they are written
-// in, so lane r of the group takes edges cursor + r, cursor + r + G, ... (the
-// records' atomics of a spike then overlap) and the group agrees on the new
-// cursor, the first edge not due.  New spikes take the free slots in slot order.
+// neuron's kMaxInflight slots: spike step (-1 = free), edge cursor and the
+// delivery time of the cursor's edge (the table is valid while fl.count, the
+// number of held slots, is non-zero).  The row's edges are sorted by delay, so
+// the edges due by t1 are a prefix from the cursor; their pops become inbox
+// records.  The sync sums a window's records in the reference's order
+// whatever order they are written in, so lane r of the group takes edges
+// cursor + r, cursor + r + G, ... (the records' atomics of a spike overlap)
+// and the group agrees on the new cursor, the first edge not due.  New spikes
+// take the free slots in slot order.
 template <int G>
 __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin,
                                            int row_end, const int* edge_ids, const int* tgt_arr, const double* w_arr,
@@ -293,14 +313,23 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
     if (deg == 0 || (held == 0 && new_mask == 0)) return;
     int* fstep = fl.step + (size_t)pre * kMaxInflight;
     int* fcur = fl.cursor + (size_t)pre * kMaxInflight;
-    unsigned held_mask = 0;
+    double* fnext = fl.tnext + (size_t)pre * kMaxInflight;
+    const double tend = wclk[t1 - t0 + 1];
+    // the held slots and which of them have an edge due in this window (the
+    // step, cursor and next-time tables are read together)
+    unsigned held_mask = 0, due_mask = 0;
     if (held > 0) {
         const int4* q4 = (const int4*)fstep;
+        const double2* t2 = (const double2*)fnext;
 #pragma unroll
         for (int k = 0; k < kMaxInflight / 4; ++k) {
             const int4 v = q4[k];
-            held_mask |= (v.x >= 0 ? 1u : 0u) << (4 * k) | (v.y >= 0 ? 2u : 0u) << (4 * k) |
-                         (v.z >= 0 ? 4u : 0u) << (4 * k) | (v.w >= 0 ? 8u : 0u) << (4 * k);
+            const double2 ta = t2[2 * k], tb = t2[2 * k + 1];
+            held_mask |= ((v.x >= 0 ? 1u : 0u) | (v.y >= 0 ? 2u : 0u) | (v.z >= 0 ? 4u : 0u) | (v.w >= 0 ? 8u : 0u))
+                         << (4 * k);
+            due_mask |= ((v.x >= 0 && ta.x <= tend ? 1u : 0u) | (v.y >= 0 && ta.y <= tend ? 2u : 0u) |
+                         (v.z >= 0 && tb.x <= tend ? 4u : 0u) | (v.w >= 0 && tb.y <= tend ? 8u : 0u))
+                        << (4 * k);
         }
     }
     // the first nnew free slots take the window's new spikes (in step order)
@@ -316,12 +345,11 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         // the table was not valid: mark the other slots free
         for (unsigned m = free_mask; m; m &= m - 1) fstep[__ffs(m) - 1] = -1;
     }
-    const double tend = wclk[t1 - t0 + 1];
-    int now_held = 0;
-    for (unsigned m = held_mask | assigned; m; m &= m - 1) {
+    int now_held = __popc(held_mask & ~due_mask);   // held slots with nothing due stay as they are
+    for (unsigned m = due_mask | assigned; m; m &= m - 1) {
         const int k = __ffs(m) - 1;
-        int s, c;
         const bool fresh = (assigned >> k) & 1u;
+        int s, c;
         if (fresh) {
             uint64_t nm = new_mask;
             for (int j = __popc(assigned & ((1u << k) - 1u)); j > 0; --j) nm &= nm - 1;
@@ -334,6 +362,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         const double cs = nd.clock[s];
         int lo = s > t0 ? s : t0;
         int cc = c + r, stop = deg;
+        double tstop = HUGE_VAL;   // delivery time of this lane's first edge not due
         // the next edge's delay is loaded while this edge's record is written
         int e = cc < deg ? (edge_ids ? edge_ids[row_begin + cc] : row_begin + cc) : 0;
         double d = cc < deg ? d_arr[e] : 0.0;
@@ -341,6 +370,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
             const double T = cs + d;
             if (T > tend) {
                 stop = cc;
+                tstop = T;
                 break;
             }
             const int tg = tgt_arr[e];
@@ -355,7 +385,10 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
             e = en;
             d = dn;
         }
-        const int cnew = group_min<G>(stop);   // first edge not due (deg: the spike is done)
+        // the first edge not due over the group (deg: the spike is done) and its
+        // delivery time (delays are sorted, so it is the group's smallest)
+        const int cnew = group_min<G>(stop);
+        const double tnew = group_min<G>(tstop);
         if (cnew < deg) ++now_held;
         if (r == 0) {
             if (cnew >= deg) {
@@ -363,6 +396,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
             } else {
                 if (fresh) fstep[k] = s;
                 fcur[k] = cnew;
+                fnext[k] = tnew;
             }
         }
     }
