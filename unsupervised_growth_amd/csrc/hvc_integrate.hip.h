@@ -87,8 +87,8 @@ __device__ __forceinline__ void ra_step_runge4(RAState& s, real el, real gca, co
                      (ra_runge4<true, LANES>(bad, s, el, gca, u, sl, E_third, E_twothird, E_dt)))
 }
 
-template <int MODEL>
-__device__ __forceinline__ void i_step(IState& s, real dt, const DecayRow& row)
+template <int MODEL, class Row>
+__device__ __forceinline__ void i_step(IState& s, real dt, const Row& row)
 {
     if (MODEL == 0) {
         HVC_GUARDED_STEP(IState, (i_dp8<false>(bad, s, dt, row)), (i_dp8<true>(bad, s, dt, row)))
@@ -97,14 +97,16 @@ __device__ __forceinline__ void i_step(IState& s, real dt, const DecayRow& row)
     }
 }
 
-__device__ __forceinline__ void i_step_lanes(IState& s, real& xlo, int role, int base, real dt, const DecayRow& row)
+template <class Row>
+__device__ __forceinline__ void i_step_lanes(IState& s, real& xlo, int role, int base, real dt, const Row& row)
 {
     HVC_GUARDED_STEP(IState, (i_dp8_lanes<false>(bad, s, xlo, role, base, dt, row)),
                      (i_dp8_lanes<true>(bad, s, xlo, role, base, dt, row)))
 }
 
+template <class Row>
 __device__ __forceinline__ void i_step_quad(IState& s, real& xlo, real& wlo, int role, const GateSlot& ga,
-                                            const GateSlot& gb, real dt, const DecayRow& row)
+                                            const GateSlot& gb, real dt, const Row& row)
 {
     // the low parts are state too: a repeated step starts from their saved values
     real xlo0 = xlo, wlo0 = wlo;
@@ -139,21 +141,34 @@ __device__ __forceinline__ RAUniform ra_uniform(const NetDev& nd)
 // rows of the steps t0..t1 and the clock at t0 - 1 .. t1 (every step of every
 // neuron reads them; from HBM / L2 each read would stall the step).
 struct Win {
-    const DecayRow* rows;   // rows[t - t0]
+    const WinRow* rows;     // rows[t - t0]
     const double* clock;    // clock[t - t0 + 1] = clock at step t
     int t0;
     double* lds;            // the block's copy: nsteps rows, then the clock
     int nsteps;
-    __device__ __forceinline__ const DecayRow& row(int t) const { return rows[t - t0]; }
+    __device__ __forceinline__ const WinRow& row(int t) const { return rows[t - t0]; }
     __device__ __forceinline__ double clk(int t) const { return clock[t - t0 + 1]; }
     // copy the tables into LDS (every thread of the block, after it has issued
     // its state loads, so their latencies overlap)
     __device__ __forceinline__ void stage(const NetDev& nd) const
     {
+        const DecayRow* grow = (const DecayRow*)nd.decay + t0;
+        WinRow* rl = (WinRow*)lds;
+#ifdef HVC_FP32
+        for (int k = threadIdx.x; k < nsteps * kNumOffsets; k += blockDim.x) {
+            const int r = k / kNumOffsets, o = k - r * kNumOffsets;
+            rl[r].e5[o] = (float)grow[r].e5[o];
+            rl[r].e2[o] = (float)grow[r].e2[o];
+            if (o == kOffDt) {
+                rl[r].e5dt = grow[r].e5[o];
+                rl[r].e2dt = grow[r].e2[o];
+            }
+        }
+#else
         constexpr int kRowWords = (int)(sizeof(DecayRow) / sizeof(double));
-        const double* grow = (const double*)((const DecayRow*)nd.decay + t0);
-        for (int k = threadIdx.x; k < nsteps * kRowWords; k += blockDim.x) lds[k] = grow[k];
-        double* wc = lds + nsteps * kRowWords;
+        for (int k = threadIdx.x; k < nsteps * kRowWords; k += blockDim.x) lds[k] = ((const double*)grow)[k];
+#endif
+        double* wc = (double*)(rl + nsteps);
         for (int k = threadIdx.x; k <= nsteps; k += blockDim.x) wc[k] = nd.clock[t0 - 1 + k];
         __syncthreads();
     }
@@ -416,7 +431,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     for (int t = t0; t <= t1; ++t) {
         if (kick == t) s.Ge_d += 3.0;  // raiseE(G_TRAINING_KICK), NGS:8933-8940
         int f = ra_state_check(s.Vs, s.Vd, fs, fd);
-        const DecayRow& row = win.row(t);
+        const WinRow& row = win.row(t);
         if (MODEL == 0) {
             U4 o = philox4x32((uint32_t)t, sid.local, sid.trial, kStreamRAWhite, sid.k0, sid.k1);
             ra_step_debraband(s, el, gca, u, kPointDistribution[point_index(o.x)], kPointDistribution[point_index(o.y)],
@@ -686,8 +701,7 @@ __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, in
     // win_bytes(t1 - t0 + 1) bytes) by Win::stage in the neuron functions
     extern __shared__ double win_lds[];
     const int nsteps = t1 - t0 + 1;
-    constexpr int kRowWords = (int)(sizeof(DecayRow) / sizeof(double));
-    const Win win{(const DecayRow*)win_lds, win_lds + nsteps * kRowWords, t0, win_lds, nsteps};
+    const Win win{(const WinRow*)win_lds, (const double*)((const WinRow*)win_lds + nsteps), t0, win_lds, nsteps};
     if ((int)blockIdx.x < iblocks) {
         if constexpr ((LANES & 4) != 0)
             integrate_i_quad(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
@@ -705,7 +719,7 @@ __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, in
 }
 
 // LDS bytes of k_integrate for a window of n steps
-static inline size_t win_bytes(int n) { return (size_t)n * sizeof(DecayRow) + (size_t)(n + 1) * sizeof(double); }
+static inline size_t win_bytes(int n) { return (size_t)n * sizeof(WinRow) + (size_t)(n + 1) * sizeof(double); }
 
 }  // namespace p32 / p64
 

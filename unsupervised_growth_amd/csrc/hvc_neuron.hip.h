@@ -101,6 +101,25 @@ struct DecayRow {
     double e5[kNumOffsets];
     double e2[kNumOffsets];
 };
+// the row as the FP32 integration reads it from LDS: the stage factors
+// rounded to float once, when the window is staged (they enter float
+// expressions anyway), and the whole-step factors kept in FP64 for the
+// conductance decays, which take one rounding to float per step
+struct DecayRowF {
+    float e5[kNumOffsets];
+    float e2[kNumOffsets];
+    double e5dt, e2dt;
+};
+// the whole-step decay factors of a row (FP64 in both row types)
+__device__ __forceinline__ double row_e5dt(const DecayRow& r) { return r.e5[kOffDt]; }
+__device__ __forceinline__ double row_e2dt(const DecayRow& r) { return r.e2[kOffDt]; }
+__device__ __forceinline__ double row_e5dt(const DecayRowF& r) { return r.e5dt; }
+__device__ __forceinline__ double row_e2dt(const DecayRowF& r) { return r.e2dt; }
+#ifdef HVC_FP32
+typedef DecayRowF WinRow;
+#else
+typedef DecayRow WinRow;
+#endif
 
 // ---------------- HVC-RA ----------------
 struct RAState {
@@ -426,8 +445,8 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
 }
 
 // HHI_buffer::DP8_step; row = decay factors of this step
-template <bool EXACT>
-__device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const DecayRow& row)
+template <bool EXACT, class Row>
+__device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Row& row)
 {
     HVC_FP_FAST
     IK k1 = i_rhs<EXACT>(bad, s, s.V, s.n, s.m, s.h, s.w, HR(1.0), HR(1.0));
@@ -480,8 +499,8 @@ __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Deca
     HVC_ACC(s, m, dt * CF(m))
     HVC_ACC(s, h, dt * CF(h))
     HVC_ACC(s, w, dt * CF(w))
-    s.Ge = s.Ge * row.e2[kOffDt];
-    s.Gi = s.Gi * row.e5[kOffDt];
+    s.Ge = s.Ge * row_e2dt(row);
+    s.Gi = s.Gi * row_e5dt(row);
 #undef C5
 #undef C6
 #undef C7
@@ -494,8 +513,8 @@ __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Deca
 }
 
 // HHI_buffer::Runge4_step
-template <bool EXACT>
-__device__ __forceinline__ void i_runge4(Guard& bad, IState& s, real dt, const DecayRow& row)
+template <bool EXACT, class Row>
+__device__ __forceinline__ void i_runge4(Guard& bad, IState& s, real dt, const Row& row)
 {
     HVC_FP_FAST
     IK k1 = i_rhs<EXACT>(bad, s, s.V, s.n, s.m, s.h, s.w, HR(1.0), HR(1.0));
@@ -516,8 +535,8 @@ __device__ __forceinline__ void i_runge4(Guard& bad, IState& s, real dt, const D
     HVC_ACC(s, h, F(h))
     HVC_ACC(s, w, F(w))
 #undef F
-    s.Ge = s.Ge * row.e2[kOffDt];
-    s.Gi = s.Gi * row.e5[kOffDt];
+    s.Ge = s.Ge * row_e2dt(row);
+    s.Gi = s.Gi * row_e5dt(row);
 }
 
 // HHI_buffer::state_check (HHI_buffer.cpp:260-276)
@@ -687,9 +706,9 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
     w1 = __shfl(x, base + 4);
 
 // HHI_buffer::DP8_step with five lanes per neuron; s is replicated in the group
-template <bool EXACT>
+template <bool EXACT, class Row>
 __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, int role, int base, real dt,
-                                            const DecayRow& row)
+                                            const Row& row)
 {
     HVC_FP_FAST
     const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
@@ -737,8 +756,8 @@ __device__ __forceinline__ void i_dp8_lanes(Guard& bad, IState& s, real& xlo, in
     }
     HVC_BCAST5(x)
     s.V = v; s.n = n1; s.m = m1; s.h = h1; s.w = w1;
-    s.Ge = s.Ge * row.e2[kOffDt];
-    s.Gi = s.Gi * row.e5[kOffDt];
+    s.Ge = s.Ge * row_e2dt(row);
+    s.Gi = s.Gi * row_e5dt(row);
 }
 #undef HVC_BCAST5
 
@@ -818,9 +837,9 @@ __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role
 
 // HHI_buffer::DP8_step with four lanes per neuron; s is replicated in the quad
 // except w (lane 0); xlo / wlo: low parts of the lane's variable and of w (FP32 form)
-template <bool EXACT>
+template <bool EXACT, class Row>
 __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, real& wlo, int role, const GateSlot& ga,
-                                           const GateSlot& gb, real dt, const DecayRow& row)
+                                           const GateSlot& gb, real dt, const Row& row)
 {
     HVC_FP_FAST
     const real x0 = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
@@ -863,8 +882,8 @@ __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, rea
     }
     HVC_BCAST4(x)
     s.V = v; s.n = n1; s.m = m1; s.h = h1; s.w = w1;
-    s.Ge = s.Ge * row.e2[kOffDt];
-    s.Gi = s.Gi * row.e5[kOffDt];
+    s.Ge = s.Ge * row_e2dt(row);
+    s.Gi = s.Gi * row_e5dt(row);
 }
 #undef HVC_BCAST4
 
