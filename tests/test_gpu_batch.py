@@ -109,3 +109,35 @@ def test_autotune_keeps_a_valid_shape_and_results():
     for b in range(len(nets)):
         for kind in ("soma", "dend", "interneuron"):
             _same(getattr(again[b], kind), getattr(ref[b], kind), "net %d %s" % (b, kind))
+
+
+def test_large_batch_pushes_equal_each_network_alone():
+    """A batch above 65 536 neurons delivers its fixed-weight edges with one lane
+    per presynaptic neuron, a network alone with a quad per neuron: same bits."""
+    from unsupervised_growth_amd import configs
+    from unsupervised_growth_amd.engine import Engine
+    nets = [configs.sphere_network(np.random.default_rng(41 + b), 18000, 5000, 10.0, k_rr=16, n_train=60,
+                                   spread_max=20.0) for b in range(3)]
+    assert sum(n.n_ra + n.n_i for n in nets) > 65536 and all(n.n_ra + n.n_i <= 65536 for n in nets)
+    keys = [(300 + b, 5) for b in range(len(nets))]
+    trial_ms = 150.0
+    trials = [dict(mode=0, training=True, trial_index=b, first_trial=True, mean_injection_time=40.0 + 5 * b)
+              for b in range(len(nets))]
+    print("large batch sizes=%s" % [(n.n_ra, n.n_i) for n in nets], flush=True)
+    alone = []
+    for b, net in enumerate(nets):
+        with Engine(precision=1, trial_ms=trial_ms, seed=keys[b]) as e:
+            e.upload(net)
+            alone.append((e.run_trial(**trials[b]), e.download_synapses()))
+    with Engine(precision=1, trial_ms=trial_ms) as e:
+        e.upload_batch(nets, keys)
+        batch = e.run_batch(trials)
+        syns = [e.download_synapses(b) for b in range(len(nets))]
+    spikes = 0
+    for b in range(len(nets)):
+        res, syn = alone[b]
+        for kind in ("soma", "dend", "interneuron"):
+            _same(getattr(batch[b], kind), getattr(res, kind), "net %d %s" % (b, kind))
+        spikes += sum(map(len, res.interneuron))
+        np.testing.assert_array_equal(syns[b].w, syn.w, err_msg="net %d w" % b)
+    assert spikes > 0

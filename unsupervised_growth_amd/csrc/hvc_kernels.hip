@@ -452,6 +452,8 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
     return true;
 }
 
+constexpr int kQuadPushNeurons = 65536;   // neurons per context up to which the pushes take a quad per neuron
+
 // heads (T, target, edge) of a row's first kRowHeads streams, one column of
 // the block's LDS per thread
 constexpr int kRowHeads = 8;
@@ -657,6 +659,7 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
 //   role 0: RA->RA rows (growth) or chain-test RA->RA push
 //   role 1: RA->I push        role 2: I->RA push
 // ---------------------------------------------------------------------------
+template <int G>
 __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int b0, int b1)
 {
     int b = blockIdx.x;
@@ -681,17 +684,17 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
                           nd.rr_d, nd.fl_rr, nd.soma_mask[i], t0, t1, nd.in_ge_ra);
         return;
     }
-    // fixed-weight pushes: a quad per presynaptic neuron
+    // fixed-weight pushes: G lanes per presynaptic neuron
     const bool ra = b < b0 + b1;
     const int q = (ra ? b - b0 : b - b0 - b1) * blockDim.x + threadIdx.x;
-    const int pre = q >> 2, r = q & 3;
+    const int pre = q / G, r = q % G;
     if (ra) {
         if (pre >= nd.n_ra) return;
-        push_fixed<4>(nd, wclk, pre, r, nd.ra2i.ptr[pre], nd.ra2i.ptr[pre + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
+        push_fixed<G>(nd, wclk, pre, r, nd.ra2i.ptr[pre], nd.ra2i.ptr[pre + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
                       nd.ra2i.d, nd.fl_ra2i, nd.soma_mask[pre], t0, t1, nd.in_ge_i);
     } else {
         if (pre >= nd.n_i) return;
-        push_fixed<4>(nd, wclk, pre, r, nd.i2ra.ptr[pre], nd.i2ra.ptr[pre + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
+        push_fixed<G>(nd, wclk, pre, r, nd.i2ra.ptr[pre], nd.i2ra.ptr[pre + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
                       nd.i2ra.d, nd.fl_i2ra, nd.i_mask[pre], t0, t1, nd.in_gi_ra);
     }
 }
@@ -805,9 +808,17 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
     const int bs = tu.deliver_block;
-    const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(4 * nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
     const size_t lds = (size_t)kRowHeads * bs * (sizeof(double) + 2 * sizeof(int));
-    k_deliver<<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+    // a quad per presynaptic neuron shortens a neuron's chain of inbox atomics
+    // when few waves are in flight (one network); with many networks the extra
+    // waves cost more than they save, and one lane per neuron is faster
+    if (nd.n_ra + nd.n_i <= kQuadPushNeurons) {
+        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(4 * nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
+        k_deliver<4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+    } else {
+        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(nd.n_ra, bs), b2 = blocks_for(nd.n_i, bs);
+        k_deliver<1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+    }
 }
 
 __global__ void k_add_i32(int* a, long long n, int delta)
