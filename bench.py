@@ -216,7 +216,8 @@ def bench_batch(args):
         "roofline": {"bound": "mfma", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "flops_per_launch": flops_launch,
                      "avg_launch_ms": kern["integrate"],
-                     "kernels_ms_per_trial": {k: v * (1 if k == "decay" else windows) for k, v in kern.items()}},
+                     "kernels_ms_per_trial": {k: v * (1 if k in ("decay", "sync") else windows)
+                                              for k, v in kern.items()}},
         "batch_vs_alone": worst_alone,
         "autotune": {"tuning": tuning, "s": round(t_tune, 2),
                      "ms_per_window": None if tune_ms is None else [float(x) for x in tune_ms]},
