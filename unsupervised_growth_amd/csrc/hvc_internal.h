@@ -95,6 +95,7 @@ struct InFlight {
     int* step;      // [n][kMaxInflight] spike step of a held slot, -1 free
     int* cursor;    // [n][kMaxInflight]
     double* tnext;  // [n][kMaxInflight] delivery time of the cursor's edge
+    double* tspike; // [n][kMaxInflight] clock at the spike step
     int* count;     // [n] held slots (0: the slot table is not valid)
 };
 

@@ -314,6 +314,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
     int* fstep = fl.step + (size_t)pre * kMaxInflight;
     int* fcur = fl.cursor + (size_t)pre * kMaxInflight;
     double* fnext = fl.tnext + (size_t)pre * kMaxInflight;
+    double* fcs = fl.tspike + (size_t)pre * kMaxInflight;
     const double tend = wclk[t1 - t0 + 1];
     // the held slots and which of them have an edge due in this window (the
     // step, cursor and next-time tables are read together)
@@ -350,22 +351,31 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         const int k = __ffs(m) - 1;
         const bool fresh = (assigned >> k) & 1u;
         int s, c;
+        double cs;   // clock at the spike step (a fresh spike is in this window: from the LDS clock)
         if (fresh) {
             uint64_t nm = new_mask;
             for (int j = __popc(assigned & ((1u << k) - 1u)); j > 0; --j) nm &= nm - 1;
             s = t0 + __ffsll((unsigned long long)nm) - 1;
             c = 0;
+            cs = wclk[s - t0 + 1];
         } else {
             s = fstep[k];
             c = fcur[k];
+            cs = fcs[k];
         }
-        const double cs = nd.clock[s];
         int lo = s > t0 ? s : t0;
         int cc = c + r, stop = deg;
         double tstop = HUGE_VAL;   // delivery time of this lane's first edge not due
-        // the next edge's delay is loaded while this edge's record is written
+        // an edge's target, weight and delay are loaded together, the next
+        // edge's while this edge's record is written
         int e = cc < deg ? (edge_ids ? edge_ids[row_begin + cc] : row_begin + cc) : 0;
-        double d = cc < deg ? d_arr[e] : 0.0;
+        double d = 0.0, w = 0.0;
+        int tg = 0;
+        if (cc < deg) {
+            d = d_arr[e];
+            tg = tgt_arr[e];
+            w = w_arr[e];
+        }
         while (cc < deg) {
             const double T = cs + d;
             if (T > tend) {
@@ -373,17 +383,19 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
                 tstop = T;
                 break;
             }
-            const int tg = tgt_arr[e];
-            const double w = w_arr[e];
             const int cn = cc + G;
             const int en = cn < deg ? (edge_ids ? edge_ids[row_begin + cn] : row_begin + cn) : e;
             const double dn = d_arr[en];
+            const int tgn = tgt_arr[en];
+            const double wn = w_arr[en];
             const int a = arrival_step(wclk, t0, lo, T);
             lo = a;
             inbox_put(ib, tg, a, pre, w, nd.err);
             cc = cn;
             e = en;
             d = dn;
+            tg = tgn;
+            w = wn;
         }
         // the first edge not due over the group (deg: the spike is done) and its
         // delivery time (delays are sorted, so it is the group's smallest)
@@ -394,7 +406,10 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
             if (cnew >= deg) {
                 if (!fresh || held == 0) fstep[k] = -1;   // (a fresh slot of a valid table is still free)
             } else {
-                if (fresh) fstep[k] = s;
+                if (fresh) {
+                    fstep[k] = s;
+                    fcs[k] = cs;
+                }
                 fcur[k] = cnew;
                 fnext[k] = tnew;
             }
