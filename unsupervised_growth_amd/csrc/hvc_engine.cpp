@@ -215,7 +215,7 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     // default launch shape for one network: HVC-I in quads in the FP32 form
     // (the DPP exchange shortens DP8's dependent chain), five lanes in FP64
     // (there the extra gate per lane costs more than the exchange saves)
-    ctx->tuning.i_lanes = prm->precision == 1 ? 2 : 1;
+    ctx->tuning.i_lanes = 2;   // four lanes per HVC-I neuron: the fastest single-network layout in both forms
     ctx->tuning.ra_lanes = 0;
 
     // trial clock and synchronisation points (NGS:8906-8925, 9291, 9563)
