@@ -639,11 +639,8 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.i2ra = FixedEdges{ctx->d_b_ptr, ctx->d_b_tgt, ctx->d_b_w, ctx->d_b_d};
     auto mk_inflight = [&](int n) {
         InFlight f;
-        f.step = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
-        f.cursor = dalloc<int>(ctx, (size_t)n * kMaxInflight, &st);
-        f.tnext = dalloc<double>(ctx, (size_t)n * kMaxInflight, &st);
-        f.tspike = dalloc<double>(ctx, (size_t)n * kMaxInflight, &st);
-        f.count = dalloc<int>(ctx, n, &st);
+        f.head = dalloc<FlHead>(ctx, (size_t)n, &st);
+        f.more = dalloc<FlSlot>(ctx, (size_t)n * (kMaxInflight - kHeadSlots), &st);
         return f;
     };
     nd.fl_ra2i = mk_inflight(nra);
@@ -1327,7 +1324,7 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     const double NssA = std::max(prm->syn.Nss, 1);
     const double inbox = (double)(max_indeg + 8) * sizeof(Record) + sizeof(int);
     const double logs = 3.0 * spikes_per_neuron * sizeof(int2);
-    const double inflight = 2.0 * kMaxInflight * sizeof(int) + 2.0 * kMaxInflight * sizeof(double) + sizeof(int);
+    const double inflight = sizeof(FlHead) + (kMaxInflight - kHeadSlots) * sizeof(FlSlot);
     PoolCost c;
     const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
     c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 16 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +

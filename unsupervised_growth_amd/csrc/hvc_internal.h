@@ -91,12 +91,23 @@ struct Inbox {
     int cap;
 };
 
+// one in-flight spike of a fixed-weight push: spike step (-1: free slot), edge
+// cursor, delivery time of the cursor's edge, clock at the spike step
+struct FlSlot {
+    int step, cur;
+    double tnext, tspike;
+};
+constexpr int kHeadSlots = 5;
+// a neuron's count of held slots and its first slots in one 128-byte line, read
+// in one go when a window starts; the other slots beside it
+struct alignas(128) FlHead {
+    int count, pad;
+    FlSlot s[kHeadSlots];
+};
+static_assert(sizeof(FlHead) == 128, "one line per neuron");
 struct InFlight {
-    int* step;      // [n][kMaxInflight] spike step of a held slot, -1 free
-    int* cursor;    // [n][kMaxInflight]
-    double* tnext;  // [n][kMaxInflight] delivery time of the cursor's edge
-    double* tspike; // [n][kMaxInflight] clock at the spike step
-    int* count;     // [n] held slots (0: the slot table is not valid)
+    FlHead* head;   // [n]
+    FlSlot* more;   // [n][kMaxInflight - kHeadSlots]
 };
 
 struct NetDev {

@@ -52,6 +52,15 @@ __global__ void k_decay_table(const double* clock, const double* offsets, int nu
 // trial start: setToRest_afterEpoch (NGS:4591-4621), HH2_test::set_to_rest
 // (HH2_test.cpp:191-242), HHI_buffer::set_to_rest (HHI_buffer.cpp:178-200)
 // ---------------------------------------------------------------------------
+// no spike in flight: count 0, every slot free
+__device__ __forceinline__ void fl_clear(const InFlight& fl, int i)
+{
+    fl.head[i].count = 0;
+    for (int k = 0; k < kHeadSlots; ++k) fl.head[i].s[k].step = -1;
+    FlSlot* m = fl.more + (size_t)i * (kMaxInflight - kHeadSlots);
+    for (int k = 0; k < kMaxInflight - kHeadSlots; ++k) m[k].step = -1;
+}
+
 __global__ void k_reset(NetDev nd)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -71,8 +80,8 @@ __global__ void k_reset(NetDev nd)
         nd.post_pend[i] = 0;
         nd.last_pre[i] = -1.0e300;
         nd.st_count[i] = 0;
-        nd.fl_ra2i.count[i] = 0;
-        nd.fl_rr.count[i] = 0;
+        fl_clear(nd.fl_ra2i, i);
+        fl_clear(nd.fl_rr, i);
         nd.in_gi_ra.cnt[i] = 0;
         nd.in_ge_ra.cnt[i] = 0;
     }
@@ -88,7 +97,7 @@ __global__ void k_reset(NetDev nd)
             nd.i_noise[(size_t)nd.n_i + i] = i_init_noise(ps, init_dt);
         }
         nd.i_draw[i] = ps.draw;
-        nd.fl_i2ra.count[i] = 0;
+        fl_clear(nd.fl_i2ra, i);
         nd.in_ge_i.cnt[i] = 0;
     }
     if (i == 0) {
@@ -293,44 +302,45 @@ __device__ __forceinline__ double group_min(double x)
 // Fixed-weight delay queues of one presynaptic neuron (RA->I NGS:8971-8986 +
 // 9039-9062, I->RA NGS:9235-9284, chain-test RA->RA NGS:13222-13250), G lanes
 // per neuron (G = 1 or a quad).  Each in-flight spike holds one of the
-// neuron's kMaxInflight slots: spike step (-1 = free), edge cursor and the
-// delivery time of the cursor's edge (the table is valid while fl.count, the
-// number of held slots, is non-zero).  The row's edges are sorted by delay, so
-// the edges due by t1 are a prefix from the cursor; their pops become inbox
-// records.  The sync sums a window's records in the reference's order
-// whatever order they are written in, so lane r of the group takes edges
-// cursor + r, cursor + r + G, ... (the records' atomics of a spike overlap)
-// and the group agrees on the new cursor, the first edge not due.  New spikes
-// take the free slots in slot order.
+// neuron's kMaxInflight slots (FlSlot: spike step, edge cursor, delivery time
+// of the cursor's edge, clock at the spike step; free slots have step -1).
+// The held count and the first slots share one line, read in one go, so a
+// window's work starts one memory round trip after the kernel does.  The
+// row's edges are sorted by delay, so the edges due by t1 are a prefix from
+// the cursor; their pops become inbox records.  The sync sums a window's
+// records in the reference's order whatever order they are written in, so
+// lane r of the group takes edges cursor + r, cursor + r + G, ... (the records'
+// atomics of a spike overlap) and the group agrees on the new cursor, the
+// first edge not due.  New spikes take the free slots in slot order.
 template <int G>
 __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin,
                                            int row_end, const int* edge_ids, const int* tgt_arr, const double* w_arr,
-                                           const double* d_arr, const InFlight& fl, uint64_t new_mask, int t0, int t1,
-                                           const Inbox& ib)
+                                           const double* d_arr, const InFlight& fl, FlHead h,
+                                           uint64_t new_mask, int t0, int t1, const Inbox& ib)
 {
     const int deg = row_end - row_begin;
-    const int held = fl.count[pre];
+    FlHead* hp = fl.head + pre;
+    const int held = h.count;
     if (deg == 0 || (held == 0 && new_mask == 0)) return;
-    int* fstep = fl.step + (size_t)pre * kMaxInflight;
-    int* fcur = fl.cursor + (size_t)pre * kMaxInflight;
-    double* fnext = fl.tnext + (size_t)pre * kMaxInflight;
-    double* fcs = fl.tspike + (size_t)pre * kMaxInflight;
+    FlSlot* more = fl.more + (size_t)pre * (kMaxInflight - kHeadSlots);
+    auto slot = [&](int k) -> FlSlot* { return k < kHeadSlots ? &hp->s[k] : more + (k - kHeadSlots); };
     const double tend = wclk[t1 - t0 + 1];
-    // the held slots and which of them have an edge due in this window (the
-    // step, cursor and next-time tables are read together)
+    // the held slots and which of them have an edge due in this window
     unsigned held_mask = 0, due_mask = 0;
-    if (held > 0) {
-        const int4* q4 = (const int4*)fstep;
-        const double2* t2 = (const double2*)fnext;
 #pragma unroll
-        for (int k = 0; k < kMaxInflight / 4; ++k) {
-            const int4 v = q4[k];
-            const double2 ta = t2[2 * k], tb = t2[2 * k + 1];
-            held_mask |= ((v.x >= 0 ? 1u : 0u) | (v.y >= 0 ? 2u : 0u) | (v.z >= 0 ? 4u : 0u) | (v.w >= 0 ? 8u : 0u))
-                         << (4 * k);
-            due_mask |= ((v.x >= 0 && ta.x <= tend ? 1u : 0u) | (v.y >= 0 && ta.y <= tend ? 2u : 0u) |
-                         (v.z >= 0 && tb.x <= tend ? 4u : 0u) | (v.w >= 0 && tb.y <= tend ? 8u : 0u))
-                        << (4 * k);
+    for (int k = 0; k < kHeadSlots; ++k) {
+        if (h.s[k].step >= 0) {
+            held_mask |= 1u << k;
+            if (h.s[k].tnext <= tend) due_mask |= 1u << k;
+        }
+    }
+    if (held > __popc(held_mask)) {
+        for (int k = kHeadSlots; k < kMaxInflight; ++k) {
+            const FlSlot sl = more[k - kHeadSlots];
+            if (sl.step >= 0) {
+                held_mask |= 1u << k;
+                if (sl.tnext <= tend) due_mask |= 1u << k;
+            }
         }
     }
     // the first nnew free slots take the window's new spikes (in step order)
@@ -342,10 +352,6 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         free_mask &= ~low;
     }
     if (r == 0 && nnew > __popc(assigned)) atomicOr(nd.err, (unsigned)kErrInflight);
-    if (held == 0 && r == 0) {
-        // the table was not valid: mark the other slots free
-        for (unsigned m = free_mask; m; m &= m - 1) fstep[__ffs(m) - 1] = -1;
-    }
     int now_held = __popc(held_mask & ~due_mask);   // held slots with nothing due stay as they are
     for (unsigned m = due_mask | assigned; m; m &= m - 1) {
         const int k = __ffs(m) - 1;
@@ -359,9 +365,18 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
             c = 0;
             cs = wclk[s - t0 + 1];
         } else {
-            s = fstep[k];
-            c = fcur[k];
-            cs = fcs[k];
+            FlSlot sl;
+            if (k < kHeadSlots) {   // from the registers (no second round trip)
+                sl = h.s[0];
+#pragma unroll
+                for (int j = 1; j < kHeadSlots; ++j)
+                    if (k == j) sl = h.s[j];
+            } else {
+                sl = more[k - kHeadSlots];
+            }
+            s = sl.step;
+            c = sl.cur;
+            cs = sl.tspike;
         }
         int lo = s > t0 ? s : t0;
         int cc = c + r, stop = deg;
@@ -403,19 +418,15 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         const double tnew = group_min<G>(tstop);
         if (cnew < deg) ++now_held;
         if (r == 0) {
+            FlSlot* sp = slot(k);
             if (cnew >= deg) {
-                if (!fresh || held == 0) fstep[k] = -1;   // (a fresh slot of a valid table is still free)
+                if (!fresh) sp->step = -1;   // (a fresh slot is still free)
             } else {
-                if (fresh) {
-                    fstep[k] = s;
-                    fcs[k] = cs;
-                }
-                fcur[k] = cnew;
-                fnext[k] = tnew;
+                *sp = FlSlot{s, cnew, tnew, cs};
             }
         }
     }
-    if (r == 0) fl.count[pre] = now_held;
+    if (r == 0) hp->count = now_held;
 }
 
 // LTD against the target's post events not older than STDP_WINDOW (NGS:9111-9180)
@@ -677,10 +688,33 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
 template <int G>
 __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int b0, int b1)
 {
-    int b = blockIdx.x;
+    const int b = blockIdx.x, bs = blockDim.x, tid = threadIdx.x;
     __shared__ double wclk[kMaxWindowSteps + 1];
     extern __shared__ double heads_lds[];   // kRowHeads x blockDim: T, then target and edge ids
-    for (int k = threadIdx.x; k <= t1 - t0 + 1; k += blockDim.x) wclk[k] = nd.clock[t0 - 1 + k];
+    // the window's clock, and for a fixed-weight push its row, the window's
+    // spikes and the slot-table head: all issued before the block waits on the
+    // clock, so the push starts one memory round trip after the kernel does
+    const int nclk = t1 - t0 + 2;
+    const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
+    const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
+    const bool ra = b < b0 + b1;
+    const int q = (ra ? b - b0 : b - b0 - b1) * bs + tid;
+    const int pre = q / G, r = q % G;
+    const bool push = b >= b0 && pre < (ra ? nd.n_ra : nd.n_i);
+    const FixedEdges fe = ra ? nd.ra2i : nd.i2ra;
+    const InFlight fl = ra ? nd.fl_ra2i : nd.fl_i2ra;
+    int rb = 0, re = 0;
+    uint64_t spikes = 0;
+    FlHead h;
+    if (push) {
+        rb = fe.ptr[pre];
+        re = fe.ptr[pre + 1];
+        spikes = ra ? nd.soma_mask[pre] : nd.i_mask[pre];
+        h = fl.head[pre];
+    }
+    if (tid < nclk) wclk[tid] = c0;
+    if (tid + bs < nclk) wclk[tid + bs] = c1;
+    for (int k = tid + 2 * bs; k < nclk; k += bs) wclk[k] = nd.clock[t0 - 1 + k];
     __syncthreads();
     // the other post-event buffer was read by the previous window's STDP (in
     // this window's integration); the next window's integration fills it
@@ -696,22 +730,13 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
             rr_row(nd, wclk, i, t0, t1, hd);
         else
             push_fixed<1>(nd, wclk, i, 0, nd.rr_ptr[i], nd.rr_ptr[i] + nd.act_cnt[i], nd.chain_e, nd.rr_tgt, nd.w,
-                          nd.rr_d, nd.fl_rr, nd.soma_mask[i], t0, t1, nd.in_ge_ra);
+                          nd.rr_d, nd.fl_rr, nd.fl_rr.head[i], nd.soma_mask[i], t0, t1, nd.in_ge_ra);
         return;
     }
     // fixed-weight pushes: G lanes per presynaptic neuron
-    const bool ra = b < b0 + b1;
-    const int q = (ra ? b - b0 : b - b0 - b1) * blockDim.x + threadIdx.x;
-    const int pre = q / G, r = q % G;
-    if (ra) {
-        if (pre >= nd.n_ra) return;
-        push_fixed<G>(nd, wclk, pre, r, nd.ra2i.ptr[pre], nd.ra2i.ptr[pre + 1], nullptr, nd.ra2i.tgt, nd.ra2i.w,
-                      nd.ra2i.d, nd.fl_ra2i, nd.soma_mask[pre], t0, t1, nd.in_ge_i);
-    } else {
-        if (pre >= nd.n_i) return;
-        push_fixed<G>(nd, wclk, pre, r, nd.i2ra.ptr[pre], nd.i2ra.ptr[pre + 1], nullptr, nd.i2ra.tgt, nd.i2ra.w,
-                      nd.i2ra.d, nd.fl_i2ra, nd.i_mask[pre], t0, t1, nd.in_gi_ra);
-    }
+    if (push)
+        push_fixed<G>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1,
+                      ra ? nd.in_ge_i : nd.in_gi_ra);
 }
 
 // sync: apply the window's conductance updates (NGS:9309-9366).  Within a
