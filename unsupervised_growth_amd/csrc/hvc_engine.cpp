@@ -41,6 +41,7 @@ void launch_sync(hipStream_t st, const NetDev& nd, const Tuning& tu);
 void launch_stdp(hipStream_t st, const NetDev& nd, const StdpArgs& sa);
 void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int gen_ra, int gen_i);
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta);
+void launch_rr_pos(hipStream_t st, const NetDev& nd, RrPos* pos, int E);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
 void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out);
@@ -112,6 +113,8 @@ struct hvc_ctx {
     int* d_rr_tgt = nullptr;
     double* d_rr_d = nullptr;
     int* d_perm = nullptr;
+    RrPos* d_rr_pos = nullptr;     // rebuilt from d_perm before the next trial when rr_pos_stale
+    bool rr_pos_stale = true;
     int *d_a_ptr = nullptr, *d_a_tgt = nullptr, *d_b_ptr = nullptr, *d_b_tgt = nullptr;
     double *d_a_w = nullptr, *d_a_d = nullptr, *d_b_w = nullptr, *d_b_d = nullptr;
     Tuning tuning;
@@ -610,7 +613,9 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     ctx->d_rr_tgt = dalloc<int>(ctx, E, &st);
     ctx->d_rr_d = dalloc<double>(ctx, E, &st);
     ctx->d_perm = dalloc<int>(ctx, E, &st);
+    ctx->d_rr_pos = dalloc<RrPos>(ctx, E, &st);
     nd.rr_ptr = ctx->d_rr_ptr; nd.rr_tgt = ctx->d_rr_tgt; nd.rr_d = ctx->d_rr_d; nd.rr_perm = ctx->d_perm;
+    nd.rr_pos = ctx->d_rr_pos;
     nd.w = dalloc<double>(ctx, E, &st);
     nd.act = dalloc<uint8_t>(ctx, E, &st);
     nd.sup = dalloc<uint8_t>(ctx, E, &st);
@@ -743,6 +748,7 @@ static hvc_status upload_impl(hvc_ctx* ctx, const hvc_network* net, int n_nets, 
         HVC_TRY(hipMemcpyAsync(ctx->d_rr_tgt, net->rr_tgt, sizeof(int) * E, hipMemcpyHostToDevice, s));
         HVC_TRY(hipMemcpyAsync(ctx->d_rr_d, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
         HVC_TRY(hipMemcpyAsync(ctx->d_perm, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+        ctx->rr_pos_stale = true;
     }
     HVC_TRY(hipMemcpyAsync(ctx->d_a_ptr, net->ra2i_ptr, sizeof(int) * (nra + 1), hipMemcpyHostToDevice, s));
     if (!a_tgt.empty()) {
@@ -994,6 +1000,14 @@ extern "C" hvc_status hvc_trial_spikes(hvc_ctx* ctx, int kind, int* counts, doub
 // One trial of every network in the context (trials[b] for network b); the
 // networks advance together, one launch per kernel and coupling window.
 // noise events of the chunk that starts at window w (k_noise_events)
+// the candidate edges by row position (RrPos) after the order tables changed
+static void refresh_rr_pos(hvc_ctx* ctx, const NetDev& nd)
+{
+    if (!ctx->rr_pos_stale) return;
+    launch_rr_pos(ctx->stream, nd, ctx->d_rr_pos, ctx->E);
+    ctx->rr_pos_stale = false;
+}
+
 static void noise_chunk(hvc_ctx* ctx, const NetDev& nd, int w)
 {
     const int gen_ra = ctx->prm.ra_model == 1 && ctx->prm.ra_poisson, gen_i = ctx->prm.i_poisson != 0;
@@ -1040,6 +1054,7 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, ctx->h_net_trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_first, ctx->h_net_first.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_growth, ctx->h_net_growth.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    refresh_rr_pos(ctx, nd);
     launch_reset(s, nd, ctx->E);
     if (any_chain) launch_chain_edges(s, nd);
     const int nw = (int)ctx->windows.size();
@@ -1221,6 +1236,7 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
     HVC_TRY(hipMemcpyAsync(ctx->d_net_growth, zeros.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_first, zeros.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
+    refresh_rr_pos(ctx, nd);
     launch_reset(s, nd, ctx->E);
     launch_chain_edges(s, nd);
     const int nw = std::min(windows, (int)ctx->windows.size());
@@ -1330,7 +1346,7 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 16 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
            4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise;
     c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise;
-    c.rr = 4 + 8 + 4 + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;
+    c.rr = 4 + 8 + 4 + sizeof(RrPos) + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;
     c.a2i = 4 + 8 + 8;
     c.i2ra = 4 + 8 + 8;
     c.net = 2 * 4 + 2 * 4 + 4 * 4 + 3 * 4;
@@ -1462,6 +1478,7 @@ extern "C" hvc_status hvc_pool_admit(hvc_ctx* ctx, const hvc_network* net, uint3
         HVC_TRY(hipMemcpyAsync(ctx->d_rr_tgt + eo, rt.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
         HVC_TRY(hipMemcpyAsync(ctx->d_rr_d + eo, net->rr_d, sizeof(double) * E, hipMemcpyHostToDevice, s));
         HVC_TRY(hipMemcpyAsync(ctx->d_perm + eo, perm.data(), sizeof(int) * E, hipMemcpyHostToDevice, s));
+        ctx->rr_pos_stale = true;
     }
     if (A > 0) {
         HVC_TRY(hipMemcpyAsync(ctx->d_a_tgt + ao, a_tgt.data(), sizeof(int) * A, hipMemcpyHostToDevice, s));
@@ -1571,6 +1588,7 @@ extern "C" hvc_status hvc_pool_release(hvc_ctx* ctx, int handle)
         };
         for (const Mv& m : mv)
             if ((r = move_down(ctx, m.base, m.esz, m.from, m.to, m.count, m.reloc)) != HVC_OK) return r;
+        ctx->rr_pos_stale = true;
     }
     // host mirror of the candidate rows
     const int ro_g = ctx->ra_off[p];

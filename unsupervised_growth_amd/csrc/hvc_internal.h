@@ -77,6 +77,13 @@ struct Record {
     double w;
 };
 
+// a candidate edge at one position of its row's (delay, target) order: the
+// edge, its target and its delay, read in one load
+struct RrPos {
+    int e, tgt;
+    double d;
+};
+
 // Fixed-weight delivery edges sorted by delay within each presynaptic row.
 struct FixedEdges {
     const int* ptr;
@@ -178,6 +185,7 @@ struct NetDev {
     const int* rr_tgt;
     const double* rr_d;
     const int* rr_perm;          // per row: candidate edges sorted by (delay, target)
+    const RrPos* rr_pos;         // [E] rr_perm's edges with their targets and delays
     double* w;
     uint8_t* act;
     uint8_t* sup;

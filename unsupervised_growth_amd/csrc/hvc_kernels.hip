@@ -429,24 +429,8 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
     if (r == 0) hp->count = now_held;
 }
 
-// LTD against the target's post events not older than STDP_WINDOW (NGS:9111-9180)
-__device__ void ltd_on_delivery(const NetDev& nd, int i, int e, int tgt, double T, double now)
-{
-    int pc = nd.post_cnt[tgt];
-    if (pc == 0) return;
-    int first = pc > kPostRing ? pc - kPostRing : 0;
-    const double* ring = nd.post_ring + (size_t)tgt * kPostRing;
-    double lim = now - 100.0;
-    int k = first;
-    while (k < pc && ring[k % kPostRing] < lim) ++k;
-    for (; k < pc; ++k) {
-        double dt = ring[k % kPostRing] - T;
-        nd.w[e] = ltd(nd.syn, nd.w[e], dt, row_warn(nd, i));
-        update_synapse(nd, i, e);
-    }
-}
-
-// next item of a stream: (T, target, edge); returns false when exhausted
+// next item of stream sidx of row i from the row's stream state in memory:
+// (T, target, edge); returns false when exhausted
 __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, double& T, int& tgt, int& e)
 {
     int* cur = nd.st_cursor + (size_t)i * kMaxStreams + sidx;
@@ -456,13 +440,13 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
     const double cs = nd.clock[s];
     int c = *cur;
     if (mode == 0) {
-        int base = nd.rr_ptr[i];
+        const int base = nd.rr_ptr[i];
         while (c < n) {
-            int ee = nd.rr_perm[base + c];
-            if (nd.rr_tgt[ee] != i) {
-                e = ee;
-                tgt = nd.rr_tgt[ee];
-                T = cs + nd.rr_d[ee];
+            const RrPos p = nd.rr_pos[base + c];
+            if (p.tgt != i) {
+                e = p.e;
+                tgt = p.tgt;
+                T = cs + p.d;
                 *cur = c;
                 return true;
             }
@@ -480,49 +464,201 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
 
 constexpr int kQuadPushNeurons = 65536;   // neurons per context up to which the pushes take a quad per neuron
 
-// heads (T, target, edge) of a row's first kRowHeads streams, one column of
-// the block's LDS per thread
+// A row's first kRowHeads streams while a window drains them, one column of the
+// block's LDS per thread: the head item (T, target, edge) and the stream's
+// cursor, length, mode and spike clock.  The cursors go back to memory when
+// the row leaves the cached path.
 constexpr int kRowHeads = 8;
+constexpr int kHeadDoubles = 2;   // T, spike clock
+constexpr int kHeadInts = 5;      // target, edge, cursor, length, mode
 struct StreamHeads {
     double* t_;
-    int* tg_;
-    int* e_;
+    int* i_;
     int stride;
     __device__ __forceinline__ double T(int q) const { return t_[q * stride]; }
-    __device__ __forceinline__ int tg(int q) const { return tg_[q * stride]; }
-    __device__ __forceinline__ int e(int q) const { return e_[q * stride]; }
+    __device__ __forceinline__ double cs(int q) const { return t_[(kRowHeads + q) * stride]; }
+    __device__ __forceinline__ int tg(int q) const { return i_[q * stride]; }
+    __device__ __forceinline__ int e(int q) const { return i_[(kRowHeads + q) * stride]; }
+    __device__ __forceinline__ int cur(int q) const { return i_[(2 * kRowHeads + q) * stride]; }
+    __device__ __forceinline__ int n(int q) const { return i_[(3 * kRowHeads + q) * stride]; }
+    __device__ __forceinline__ int mode(int q) const { return i_[(4 * kRowHeads + q) * stride]; }
     __device__ __forceinline__ void set(int q, double T, int tg, int e)
     {
         t_[q * stride] = T;
-        tg_[q * stride] = tg;
-        e_[q * stride] = e;
+        i_[q * stride] = tg;
+        i_[(kRowHeads + q) * stride] = e;
+    }
+    __device__ __forceinline__ void set_cur(int q, int c) const { i_[(2 * kRowHeads + q) * stride] = c; }
+    __device__ __forceinline__ void open(int q, double cs, int c, int n, int mode) const
+    {
+        t_[(kRowHeads + q) * stride] = cs;
+        i_[(2 * kRowHeads + q) * stride] = c;
+        i_[(3 * kRowHeads + q) * stride] = n;
+        i_[(4 * kRowHeads + q) * stride] = mode;
     }
 };
 
-// one drained item of row i: edge e to tgt, delivery time T, at step t
-// (NGS:9080-9191)
-__device__ __forceinline__ void rr_item(const NetDev& nd, int i, int e, int tgt, double T, int t, double now, int Nss)
+// Head of cached stream q at cursor c (c >= the stream's cursor): the first
+// item at or after c, self-edges skipped (mode 0).  The edge record p of
+// position c may already be loaded (have_p).
+__device__ __forceinline__ bool head_find(const NetDev& nd, int i, int base, int q, const StreamHeads& hd, int c,
+                                          RrPos p, bool have_p, double& T, int& tgt, int& e)
 {
-    // conductance of active synapses (NGS:9080-9084)
-    if (nd.act[e] == 1) inbox_put(nd.in_ge_ra, tgt, t, i, nd.w[e], nd.err);
-    // new presynaptic event (NGS:9091-9102)
-    int pc = nd.pre_cnt[e];
-    double* ring = nd.pre_ring + (size_t)e * kPreRing;
-    bool new_pre = false;
-    if (pc == 0 || T > ring[(pc - 1) % kPreRing] + 30.0) {
-        ring[pc % kPreRing] = T;
-        nd.pre_cnt[e] = pc + 1;
-        if (T > nd.last_pre[i]) nd.last_pre[i] = T;
-        new_pre = true;
+    const int n = hd.n(q);
+    const double cs = hd.cs(q);
+    if (hd.mode(q) == 0) {
+        while (c < n) {
+            if (!have_p) p = nd.rr_pos[base + c];
+            have_p = false;
+            if (p.tgt != i) {
+                e = p.e;
+                tgt = p.tgt;
+                T = cs + p.d;
+                hd.set_cur(q, c);
+                return true;
+            }
+            ++c;
+        }
+        hd.set_cur(q, c);
+        return false;
     }
-    if (new_pre) {
-        if (nd.sup_cnt[i] == Nss) {
-            if (nd.sup[e] == 1) ltd_on_delivery(nd, i, e, tgt, T, now);
-            if (nd.sup_cnt[i] < Nss) update_row(nd, i);
-        } else {
-            ltd_on_delivery(nd, i, e, tgt, T, now);
+    hd.set_cur(q, c);
+    if (c >= n) return false;
+    e = have_p ? p.e : nd.st_items[((size_t)i * kMaxStreams + q) * nd.syn.Nss + c];
+    tgt = nd.rr_tgt[e];
+    T = cs + nd.rr_d[e];
+    return true;
+}
+
+// the cached cursors back to the row's stream state in memory
+__device__ __forceinline__ void heads_store(const NetDev& nd, int i, int sc, const StreamHeads& hd)
+{
+    int* st_cur = nd.st_cursor + (size_t)i * kMaxStreams;
+    for (int q = 0; q < sc; ++q) st_cur[q] = hd.cur(q);
+}
+
+// What one drained item reads of its edge, loaded before any of its writes:
+// the edge's state and pre-event count, and the target's post-event count (the
+// delivery writes no post event).
+struct ItemIn {
+    int pc, post_n;
+    uint8_t act, sup;
+    double w;
+};
+__device__ __forceinline__ ItemIn item_load(const NetDev& nd, int e, int tgt)
+{
+    ItemIn in;
+    in.act = nd.act[e];
+    in.sup = nd.sup[e];
+    in.w = nd.w[e];
+    in.pc = nd.pre_cnt[e];
+    in.post_n = nd.post_cnt[tgt];
+    return in;
+}
+
+// the row's synapse counts and remodeled flag, kept in registers while the row
+// drains (only update_synapse changes them here)
+struct RowState {
+    int scnt, acnt, rem;
+    __device__ __forceinline__ void load(const NetDev& nd, int i)
+    {
+        scnt = nd.sup_cnt[i];
+        acnt = nd.act_cnt[i];
+        rem = nd.remodeled[i];
+    }
+};
+
+// update_synapse (NGS:16546-16592) of edge e to target j with the edge's and
+// the row's state in registers; the lists change in memory, in the same order
+struct SynCache {
+    int act, sup, rem, scnt, acnt;
+};
+__device__ __forceinline__ void update_synapse_c(const NetDev& nd, int i, int j, double w, SynCache& c)
+{
+    const hvc_synaptic_params& p = nd.syn;
+    int* alist = nd.act_list + nd.rr_ptr[i];
+    int* slist = nd.sup_list + (size_t)i * p.Nss;
+    if ((w >= p.ACTIVATION_THRESHOLD) && (c.act == 0) && (c.rem == 0)) {
+        c.act = 1;
+        alist[c.acnt++] = j;
+    }
+    if ((w >= p.SUPERSYNAPSE_THRESHOLD) && (c.sup == 0) && (c.scnt < p.Nss)) {
+        c.sup = 1;
+        slist[c.scnt++] = j;
+    }
+    if ((w < p.SUPERSYNAPSE_THRESHOLD) && (c.sup == 1)) {
+        c.sup = 0;
+        c.rem = 0;
+        if (!list_erase(slist, &c.scnt, j)) atomicAdd(&row_warn(nd, i)[2], 1);
+    }
+    if ((w < p.ACTIVATION_THRESHOLD) && (c.act == 1)) {
+        c.act = 0;
+        if (!list_erase(alist, &c.acnt, j)) atomicAdd(&row_warn(nd, i)[2], 1);
+    }
+}
+
+// LTD of edge e against the target's post events not older than STDP_WINDOW
+// (NGS:9111-9180), each step followed by update_synapse.  The ring's next
+// entry is loaded a step ahead; the edge's and the row's synapse state stay in
+// registers through the steps and are written back once (rs follows them).
+__device__ __forceinline__ void ltd_on_delivery(const NetDev& nd, int i, int e, int tgt, double T, double now,
+                                                const ItemIn& in, RowState& rs)
+{
+    const int pc = in.post_n;
+    if (pc == 0) return;
+    const int first = pc > kPostRing ? pc - kPostRing : 0;
+    const double* ring = nd.post_ring + (size_t)tgt * kPostRing;
+    const double lim = now - 100.0;
+    SynCache c{in.act, in.sup, rs.rem, rs.scnt, rs.acnt};
+    double w = in.w;
+    bool on = false;   // from the first event not older than the window on
+    double vn = ring[first % kPostRing];   // the next event, loaded one step ahead
+#pragma unroll 1
+    for (int j = 0; j < pc - first; ++j) {
+        const double v = vn;
+        if (j + 1 < pc - first) vn = ring[(first + j + 1) % kPostRing];
+        on = on || !(v < lim);
+        if (on) {
+            w = ltd(nd.syn, w, v - T, row_warn(nd, i));
+            update_synapse_c(nd, i, tgt, w, c);
         }
     }
+    if (!on) return;
+    nd.w[e] = w;
+    if (c.act != in.act) nd.act[e] = (uint8_t)c.act;
+    if (c.sup != in.sup) nd.sup[e] = (uint8_t)c.sup;
+    if (c.rem != rs.rem) nd.remodeled[i] = c.rem;
+    if (c.scnt != rs.scnt) nd.sup_cnt[i] = c.scnt;
+    if (c.acnt != rs.acnt) nd.act_cnt[i] = c.acnt;
+    rs.rem = c.rem;
+    rs.scnt = c.scnt;
+    rs.acnt = c.acnt;
+}
+
+// one drained item of row i: edge e to tgt, delivery time T, at step t
+// (NGS:9080-9191); lastp is the row's latest pre-event time
+__device__ __forceinline__ void rr_item(const NetDev& nd, int i, int e, int tgt, double T, int t, double now, int Nss,
+                                        const ItemIn& in, RowState& rs, double& lastp)
+{
+    double* ring = nd.pre_ring + (size_t)e * kPreRing;
+    const double last = in.pc > 0 ? ring[(in.pc - 1) % kPreRing] : 0.0;
+    // new presynaptic event (NGS:9091-9102)
+    if (in.pc == 0 || T > last + 30.0) {
+        ring[in.pc % kPreRing] = T;
+        nd.pre_cnt[e] = in.pc + 1;
+        if (T > lastp) lastp = T;
+        // a saturated row takes LTD on supersynapses only, and reverts its
+        // remodeling when one of them falls below the threshold
+        const bool saturated = rs.scnt == Nss;
+        if (!saturated || in.sup == 1) ltd_on_delivery(nd, i, e, tgt, T, now, in, rs);
+        if (saturated && rs.scnt < Nss) {
+            update_row(nd, i);
+            rs.load(nd, i);
+        }
+    }
+    // conductance of active synapses with the weight before this item's LTD
+    // (NGS:9080-9084; the inbox is read only at the sync)
+    if (in.act == 1) inbox_put(nd.in_ge_ra, tgt, t, i, in.w, nd.err);
 }
 
 // Growth-mode RA->RA row of presynaptic neuron i for the window t0..t1
@@ -536,13 +672,19 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
     // at a step before it, with no spike to enqueue, the drain and the
     // exhausted-stream pass change nothing, so the step is skipped
     double next_T = sc > 0 ? nd.st_next[i] : HUGE_VAL;
+    if (mask == 0 && !(wclk[t1 - t0 + 1] >= next_T)) return;
     int* st_step = nd.st_step + (size_t)i * kMaxStreams;
     int* st_mode = nd.st_mode + (size_t)i * kMaxStreams;
     int* st_cur = nd.st_cursor + (size_t)i * kMaxStreams;
     int* st_n = nd.st_n + (size_t)i * kMaxStreams;
-    // heads of the first kRowHeads streams kept in LDS (hd; T = +inf: exhausted):
-    // a drain step then peeks only the stream it consumed.  Rows with more
-    // streams take the uncached path.
+    const int base = nd.rr_ptr[i], deg = nd.rr_ptr[i + 1] - base;
+    double lastp = nd.last_pre[i];
+    const double lastp0 = lastp;
+    RowState rs;
+    rs.load(nd, i);
+    // the first kRowHeads streams are drained from LDS (hd; T = +inf:
+    // exhausted): a drain step then loads only the consumed stream's next item.
+    // Rows with more streams take the path through memory.
     bool cached = false;
     for (int t = t0; t <= t1; ++t) {
         const bool spike = (mask >> (t - t0)) & 1ull;
@@ -555,12 +697,13 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
             } else {
                 st_step[sc] = t;
                 st_cur[sc] = 0;
-                if (nd.sup_cnt[i] == Nss) {
+                int mode, n;
+                if (rs.scnt == Nss) {
                     // saturated: deliver to supersynaptic targets only, queue sorted by (T, target)
-                    st_mode[sc] = 1;
-                    int n = nd.sup_cnt[i];
+                    mode = 1;
+                    n = rs.scnt;
                     int* items = nd.st_items + ((size_t)i * kMaxStreams + sc) * Nss;
-                    const double cs = nd.clock[t];
+                    const double cs = wclk[t - t0 + 1];
                     const int* slist = nd.sup_list + (size_t)i * Nss;
                     for (int k = 0; k < n; ++k) {
                         int e = edge_of(nd, i, slist[k]);
@@ -579,19 +722,25 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
                         }
                         items[q] = e;
                     }
-                    st_n[sc] = n;
                 } else {
-                    st_mode[sc] = 0;
-                    st_n[sc] = nd.rr_ptr[i + 1] - nd.rr_ptr[i];
+                    mode = 0;
+                    n = deg;
                 }
+                st_mode[sc] = mode;
+                st_n[sc] = n;
                 ++sc;
                 if (cached) {
                     if (sc > kRowHeads) {
+                        heads_store(nd, i, sc - 1, hd);
                         cached = false;
                     } else {
+                        hd.open(sc - 1, wclk[t - t0 + 1], 0, n, mode);
                         double T;
                         int tg, e;
-                        if (!stream_peek(nd, i, sc - 1, T, tg, e)) { T = HUGE_VAL; exhausted = true; }
+                        if (!head_find(nd, i, base, sc - 1, hd, 0, RrPos{}, false, T, tg, e)) {
+                            T = HUGE_VAL;
+                            exhausted = true;
+                        }
                         hd.set(sc - 1, T, tg, e);
                     }
                 }
@@ -599,10 +748,14 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
         }
         nd.st_count[i] = sc;
         if (!cached && sc <= kRowHeads) {
+            for (int q = 0; q < sc; ++q) hd.open(q, nd.clock[st_step[q]], st_cur[q], st_n[q], st_mode[q]);
             for (int q = 0; q < sc; ++q) {
                 double T;
                 int tg, e;
-                if (!stream_peek(nd, i, q, T, tg, e)) { T = HUGE_VAL; exhausted = true; }
+                if (!head_find(nd, i, base, q, hd, hd.cur(q), RrPos{}, false, T, tg, e)) {
+                    T = HUGE_VAL;
+                    exhausted = true;
+                }
                 hd.set(q, T, tg, e);
             }
             cached = true;
@@ -621,13 +774,25 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
                     }
                 }
                 if (best < 0) break;
-                st_cur[best] += 1;
-                // the consumed stream's next head, loaded while the item is processed
+                // the item's reads and the consumed stream's next item, loaded together
+                const ItemIn in = item_load(nd, be, btg);
+                const int c = hd.cur(best) + 1;
+                const bool more = c < hd.n(best);
+                RrPos p{};
+                if (more) {
+                    if (hd.mode(best) == 0)
+                        p = nd.rr_pos[base + c];
+                    else
+                        p.e = nd.st_items[((size_t)i * kMaxStreams + best) * Nss + c];
+                }
                 double nT;
                 int ntg, ne;
-                if (!stream_peek(nd, i, best, nT, ntg, ne)) { nT = HUGE_VAL; exhausted = true; }
+                if (!head_find(nd, i, base, best, hd, c, p, more, nT, ntg, ne)) {
+                    nT = HUGE_VAL;
+                    exhausted = true;
+                }
                 hd.set(best, nT, ntg, ne);
-                rr_item(nd, i, be, btg, bT, t, now, Nss);
+                rr_item(nd, i, be, btg, bT, t, now, Nss, in, rs, lastp);
                 continue;
             }
             int best = -1;
@@ -644,7 +809,7 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
             }
             if (best < 0) break;
             st_cur[best] += 1;
-            rr_item(nd, i, be, btg, bT, t, now, Nss);
+            rr_item(nd, i, be, btg, bT, t, now, Nss, item_load(nd, be, btg), rs, lastp);
         }
         if (cached) {
             // no stream ran out: the table is unchanged, next_T from the heads
@@ -653,6 +818,7 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
                 for (int q = 0; q < sc; ++q) next_T = hd.T(q) < next_T ? hd.T(q) : next_T;
                 continue;
             }
+            heads_store(nd, i, sc, hd);
             cached = false;   // the pass below renumbers the streams
         }
         // drop exhausted streams, keeping order
@@ -677,6 +843,8 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
         sc = w;
         nd.st_count[i] = sc;
     }
+    if (cached) heads_store(nd, i, sc, hd);
+    if (lastp != lastp0) nd.last_pre[i] = lastp;
     nd.st_next[i] = next_T;
 }
 
@@ -685,12 +853,16 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
 //   role 0: RA->RA rows (growth) or chain-test RA->RA push
 //   role 1: RA->I push        role 2: I->RA push
 // ---------------------------------------------------------------------------
+// two waves per SIMD: the register budget of the fixed-weight pushes and the
+// row drain (its LTD steps) is held to 256 so that a launch's ~2 waves per SIMD
+// are resident at once
 template <int G>
-__global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int b0, int b1)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deliver(NetDev nd, int t0, int t1,
+                                                                                       int b0, int b1)
 {
     const int b = blockIdx.x, bs = blockDim.x, tid = threadIdx.x;
     __shared__ double wclk[kMaxWindowSteps + 1];
-    extern __shared__ double heads_lds[];   // kRowHeads x blockDim: T, then target and edge ids
+    extern __shared__ double heads_lds[];   // StreamHeads of the block's rows
     // the window's clock, and for a fixed-weight push its row, the window's
     // spikes and the slot-table head: all issued before the block waits on the
     // clock, so the push starts one memory round trip after the kernel does
@@ -724,8 +896,7 @@ __global__ __launch_bounds__(256) void k_deliver(NetDev nd, int t0, int t1, int 
         const int i = b * blockDim.x + threadIdx.x;
         if (i >= nd.n_ra) return;
         const int bs = blockDim.x;
-        const StreamHeads hd{heads_lds + threadIdx.x, (int*)(heads_lds + kRowHeads * bs) + threadIdx.x,
-                             (int*)(heads_lds + kRowHeads * bs) + kRowHeads * bs + threadIdx.x, bs};
+        const StreamHeads hd{heads_lds + tid, (int*)(heads_lds + kHeadDoubles * kRowHeads * bs) + tid, bs};
         if (nd.net_growth[nd.ra_net[i]])
             rr_row(nd, wclk, i, t0, t1, hd);
         else
@@ -848,7 +1019,7 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
 void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuning& tu)
 {
     const int bs = tu.deliver_block;
-    const size_t lds = (size_t)kRowHeads * bs * (sizeof(double) + 2 * sizeof(int));
+    const size_t lds = (size_t)kRowHeads * bs * (kHeadDoubles * sizeof(double) + kHeadInts * sizeof(int));
     // a quad per presynaptic neuron shortens a neuron's chain of inbox atomics
     // when few waves are in flight (one network); with many networks the extra
     // waves cost more than they save, and one lane per neuron is faster
@@ -870,6 +1041,21 @@ __global__ void k_add_i32(int* a, long long n, int delta)
 void launch_add_i32(hipStream_t st, int* a, long long n, int delta)
 {
     if (n > 0) k_add_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(a, n, delta);
+}
+
+// pos[k] = (rr_perm[k], its target, its delay) for the positions [0, E)
+__global__ void k_rr_pos(NetDev nd, RrPos* pos, int E)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < E) {
+        const int e = nd.rr_perm[k];
+        pos[k] = RrPos{e, nd.rr_tgt[e], nd.rr_d[e]};
+    }
+}
+
+void launch_rr_pos(hipStream_t st, const NetDev& nd, RrPos* pos, int E)
+{
+    if (E > 0) k_rr_pos<<<blocks_for(E, 256), 256, 0, st>>>(nd, pos, E);
 }
 
 void launch_noise_events(hipStream_t st, const NetDev& nd, int tb, int te, int gen_ra, int gen_i)
