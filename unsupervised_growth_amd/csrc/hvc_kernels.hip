@@ -856,7 +856,8 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
 // two waves per SIMD: the register budget of the fixed-weight pushes and the
 // row drain (its LTD steps) is held to 256 so that a launch's ~2 waves per SIMD
 // are resident at once
-template <int G>
+// GA / GB lanes per presynaptic neuron of the RA->I / I->RA pushes
+template <int GA, int GB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deliver(NetDev nd, int t0, int t1,
                                                                                        int b0, int b1)
 {
@@ -871,6 +872,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
     const bool ra = b < b0 + b1;
     const int q = (ra ? b - b0 : b - b0 - b1) * bs + tid;
+    const int G = ra ? GA : GB;
     const int pre = q / G, r = q % G;
     const bool push = b >= b0 && pre < (ra ? nd.n_ra : nd.n_i);
     const FixedEdges fe = ra ? nd.ra2i : nd.i2ra;
@@ -905,9 +907,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         return;
     }
     // fixed-weight pushes: G lanes per presynaptic neuron
-    if (push)
-        push_fixed<G>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1,
-                      ra ? nd.in_ge_i : nd.in_gi_ra);
+    if (!push) return;
+    if constexpr (GA == GB) {
+        push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1,
+                       ra ? nd.in_ge_i : nd.in_gi_ra);
+    } else {
+        if (ra)
+            push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_ge_i);
+        else
+            push_fixed<GB>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_gi_ra);
+    }
 }
 
 // sync: apply the window's conductance updates (NGS:9309-9366).  Within a
@@ -1020,15 +1029,16 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuni
 {
     const int bs = tu.deliver_block;
     const size_t lds = (size_t)kRowHeads * bs * (kHeadDoubles * sizeof(double) + kHeadInts * sizeof(int));
-    // a quad per presynaptic neuron shortens a neuron's chain of inbox atomics
-    // when few waves are in flight (one network); with many networks the extra
-    // waves cost more than they save, and one lane per neuron is faster
+    // a quad per HVC-I neuron shortens the chain of inbox atomics of its ~30
+    // I->RA edges when few waves are in flight (one network); an HVC-RA neuron's
+    // few RA->I edges take one lane.  With many networks the extra waves cost
+    // more than they save, and every push takes one lane.
     if (nd.n_ra + nd.n_i <= kQuadPushNeurons) {
-        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(4 * nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
-        k_deliver<4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
+        k_deliver<1, 4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
     } else {
         const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(nd.n_ra, bs), b2 = blocks_for(nd.n_i, bs);
-        k_deliver<1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        k_deliver<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
     }
 }
 
