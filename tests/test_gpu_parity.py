@@ -96,6 +96,36 @@ def test_growth_trials_bit_exact(rr, ra_model, ra_lanes, i_lanes):
         assert ora.synapses()["sup_cnt"].sum() > 0, "test network should reach supersynapses"
 
 
+def test_growth_rows_beyond_the_stream_cache_bit_exact():
+    """RA->RA rows with more spikes in flight than the drain keeps in LDS (8):
+    tonic firing (mean somatic drive 0.6) with 100-200 ms RA->RA delays holds
+    9-14 spikes per row in flight, so those rows drain through their stream
+    state in memory and drop exhausted streams there."""
+    from unsupervised_growth_amd.network import NoiseParams, random_network
+    net = random_network(30, 8, n_train=6, p_ra2i=0.3, p_i2ra=0.3, gie_max=1.0, gei_max=0.5, spread_max=20.0,
+                         seed=5, rr=8, rr_delays=("uniform", 100.0, 200.0))
+    noise = NoiseParams(white_noise_mean_soma=0.6)
+    trial_ms = 400.0
+    print("growth many streams n_ra=%d n_i=%d trial_ms=%g" % (net.n_ra, net.n_i, trial_ms), flush=True)
+    ora = OracleSim(net, trial_ms=trial_ms, noise=noise)
+    most = 0
+    with _engine(trial_ms=trial_ms, noise=noise) as eng:
+        eng.upload(net)
+        for tr in range(2):
+            kw = dict(mode=0, training=True, trial_index=tr, first_trial=(tr == 0), mean_injection_time=100.0)
+            o = ora.run_trial(**kw)
+            r = eng.run_trial(want_state=True, **kw)
+            _compare_trial(r, o)
+            _compare_synapses(eng.download_synapses(), ora.synapses())
+            for i, sp in enumerate(o["soma"]):
+                sp = np.sort(np.asarray(sp))
+                dmax = net.rr_d[net.rr_ptr[i]:net.rr_ptr[i + 1]].max()
+                for t in sp:
+                    most = max(most, int(np.sum((sp >= t) & (sp <= t + dmax))))
+    print("  most spikes of a row within its longest delay: %d" % most, flush=True)
+    assert 8 < most <= 16
+
+
 def test_chain_trial_bit_exact():
     net = _small_net()
     trial_ms = 300.0
