@@ -809,23 +809,32 @@ __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role
                                            real e2, real e5, real& kx, real& kw)
 {
     HVC_FP_FAST
-    real n4 = n1 * n1 * n1 * n1;
-    real m3 = m1 * m1 * m1;
 #ifdef HVC_FP32
+    // the same currents as one chain of multiply-adds: the two potassium
+    // conductances share (v - E_K), the leak's reversal term is a constant
     // (the reference's closing "+ 100000 * 0 / A" only turns -0 into +0)
-    real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
-                - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei);
+    const real n2 = n1 * n1;
+    const real gk = __builtin_fmaf(kI_gKdr, n2 * n2, kI_gKHT * w1);
+    real t = __builtin_fmaf(kI_gL, v, -(kI_gL * kI_El));
+    t = __builtin_fmaf(kI_gNa * h1 * (m1 * m1 * m1), v - kI_Ena, t);
+    t = __builtin_fmaf(gk, v - kI_Ek, t);
+    t = __builtin_fmaf(s.Ge * e2, v, t);
+    t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
+    const real kV = -t;
     real a = gate_rate_k(gate_k(ga), v);
     real b = gate_rate_k(gate_k(gb), v);
     kw = a - w1;   // a = wInf in lane 0
+    const real kg = __builtin_fmaf(-(a + b), xo, a);   // a (1 - x) - b x
 #else
+    real n4 = n1 * n1 * n1 * n1;
+    real m3 = m1 * m1 * m1;
     real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
                 - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
     real a = gate_rate<EXACT>(bad, ga, v);
     real b = gate_rate<EXACT>(bad, gb, v);
     kw = role == 0 ? (a - w1) : HR(0.0);   // a = wInf in lane 0
-#endif
     real kg = a * (1 - xo) - b * xo;
+#endif
     kx = role == 0 ? kV : kg;
 }
 
