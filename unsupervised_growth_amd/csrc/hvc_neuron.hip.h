@@ -194,6 +194,33 @@ __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, r
     HVC_FP_FAST
     RAK k;
     real m = g.m;
+#ifdef HVC_FP32
+    // one chain of multiply-adds per compartment: the potassium conductances
+    // share (vs - E_K), the coupling terms use RN(100000 / R) as one factor,
+    // c / (1 + 6 / [Ca]) is c [Ca] / ([Ca] + 6), and the calcium current is
+    // shared by the dendrite and the calcium equation
+    const real n2 = n * n;
+    real gk = kRA_GsK * (n2 * n2);
+    if (MODEL == 0) gk = __builtin_fmaf(kRA_GsKLT, l, gk);
+    real t = kRA_GsL * (vs - EL);
+    t = __builtin_fmaf(kRA_GsNa * (m * m * m) * h, vs - kRA_EsNa, t);
+    t = __builtin_fmaf(gk, vs - kRA_EsK, t);
+    t = __builtin_fmaf(s.Gi_s * E, vs - kRA_Egaba, t);
+    t = __builtin_fmaf(s.Ge_s * E, vs, t);
+    k.Vs = __builtin_fmaf((real)(100000.0 / 275000.0), vd - vs, u.mu_term_s - t);   // divided by cm = 1.0
+    const real ica = GCa * (r * r) * (vd - kRA_EdCa);
+    real d = __builtin_fmaf(kRA_GdL, vd - EL, ica);
+    d = __builtin_fmaf(kRA_GdCaK * (vd - kRA_EdK), c * ca * __builtin_amdgcn_rcpf(ca + 6), d);
+    d = __builtin_fmaf(s.Ge_d * E, vd, d);
+    d = __builtin_fmaf(s.Gi_d * E, vd - kRA_Egaba, d);
+    k.Vd = __builtin_fmaf((real)(100000.0 / 550000.0), vs - vd, u.mu_term_d - d);
+    k.n = DIVR(g.nI - n, g.tN);
+    k.l = (MODEL == 0) ? CD(g.lI - l, 10) : HR(0.0);
+    k.h = DIVR(g.hI - h, g.tH);
+    k.r = (g.rI - r);  // divided by tauR = 1 (exact)
+    k.c = (MODEL == 0) ? CD(g.cI - c, 15) : CD(g.cI - c, 10);
+    k.Ca = __builtin_fmaf(-HR(0.1), ica, -HR(0.02) * ca);
+#else
     real m3 = m * m * m;
     real n4 = n * n * n * n;
     real acc = -kRA_GsL * (vs - EL) - kRA_GsNa * m3 * h * (vs - kRA_EsNa) - kRA_GsK * n4 * (vs - kRA_EsK);
@@ -211,6 +238,7 @@ __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, r
     k.r = (g.rI - r);  // divided by tauR = 1 (exact)
     k.c = (MODEL == 0) ? CD(g.cI - c, 15) : CD(g.cI - c, 10);
     k.Ca = (-HR(0.1) * GCa * r * r * (vd - kRA_EdCa) - HR(0.02) * ca);
+#endif
     return k;
 }
 
@@ -433,6 +461,21 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
 {
     HVC_FP_FAST
     IK k;
+#ifdef HVC_FP32
+    // one chain of multiply-adds, as in the quad form
+    const real n2 = n1 * n1;
+    const real gk = __builtin_fmaf(kI_gKdr, n2 * n2, kI_gKHT * w1);
+    real t = __builtin_fmaf(kI_gL, v, -(kI_gL * kI_El));
+    t = __builtin_fmaf(kI_gNa * h1 * (m1 * m1 * m1), v - kI_Ena, t);
+    t = __builtin_fmaf(gk, v - kI_Ek, t);
+    t = __builtin_fmaf(s.Ge * e2, v, t);
+    t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
+    k.V = -t;
+    auto gate = [](real a, real b, real x) { return __builtin_fmaf(-(a + b), x, a); };   // a (1 - x) - b x
+    k.n = gate(i_an<EXACT>(v, bad), i_bn<EXACT>(v, bad), n1);
+    k.m = gate(i_am<EXACT>(v, bad), i_bm<EXACT>(v, bad), m1);
+    k.h = gate(i_ah<EXACT>(v, bad), i_bh<EXACT>(v, bad), h1);
+#else
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
     k.V = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
@@ -440,6 +483,7 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
     k.n = i_an<EXACT>(v, bad) * (1 - n1) - i_bn<EXACT>(v, bad) * n1;
     k.m = i_am<EXACT>(v, bad) * (1 - m1) - i_bm<EXACT>(v, bad) * m1;
     k.h = i_ah<EXACT>(v, bad) * (1 - h1) - i_bh<EXACT>(v, bad) * h1;
+#endif
     k.w = (i_wInf<EXACT>(v, bad) - w1);  // divided by tauW = 1 (exact)
     return k;
 }
