@@ -825,21 +825,23 @@ __constant__ GateSlot kIQuadGates[4][2] = {
 // FP32 form of a gate slot: with per-lane constants the three rate forms are
 // one expression, E = exp(-(v + d) / e), rate = cB E + (cA (v + d) + cC) / (1 + sg E)
 // (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects
+// rate = cB E + (cA v + cC) / (1 + sg E) with E = 2^(k1 v + k0): the slot's
+// form A, B or C with v + d folded into the exponent and the numerator
 struct GateK {
-    real cA, cB, cC, sg, d, inv2_e;
+    real cA, cB, cC, sg, k1, k0;
 };
 __device__ __forceinline__ GateK gate_k(const GateSlot& g)
 {
-    return GateK{g.type == 0 ? g.c : HR(0.0), g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : HR(0.0),
-                 g.type == 0 ? -HR(1.0) : HR(1.0), g.d, g.inv2_e};
+    const real cA = g.type == 0 ? g.c : HR(0.0);
+    return GateK{cA, g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : cA * g.d, g.type == 0 ? -HR(1.0) : HR(1.0),
+                 -g.inv2_e, -(g.d * g.inv2_e)};
 }
 #ifdef HVC_FP32
 __device__ __forceinline__ real gate_rate_k(const GateK& g, real v)
 {
     HVC_FP_FAST
-    const real vd = v + g.d;
-    const real E = EXPCDV(-vd, 0, 0, g.inv2_e);
-    const real q = __builtin_fmaf(g.cA, vd, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
+    const real E = __builtin_amdgcn_exp2f(__builtin_fmaf(g.k1, v, g.k0));
+    const real q = __builtin_fmaf(g.cA, v, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
     return __builtin_fmaf(g.cB, E, q);
 }
 #endif
