@@ -144,6 +144,31 @@ constexpr real kRA_GsKLT = HR(3.5), kRA_GsK = HR(8.0), kRA_GsNa = HR(60.0), kRA_
 constexpr real kRA_GsL = HR(0.1), kRA_GdL = HR(0.1), kRA_EdCa = HR(120.0), kRA_EdK = -HR(90.0);
 constexpr real kRA_EsNa = HR(55.0), kRA_EsK = -HR(90.0), kRA_Egaba = -HR(80.0);
 
+#ifdef HVC_FP32
+// exp(sg (v + d) / B) as 2^(k1 v + k0): v + d folded into one multiply-add
+#define HVC_EXPLIN(v, sg, d, B)                                                                          \
+    __builtin_amdgcn_exp2f(__builtin_fmaf((v), (real)((sg) * 1.4426950408889634 / (double)(B)),         \
+                                          (real)((sg) * (double)(d) * 1.4426950408889634 / (double)(B))))
+#define HVC_SIGM(v, sg, d, B) __builtin_amdgcn_rcpf(1 + HVC_EXPLIN(v, sg, d, B))   // 1 / (1 + exp(sg (v + d) / B))
+template <bool EXACT>
+__device__ __forceinline__ real nInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, 35, 10); }
+template <bool EXACT>
+__device__ __forceinline__ real tauN(real v, Guard& bad) { HVC_FP_FAST return __builtin_fmaf(HR(0.5), HVC_SIGM(v, 1, 27, 15), HR(0.1)); }
+template <bool EXACT>
+__device__ __forceinline__ real hInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, 1, 45, 7); }
+template <bool EXACT>
+__device__ __forceinline__ real tauH(real v, Guard& bad) { HVC_FP_FAST return __builtin_fmaf(HR(0.75), HVC_SIGM(v, 1, 40.5, 6), HR(0.1)); }
+template <bool EXACT>
+__device__ __forceinline__ real mInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, 30, 9.5); }
+template <bool EXACT>
+__device__ __forceinline__ real lInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, 40, 5); }
+template <bool EXACT>
+__device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, 5, 10); }
+template <bool EXACT>
+__device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, -10, 7); }
+#undef HVC_SIGM
+#undef HVC_EXPLIN
+#else
 template <bool EXACT>
 __device__ __forceinline__ real nInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 35), 10)); }
 template <bool EXACT>
@@ -160,6 +185,7 @@ template <bool EXACT>
 __device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v + 5), 10)); }
 template <bool EXACT>
 __device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-(v - 10), 7)); }
+#endif
 
 struct RAK {
     real Vs, Vd, n, l, h, r, c, Ca;
@@ -435,6 +461,33 @@ struct IState {
 constexpr real kI_gL = HR(0.1), kI_El = -HR(65.0), kI_gNa = HR(100.0), kI_Ena = HR(55.0), kI_gKdr = HR(20.0);
 constexpr real kI_Ek = -HR(80.0), kI_gKHT = HR(500.0), kI_Ei = -HR(75.0), kI_A = HR(6000.0);
 
+#ifdef HVC_FP32
+// exp(-(v + d) / B) as 2^(k1 v + k0): v + d folded into one multiply-add
+#define HVC_EXPLIN(v, d, B)                                                                          \
+    __builtin_amdgcn_exp2f(__builtin_fmaf((v), (real)(-1.4426950408889634 / (double)(B)),           \
+                                          (real)(-(double)(d) * 1.4426950408889634 / (double)(B))))
+template <bool EXACT>
+__device__ __forceinline__ real i_an(real v, Guard& bad)
+{
+    HVC_FP_FAST return __builtin_fmaf(HR(0.15), v, HR(2.25)) * __builtin_amdgcn_rcpf(1 - HVC_EXPLIN(v, 15, 10));
+}
+template <bool EXACT>
+__device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * HVC_EXPLIN(v, 25, 80); }
+template <bool EXACT>
+__device__ __forceinline__ real i_am(real v, Guard& bad)
+{
+    HVC_FP_FAST return (v + 22) * __builtin_amdgcn_rcpf(1 - HVC_EXPLIN(v, 22, 10));
+}
+template <bool EXACT>
+__device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * HVC_EXPLIN(v, 47, 18); }
+template <bool EXACT>
+__device__ __forceinline__ real i_ah(real v, Guard& bad) { HVC_FP_FAST return HR(0.7) * HVC_EXPLIN(v, 34, 20); }
+template <bool EXACT>
+__device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return 10 * __builtin_amdgcn_rcpf(1 + HVC_EXPLIN(v, 4, 10)); }
+template <bool EXACT>
+__device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return __builtin_amdgcn_rcpf(1 + HVC_EXPLIN(v, 0, 5)); }
+#undef HVC_EXPLIN
+#else
 template <bool EXACT>
 __device__ __forceinline__ real i_an(real v, Guard& bad) { HVC_FP_FAST return DIVR(HR(0.15) * (v + 15), 1 - EXPCD(-(v + 15), 10)); }
 template <bool EXACT>
@@ -449,6 +502,7 @@ template <bool EXACT>
 __device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return DIVR(10, 1 + EXPCD(-(v + 4), 10)); }
 template <bool EXACT>
 __device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return DIVR(1, 1 + EXPCD(-v, 5)); }
+#endif
 
 struct IK {
     real V, n, m, h, w;
