@@ -34,7 +34,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the integration kernel, from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
 # profiles/integrate_c3_hbm_traffic.txt; keyed by (config, precision, HVC-I layout)
-TRAFFIC_PROFILED = {("c3", 1, 2): 7.173e6, ("c3", 0, 2): 7.780e6}
+TRAFFIC_PROFILED = {("c3", 1, 2): 7.167e6, ("c3", 0, 2): 7.782e6}
 
 
 def parse():
