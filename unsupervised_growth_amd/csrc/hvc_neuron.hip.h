@@ -542,26 +542,43 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
     return k;
 }
 
+// dt times the common factor of DP8 stages 2-6 (HHI_buffer.cpp:866-890): the
+// reference's order in the FP64 form; in the FP32 form dt and the factor are
+// one loop-invariant product
+#ifdef HVC_FP32
+#define HVC_F2(x) ((x) * (dt * (real)(1.0 / 18)))
+#define HVC_F3(x) ((x) * (dt * (real)(1.0 / 48)))
+#define HVC_F4(x) ((x) * (dt * HR(0.03125)))
+#define HVC_F5(x) ((x) * (dt * HR(0.015625)))
+#define HVC_F6(x) ((x) * (dt * (real)(1.0 / 80)))
+#else
+#define HVC_F2(x) CD(dt * (x), 18)
+#define HVC_F3(x) CD(dt * (x), 48)
+#define HVC_F4(x) dt * (x) * HR(0.03125)
+#define HVC_F5(x) dt * (x) * HR(0.015625)
+#define HVC_F6(x) CD(dt * (x), 80)
+#endif
+
 // HHI_buffer::DP8_step; row = decay factors of this step
 template <bool EXACT, class Row>
 __device__ __forceinline__ void i_dp8(Guard& bad, IState& s, real dt, const Row& row)
 {
     HVC_FP_FAST
     IK k1 = i_rhs<EXACT>(bad, s, s.V, s.n, s.m, s.h, s.w, HR(1.0), HR(1.0));
-    IK k2 = i_rhs<EXACT>(bad, s, s.V + CD(dt * k1.V, 18), s.n + CD(dt * k1.n, 18), s.m + CD(dt * k1.m, 18), s.h + CD(dt * k1.h, 18),
-                  s.w + CD(dt * k1.w, 18), row.e2[kOffDp2], row.e5[kOffDp2]);
-    IK k3 = i_rhs<EXACT>(bad, s, s.V + CD(dt * (k1.V + 3 * k2.V), 48), s.n + CD(dt * (k1.n + 3 * k2.n), 48),
-                  s.m + CD(dt * (k1.m + 3 * k2.m), 48), s.h + CD(dt * (k1.h + 3 * k2.h), 48),
-                  s.w + CD(dt * (k1.w + 3 * k2.w), 48), row.e2[kOffDp3], row.e5[kOffDp3]);
-    IK k4 = i_rhs<EXACT>(bad, s, s.V + dt * (k1.V + 3 * k3.V) * HR(0.03125), s.n + dt * (k1.n + 3 * k3.n) * HR(0.03125),
-                  s.m + dt * (k1.m + 3 * k3.m) * HR(0.03125), s.h + dt * (k1.h + 3 * k3.h) * HR(0.03125),
-                  s.w + dt * (k1.w + 3 * k3.w) * HR(0.03125), row.e2[kOffDp4], row.e5[kOffDp4]);
+    IK k2 = i_rhs<EXACT>(bad, s, s.V + HVC_F2(k1.V), s.n + HVC_F2(k1.n), s.m + HVC_F2(k1.m), s.h + HVC_F2(k1.h),
+                  s.w + HVC_F2(k1.w), row.e2[kOffDp2], row.e5[kOffDp2]);
+    IK k3 = i_rhs<EXACT>(bad, s, s.V + HVC_F3(k1.V + 3 * k2.V), s.n + HVC_F3(k1.n + 3 * k2.n),
+                  s.m + HVC_F3(k1.m + 3 * k2.m), s.h + HVC_F3(k1.h + 3 * k2.h),
+                  s.w + HVC_F3(k1.w + 3 * k2.w), row.e2[kOffDp3], row.e5[kOffDp3]);
+    IK k4 = i_rhs<EXACT>(bad, s, s.V + HVC_F4(k1.V + 3 * k3.V), s.n + HVC_F4(k1.n + 3 * k3.n),
+                  s.m + HVC_F4(k1.m + 3 * k3.m), s.h + HVC_F4(k1.h + 3 * k3.h),
+                  s.w + HVC_F4(k1.w + 3 * k3.w), row.e2[kOffDp4], row.e5[kOffDp4]);
 #define C5(x) (20 * k1.x - 75 * k3.x + 75 * k4.x)
-    IK k5 = i_rhs<EXACT>(bad, s, s.V + dt * C5(V) * HR(0.015625), s.n + dt * C5(n) * HR(0.015625), s.m + dt * C5(m) * HR(0.015625), s.h + dt * C5(h) * HR(0.015625),
-                  s.w + dt * C5(w) * HR(0.015625), row.e2[kOffDp5], row.e5[kOffDp5]);
+    IK k5 = i_rhs<EXACT>(bad, s, s.V + HVC_F5(C5(V)), s.n + HVC_F5(C5(n)), s.m + HVC_F5(C5(m)), s.h + HVC_F5(C5(h)),
+                  s.w + HVC_F5(C5(w)), row.e2[kOffDp5], row.e5[kOffDp5]);
 #define C6(x) (3 * k1.x + 15 * k4.x + 12 * k5.x)
-    IK k6 = i_rhs<EXACT>(bad, s, s.V + CD(dt * C6(V), 80), s.n + CD(dt * C6(n), 80), s.m + CD(dt * C6(m), 80), s.h + CD(dt * C6(h), 80),
-                  s.w + CD(dt * C6(w), 80), row.e2[kOffDp6], row.e5[kOffDp6]);
+    IK k6 = i_rhs<EXACT>(bad, s, s.V + HVC_F6(C6(V)), s.n + HVC_F6(C6(n)), s.m + HVC_F6(C6(m)), s.h + HVC_F6(C6(h)),
+                  s.w + HVC_F6(C6(w)), row.e2[kOffDp6], row.e5[kOffDp6]);
 #define C7(x) (CK(29443841, k1.x, 614563906) + CK(77736538, k4.x, 692538347) - CK(28693883, k5.x, 1125000000) + \
                CK(23124283, k6.x, 1800000000))
     IK k7 = i_rhs<EXACT>(bad, s, s.V + dt * C7(V), s.n + dt * C7(n), s.m + dt * C7(m), s.h + dt * C7(h), s.w + dt * C7(w),
@@ -765,11 +782,11 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
 
 // DP8 stage values: the combination of stage derivatives P1.. of one
 // variable (HHI_buffer.cpp:426-691), shared by the five-lane and quad forms
-#define HVC_DP8_S2(P) CD(dt * P##1, 18)
-#define HVC_DP8_S3(P) CD(dt * (P##1 + 3 * P##2), 48)
-#define HVC_DP8_S4(P) dt * (P##1 + 3 * P##3) * HR(0.03125)
-#define HVC_DP8_S5(P) dt * (20 * P##1 - 75 * P##3 + 75 * P##4) * HR(0.015625)
-#define HVC_DP8_S6(P) CD(dt * (3 * P##1 + 15 * P##4 + 12 * P##5), 80)
+#define HVC_DP8_S2(P) HVC_F2(P##1)
+#define HVC_DP8_S3(P) HVC_F3(P##1 + 3 * P##2)
+#define HVC_DP8_S4(P) HVC_F4(P##1 + 3 * P##3)
+#define HVC_DP8_S5(P) HVC_F5(20 * P##1 - 75 * P##3 + 75 * P##4)
+#define HVC_DP8_S6(P) HVC_F6(3 * P##1 + 15 * P##4 + 12 * P##5)
 #define HVC_DP8_S7(P) dt * (CK(29443841, P##1, 614563906) + CK(77736538, P##4, 692538347) - \
                             CK(28693883, P##5, 1125000000) + CK(23124283, P##6, 1800000000))
 #define HVC_DP8_S8(P) dt * (CK(16016141, P##1, 946692911) + CK(61564180, P##4, 158732637) + \
