@@ -30,7 +30,8 @@ typedef enum {
     HVC_E_HIP = 3,      /* HIP runtime error */
     HVC_E_NUMERIC = 4,  /* check_bad_values flagged an HVC-RA neuron (HH2_test.cpp:244-260) */
     HVC_E_STATE = 5,    /* call out of order (e.g. run before upload) */
-    HVC_E_CAPACITY = 6  /* an engine buffer overflowed; result discarded */
+    HVC_E_CAPACITY = 6  /* an engine buffer overflowed; the trial's spikes are discarded (no spike query
+                           answers until the next trial) and its synapse state is undefined */
 } hvc_status;
 
 /* SynapticParameters, c++/include/ConfigurationNetworkGrowth.h:11-34 */
@@ -231,6 +232,19 @@ hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32
  * No counterpart in the reference (it has no profiler). */
 hvc_status hvc_kernel_timing(hvc_ctx* ctx, const char* name, double* avg_ms, int* launches);
 hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled);
+
+/* Branch counters of the synapse-state machine, summed over the trials (and
+ * networks) run since hvc_set_branch_counting(ctx, 1), which clears them
+ * (0 stops counting).  counts gets HVC_NUM_BRANCH_COUNTERS entries, in order:
+ *   0 activate, 1 super, 2 unsuper, 3 deactivate   (update_synapse transitions, NGS:16546-16592)
+ *   4 remodel                                      (axon_remodeling, NGS:16363-16415)
+ *   5 ltd_delivery, 6 ltd_delivery_saturated       (LTD on delivery, NGS:9104-9183, per application)
+ *   7 desaturation                                 (a saturated row lost a supersynapse, NGS:9144-9146)
+ *   8 sync_ltp, 9 sync_ltd, 10 sync_saturated      (sync-point STDP, NGS:9375-9542, per application)
+ * No counterpart in the reference: it shows which branches a workload drove. */
+#define HVC_NUM_BRANCH_COUNTERS 11
+hvc_status hvc_set_branch_counting(hvc_ctx* ctx, int enabled);
+hvc_status hvc_branch_counts(hvc_ctx* ctx, int* counts);
 
 /* Device evaluation of the engine's own math on n host values: fn 0 exp(a),
  * 1 log(a), 2 a / b through the constant-divisor path.  No counterpart in the

@@ -78,12 +78,17 @@ def oracle():
     return _ora
 
 
+def ref_built(variant):
+    """True when oracle/_ref holds the harness library of this variant (checks
+    the file only: nothing is loaded, so collecting tests loads no reference code)."""
+    return os.path.exists(os.path.join(REF_DIR, "libhvcref_%s.so" % variant))
+
+
 def ref(variant):
     """variant 'ora' (oracle exp/log interposed), 'glibc' or 'noise'; None when not built."""
-    path = os.path.join(REF_DIR, "libhvcref_%s.so" % variant)
-    if not os.path.exists(path):
+    if not ref_built(variant):
         return None
-    return ctypes.CDLL(path)
+    return ctypes.CDLL(os.path.join(REF_DIR, "libhvcref_%s.so" % variant))
 
 
 def philox(ctr, key):

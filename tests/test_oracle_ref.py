@@ -15,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import i_inputs, oracle, ra_inputs, ref, run_i_trace, run_ra_trace
+from oracle_lib import i_inputs, oracle, ra_inputs, ref, ref_built, run_i_trace, run_ra_trace
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 N, STEPS, EVERY = 8, 2000, 100
@@ -81,7 +81,7 @@ def test_hhi_rk4_diverges_at_reference_dt():
     assert np.isfinite(st0).all()
 
 
-needs_ref = pytest.mark.skipif(ref("ora") is None, reason="oracle/_ref not built (reference checkout absent)")
+needs_ref = pytest.mark.skipif(not ref_built("ora"), reason="oracle/_ref not built (reference checkout absent)")
 
 
 @needs_ref

@@ -22,10 +22,13 @@ STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: 
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
            "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
            "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_upload_batch", "hvc_run_batch",
-           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses", "hvc_pool_footprint",
+           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses",
+           "hvc_pool_footprint",
            "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
            "hvc_pool_download_synapses", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
-           "hvc_last_error"]
+           "hvc_last_error", "hvc_set_branch_counting", "hvc_branch_counts"]
+BRANCH_COUNTERS = ("activate", "super", "unsuper", "deactivate", "remodel", "ltd_delivery", "ltd_delivery_saturated",
+                   "desaturation", "sync_ltp", "sync_ltd", "sync_saturated")
 # include/hvc_topology.h
 TOPOLOGY_EXPORTS = ["hvc_topology_generate", "hvc_topology_prepare", "hvc_topology_free", "hvc_mt19937"]
 # include/hvc_driver.h
@@ -184,6 +187,8 @@ def load(path=LIB_PATH):
     lib.hvc_autotune.argtypes = [vp, ctypes.c_int, P(Tuning), c_double_p]
     lib.hvc_kernel_timing.argtypes = [vp, ctypes.c_char_p, c_double_p, c_int_p]
     lib.hvc_set_timing.argtypes = [vp, ctypes.c_int]
+    lib.hvc_set_branch_counting.argtypes = [vp, ctypes.c_int]
+    lib.hvc_branch_counts.argtypes = [vp, c_int_p]
     lib.hvc_eval_math.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_double_p, c_double_p, c_double_p]
     lib.hvc_destroy.argtypes = [vp]
     lib.hvc_destroy.restype = None

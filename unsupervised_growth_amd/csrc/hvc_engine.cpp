@@ -59,6 +59,44 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// device buffers of one call, freed on every return path
+struct CallBufs {
+    std::vector<void*> p;
+    template <class T>
+    hipError_t alloc(T** out, size_t bytes)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 1);
+        if (e == hipSuccess) p.push_back(q);
+        *out = (T*)q;
+        return e;
+    }
+    ~CallBufs()
+    {
+        for (void* q : p) hipFree(q);
+    }
+};
+
+// HIP events of one call, destroyed on every return path
+struct CallEvents {
+    std::vector<hipEvent_t> ev;
+    hipError_t create(size_t n)
+    {
+        ev.reserve(n);
+        for (size_t k = 0; k < n; ++k) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        return hipSuccess;
+    }
+    ~CallEvents()
+    {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+};
+
 }  // namespace
 
 // one resident network of a serving pool, in its own ids
@@ -132,6 +170,8 @@ struct hvc_ctx {
     std::vector<uint32_t> h_net_trial;
     std::vector<int> h_net_first, h_net_growth;
 
+    int* d_hits = nullptr;         // branch counters (hvc_set_branch_counting)
+
     bool trial_done = false;
     int log_counts[3] = {0, 0, 0};
     std::vector<int2> h_log[3];
@@ -177,10 +217,11 @@ static T* dalloc(hvc_ctx* ctx, size_t count, hvc_status* st)
     return (T*)p;
 }
 
-static void free_bufs(hvc_ctx* ctx)
+// free the device buffers allocated since ctx->bufs held `first` of them
+static void free_bufs(hvc_ctx* ctx, size_t first = 0)
 {
-    for (auto& b : ctx->bufs) hipFree(b.p);
-    ctx->bufs.clear();
+    for (size_t k = first; k < ctx->bufs.size(); ++k) hipFree(ctx->bufs[k].p);
+    ctx->bufs.resize(std::min(first, ctx->bufs.size()));
 }
 
 extern "C" const char* hvc_last_error(const hvc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
@@ -551,6 +592,7 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
 {
     hvc_status st = HVC_OK;
     NetDev& nd = ctx->nd;
+    const size_t first_buf = ctx->bufs.size();
     const int nra = std::max(c.ra, 1), ni = std::max(c.i, 1), n_nets = std::max(c.nets, 1);
     const size_t E = (size_t)std::max<long long>(c.rr, 1);
     const size_t A = (size_t)std::max<long long>(c.a2i, 1), Bn = (size_t)std::max<long long>(c.i2ra, 1);
@@ -664,7 +706,7 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.in_ge_i = mk_inbox(ni, c.cap_ge_i);
     nd.err = dalloc<unsigned int>(ctx, 1, &st);
     nd.warn = dalloc<int>(ctx, 3 * (size_t)n_nets + 1, &st);
-    if (st != HVC_OK) free_bufs(ctx);
+    if (st != HVC_OK) free_bufs(ctx, first_buf);
     return st;
 }
 
@@ -924,7 +966,7 @@ static hvc_status sorted_log(hvc_ctx* ctx, int kind, const std::vector<int2>** o
     std::vector<int2>& log = ctx->h_log[kind];
     if (!ctx->h_log_valid[kind]) {
         const int2* d_log = kind == 0 ? ctx->nd.soma_log : (kind == 1 ? ctx->nd.dend_log : ctx->nd.i_log);
-        int count = ctx->log_counts[kind];
+        const int count = std::min(ctx->log_counts[kind], ctx->nd.log_cap);
         log.resize(count);
         if (count > 0) HVC_TRY(hipMemcpy(log.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
         std::sort(log.begin(), log.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
@@ -963,6 +1005,35 @@ extern "C" hvc_status hvc_set_timing(hvc_ctx* ctx, int enabled)
         ctx->t_sum[k] = 0;
         ctx->t_cnt[k] = 0;
     }
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_set_branch_counting(hvc_ctx* ctx, int enabled)
+{
+    if (!ctx) return HVC_E_ARG;
+    HVC_TRY(hipSetDevice(ctx->device));
+    if (!enabled) {
+        ctx->nd.hits = nullptr;
+        return HVC_OK;
+    }
+    if (!ctx->d_hits) {
+        hvc_status st = HVC_OK;
+        ctx->d_hits = dalloc<int>(ctx, kNumHits, &st);
+        if (st != HVC_OK) return st;
+    }
+    HVC_TRY(hipMemsetAsync(ctx->d_hits, 0, sizeof(int) * kNumHits, ctx->stream));
+    HVC_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->nd.hits = ctx->d_hits;
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_branch_counts(hvc_ctx* ctx, int* counts)
+{
+    if (!ctx || !counts) return HVC_E_ARG;
+    if (!ctx->d_hits) return fail(ctx, HVC_E_STATE, "branch counting was never enabled");
+    HVC_TRY(hipSetDevice(ctx->device));
+    HVC_TRY(hipMemcpyAsync(counts, ctx->d_hits, sizeof(int) * kNumHits, hipMemcpyDeviceToHost, ctx->stream));
+    HVC_TRY(hipStreamSynchronize(ctx->stream));
     return HVC_OK;
 }
 
@@ -1108,16 +1179,17 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     HVC_TRY(hipMemcpyAsync(vd.data(), nd.ra[1], sizeof(double) * ctx->n_ra, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
     tcollect(ctx);
-    ctx->trial_done = true;
-    for (int k = 0; k < 3; ++k) {
-        ctx->log_counts[k] = logc[k];
-        ctx->h_log_valid[k] = false;
-    }
+    for (int k = 0; k < 3; ++k) ctx->h_log_valid[k] = false;
     if (err) {
-        char buf[160];
-        std::snprintf(buf, sizeof buf, "engine capacity exceeded (flags 0x%x); trial result discarded", err);
+        // no spike query reads this trial's logs, which may have overflowed
+        ctx->trial_done = false;
+        char buf[200];
+        std::snprintf(buf, sizeof buf,
+                      "engine capacity exceeded (flags 0x%x); trial result discarded, synapse state undefined", err);
         return fail(ctx, HVC_E_CAPACITY, buf);
     }
+    ctx->trial_done = true;
+    for (int k = 0; k < 3; ++k) ctx->log_counts[k] = std::min(logc[k], ctx->nd.log_cap);
     // check_bad_values (HH2_test.cpp:244-260), per network
     int any_bad = 0;
     for (int b = 0; b < B; ++b) {
@@ -1240,8 +1312,9 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
     launch_reset(s, nd, ctx->E);
     launch_chain_edges(s, nd);
     const int nw = std::min(windows, (int)ctx->windows.size());
-    std::vector<hipEvent_t> ev(6 * (size_t)nw);
-    for (auto& e : ev) HVC_TRY(hipEventCreate(&e));
+    CallEvents events;
+    HVC_TRY(events.create(6 * (size_t)nw));
+    std::vector<hipEvent_t>& ev = events.ev;
     hvc_status r = HVC_OK;
     for (int w = 0; w < nw; ++w) {
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
@@ -1269,7 +1342,6 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
         hipEventElapsedTime(&c, ev[6 * w + 3], ev[6 * w + 4]);
         ms[0] += a; ms[1] += b; ms[2] += c;
     }
-    for (auto& x : ev) hipEventDestroy(x);
     for (int k = 0; k < 3; ++k) ms[k] /= std::max(nw, 1);
     unsigned int err = 0;
     HVC_TRY(hipMemcpy(&err, nd.err, sizeof(err), hipMemcpyDeviceToHost));
@@ -1652,7 +1724,9 @@ extern "C" hvc_status hvc_pool_download_synapses(hvc_ctx* ctx, int handle, hvc_s
 // ---------------------------------------------------------------------------
 // single-neuron traces
 // ---------------------------------------------------------------------------
-static hvc_status trace_tables(hvc_ctx* ctx, int n_steps, double** d_clock, void** d_decay)
+// the trace clock clk[s] = clk[s - 1] + dt (clk[0] = 0) and its decay rows,
+// in buffers owned by `bufs`
+static hvc_status trace_tables(hvc_ctx* ctx, CallBufs& bufs, int n_steps, double** d_clock, void** d_decay)
 {
     std::vector<double> clk(n_steps + 1, 0.0);
     double t = 0;
@@ -1660,10 +1734,27 @@ static hvc_status trace_tables(hvc_ctx* ctx, int n_steps, double** d_clock, void
         t += ctx->prm.dt;
         clk[s] = t;
     }
-    HVC_TRY(hipMalloc((void**)d_clock, sizeof(double) * (n_steps + 1)));
-    HVC_TRY(hipMalloc(d_decay, sizeof(DecayRow) * (n_steps + 1)));
+    HVC_TRY(bufs.alloc(d_clock, sizeof(double) * (n_steps + 1)));
+    HVC_TRY(bufs.alloc(d_decay, sizeof(DecayRow) * (n_steps + 1)));
     HVC_TRY(hipMemcpyAsync(*d_clock, clk.data(), sizeof(double) * (n_steps + 1), hipMemcpyHostToDevice, ctx->stream));
     launch_decay_table(ctx->stream, *d_clock, ctx->d_offsets, n_steps + 1, *d_decay);
+    HVC_TRY(hipGetLastError());
+    return HVC_OK;
+}
+
+// HVC-RA traces of n neurons over n_steps on device arrays (layouts of
+// hvc_trace_ra), with the trace tables at d_clock / d_decay
+static hvc_status trace_ra_device(hvc_ctx* ctx, int n, int n_steps, const double* d_el, const double* d_gca,
+                                  uint32_t trial, const double* d_kick, const double* d_exc, const double* d_inh,
+                                  double* d_out, uint8_t* d_fired, const double* d_clock, const void* d_decay)
+{
+    NetDev nd = ctx->nd;
+    nd.clock = d_clock;
+    nd.decay = d_decay;
+    if (ctx->prm.precision == 1)
+        launch_trace_ra_f32(ctx->stream, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
+    else
+        launch_trace_ra(ctx->stream, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
     HVC_TRY(hipGetLastError());
     return HVC_OK;
 }
@@ -1675,39 +1766,32 @@ extern "C" hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const doubl
     if (!ctx || n <= 0 || n_steps <= 0 || !erest || !gca || !kick || !exc_w || !inh_g || !out_state || !out_fired)
         return HVC_E_ARG;
     HVC_TRY(hipSetDevice(ctx->device));
+    CallBufs bufs;
     double* d_clock = nullptr;
     void* d_decay = nullptr;
-    hvc_status r = trace_tables(ctx, n_steps, &d_clock, &d_decay);
+    hvc_status r = trace_tables(ctx, bufs, n_steps, &d_clock, &d_decay);
     if (r != HVC_OK) return r;
     size_t in_sz = (size_t)n_steps * n;
     double *d_el, *d_gca, *d_kick, *d_exc, *d_inh, *d_out;
     uint8_t* d_fired;
-    HVC_TRY(hipMalloc(&d_el, sizeof(double) * n));
-    HVC_TRY(hipMalloc(&d_gca, sizeof(double) * n));
-    HVC_TRY(hipMalloc(&d_kick, sizeof(double) * in_sz));
-    HVC_TRY(hipMalloc(&d_exc, sizeof(double) * in_sz));
-    HVC_TRY(hipMalloc(&d_inh, sizeof(double) * in_sz));
-    HVC_TRY(hipMalloc(&d_out, sizeof(double) * (in_sz + n) * 12));
-    HVC_TRY(hipMalloc(&d_fired, in_sz));
+    HVC_TRY(bufs.alloc(&d_el, sizeof(double) * n));
+    HVC_TRY(bufs.alloc(&d_gca, sizeof(double) * n));
+    HVC_TRY(bufs.alloc(&d_kick, sizeof(double) * in_sz));
+    HVC_TRY(bufs.alloc(&d_exc, sizeof(double) * in_sz));
+    HVC_TRY(bufs.alloc(&d_inh, sizeof(double) * in_sz));
+    HVC_TRY(bufs.alloc(&d_out, sizeof(double) * (in_sz + n) * 12));
+    HVC_TRY(bufs.alloc(&d_fired, in_sz));
     hipStream_t s = ctx->stream;
     HVC_TRY(hipMemcpyAsync(d_el, erest, sizeof(double) * n, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_gca, gca, sizeof(double) * n, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_kick, kick, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_exc, exc_w, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_inh, inh_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
-    NetDev nd = ctx->nd;
-    nd.clock = d_clock;
-    nd.decay = d_decay;
-    if (ctx->prm.precision == 1)
-        launch_trace_ra_f32(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
-    else
-        launch_trace_ra(s, nd, n, n_steps, trial, d_el, d_gca, d_kick, d_exc, d_inh, d_out, d_fired);
-    HVC_TRY(hipGetLastError());
+    r = trace_ra_device(ctx, n, n_steps, d_el, d_gca, trial, d_kick, d_exc, d_inh, d_out, d_fired, d_clock, d_decay);
+    if (r != HVC_OK) return r;
     HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 12, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
-    hipFree(d_el); hipFree(d_gca); hipFree(d_kick); hipFree(d_exc); hipFree(d_inh); hipFree(d_out); hipFree(d_fired);
-    hipFree(d_clock); hipFree(d_decay);
     return HVC_OK;
 }
 
@@ -1716,17 +1800,18 @@ extern "C" hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_tr
 {
     if (!ctx || n <= 0 || n_steps <= 0 || !exc_g || !inh_g || !out_state || !out_fired) return HVC_E_ARG;
     HVC_TRY(hipSetDevice(ctx->device));
+    CallBufs bufs;
     double* d_clock = nullptr;
     void* d_decay = nullptr;
-    hvc_status r = trace_tables(ctx, n_steps, &d_clock, &d_decay);
+    hvc_status r = trace_tables(ctx, bufs, n_steps, &d_clock, &d_decay);
     if (r != HVC_OK) return r;
     size_t in_sz = (size_t)n_steps * n;
     double *d_exc, *d_inh, *d_out;
     uint8_t* d_fired;
-    HVC_TRY(hipMalloc(&d_exc, sizeof(double) * in_sz));
-    HVC_TRY(hipMalloc(&d_inh, sizeof(double) * in_sz));
-    HVC_TRY(hipMalloc(&d_out, sizeof(double) * (in_sz + n) * 7));
-    HVC_TRY(hipMalloc(&d_fired, in_sz));
+    HVC_TRY(bufs.alloc(&d_exc, sizeof(double) * in_sz));
+    HVC_TRY(bufs.alloc(&d_inh, sizeof(double) * in_sz));
+    HVC_TRY(bufs.alloc(&d_out, sizeof(double) * (in_sz + n) * 7));
+    HVC_TRY(bufs.alloc(&d_fired, in_sz));
     hipStream_t s = ctx->stream;
     HVC_TRY(hipMemcpyAsync(d_exc, exc_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(d_inh, inh_g, sizeof(double) * in_sz, hipMemcpyHostToDevice, s));
@@ -1741,8 +1826,6 @@ extern "C" hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_tr
     HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 7, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
-    hipFree(d_exc); hipFree(d_inh); hipFree(d_out); hipFree(d_fired);
-    hipFree(d_clock); hipFree(d_decay);
     return HVC_OK;
 }
 
@@ -1751,10 +1834,11 @@ extern "C" hvc_status hvc_eval_math(hvc_ctx* ctx, int fn, int n, const double* a
     if (!ctx || n < 0 || fn < 0 || fn > 2 || (n > 0 && (!a || !out || (fn == 2 && !b)))) return HVC_E_ARG;
     if (n == 0) return HVC_OK;
     HVC_TRY(hipSetDevice(ctx->device));
+    CallBufs bufs;
     double *da = nullptr, *db = nullptr, *dout = nullptr;
-    HVC_TRY(hipMalloc(&da, sizeof(double) * n));
-    HVC_TRY(hipMalloc(&db, sizeof(double) * n));
-    HVC_TRY(hipMalloc(&dout, sizeof(double) * n));
+    HVC_TRY(bufs.alloc(&da, sizeof(double) * n));
+    HVC_TRY(bufs.alloc(&db, sizeof(double) * n));
+    HVC_TRY(bufs.alloc(&dout, sizeof(double) * n));
     hipStream_t s = ctx->stream;
     HVC_TRY(hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, s));
     if (fn == 2) HVC_TRY(hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
@@ -1762,8 +1846,5 @@ extern "C" hvc_status hvc_eval_math(hvc_ctx* ctx, int fn, int n, const double* a
     HVC_TRY(hipGetLastError());
     HVC_TRY(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
-    hipFree(da);
-    hipFree(db);
-    hipFree(dout);
     return HVC_OK;
 }

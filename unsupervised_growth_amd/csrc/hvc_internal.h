@@ -26,6 +26,26 @@ enum ErrFlag : uint32_t {
     kErrNoise = 64u,
 };
 
+// counters of the synapse-state machine's branches (hvc_set_branch_counting):
+// update_synapse transitions, axon remodeling, LTD on delivery (unsaturated /
+// saturated row), desaturation, and the sync-point LTP / LTD (unsaturated
+// row) and applications in saturated rows
+enum BranchHit {
+    kHitActivate = 0,
+    kHitSuper,
+    kHitUnsuper,
+    kHitDeactivate,
+    kHitRemodel,
+    kHitLtdDelivery,
+    kHitLtdDeliverySaturated,
+    kHitDesaturation,
+    kHitSyncLtp,
+    kHitSyncLtd,
+    kHitSyncSaturated,
+    kNumHits
+};
+static_assert(kNumHits == HVC_NUM_BRANCH_COUNTERS, "branch counters of the C ABI");
+
 // Poisson noise events of one neuron, generated ahead of the integration for a
 // chunk of coupling windows (the draws depend on the stream and the clock
 // only): at step `step` add `inc` to conductance `target` (HVC-RA 0 Gi_s,
@@ -215,6 +235,7 @@ struct NetDev {
 
     unsigned int* err;
     int* warn;                   // [3 n_nets]
+    int* hits;                   // [kNumHits] branch counters, nullptr when not counting
 };
 
 __device__ __forceinline__ NoiseId ra_noise_id(const NetDev& nd, int i)

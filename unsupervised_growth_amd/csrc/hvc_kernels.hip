@@ -581,19 +581,23 @@ __device__ __forceinline__ void update_synapse_c(const NetDev& nd, int i, int j,
     if ((w >= p.ACTIVATION_THRESHOLD) && (c.act == 0) && (c.rem == 0)) {
         c.act = 1;
         alist[c.acnt++] = j;
+        count_hit(nd, kHitActivate);
     }
     if ((w >= p.SUPERSYNAPSE_THRESHOLD) && (c.sup == 0) && (c.scnt < p.Nss)) {
         c.sup = 1;
         slist[c.scnt++] = j;
+        count_hit(nd, kHitSuper);
     }
     if ((w < p.SUPERSYNAPSE_THRESHOLD) && (c.sup == 1)) {
         c.sup = 0;
         c.rem = 0;
         if (!list_erase(slist, &c.scnt, j)) atomicAdd(&row_warn(nd, i)[2], 1);
+        count_hit(nd, kHitUnsuper);
     }
     if ((w < p.ACTIVATION_THRESHOLD) && (c.act == 1)) {
         c.act = 0;
         if (!list_erase(alist, &c.acnt, j)) atomicAdd(&row_warn(nd, i)[2], 1);
+        count_hit(nd, kHitDeactivate);
     }
 }
 
@@ -602,7 +606,7 @@ __device__ __forceinline__ void update_synapse_c(const NetDev& nd, int i, int j,
 // entry is loaded a step ahead; the edge's and the row's synapse state stay in
 // registers through the steps and are written back once (rs follows them).
 __device__ __forceinline__ void ltd_on_delivery(const NetDev& nd, int i, int e, int tgt, double T, double now,
-                                                const ItemIn& in, RowState& rs)
+                                                const ItemIn& in, RowState& rs, int hit)
 {
     const int pc = in.post_n;
     if (pc == 0) return;
@@ -619,6 +623,7 @@ __device__ __forceinline__ void ltd_on_delivery(const NetDev& nd, int i, int e, 
         if (j + 1 < pc - first) vn = ring[(first + j + 1) % kPostRing];
         on = on || !(v < lim);
         if (on) {
+            count_hit(nd, hit);
             w = ltd(nd.syn, w, v - T, row_warn(nd, i));
             update_synapse_c(nd, i, tgt, w, c);
         }
@@ -650,8 +655,10 @@ __device__ __forceinline__ void rr_item(const NetDev& nd, int i, int e, int tgt,
         // a saturated row takes LTD on supersynapses only, and reverts its
         // remodeling when one of them falls below the threshold
         const bool saturated = rs.scnt == Nss;
-        if (!saturated || in.sup == 1) ltd_on_delivery(nd, i, e, tgt, T, now, in, rs);
+        if (!saturated || in.sup == 1)
+            ltd_on_delivery(nd, i, e, tgt, T, now, in, rs, saturated ? kHitLtdDeliverySaturated : kHitLtdDelivery);
         if (saturated && rs.scnt < Nss) {
+            count_hit(nd, kHitDesaturation);
             update_row(nd, i);
             rs.load(nd, i);
         }

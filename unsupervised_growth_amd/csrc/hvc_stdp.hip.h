@@ -68,6 +68,12 @@ __device__ __forceinline__ bool list_erase(int* list, int* cnt, int j)
     return false;
 }
 
+// one pass through branch k of the synapse-state machine (when counting)
+__device__ __forceinline__ void count_hit(const NetDev& nd, int k)
+{
+    if (nd.hits) atomicAdd(nd.hits + k, 1);
+}
+
 // warning counters of the network that owns HVC-RA row i
 __device__ __forceinline__ int* row_warn(const NetDev& nd, int i) { return nd.warn + 3 * nd.ra_net[i]; }
 
@@ -93,19 +99,23 @@ static __device__ void update_synapse(const NetDev& nd, int i, int e)
     if ((w >= p.ACTIVATION_THRESHOLD) && (nd.act[e] == 0) && (nd.remodeled[i] == 0)) {
         nd.act[e] = 1;
         alist[nd.act_cnt[i]++] = j;
+        count_hit(nd, kHitActivate);
     }
     if ((w >= p.SUPERSYNAPSE_THRESHOLD) && (nd.sup[e] == 0) && (nd.sup_cnt[i] < p.Nss)) {
         nd.sup[e] = 1;
         slist[nd.sup_cnt[i]++] = j;
+        count_hit(nd, kHitSuper);
     }
     if ((w < p.SUPERSYNAPSE_THRESHOLD) && (nd.sup[e] == 1)) {
         nd.sup[e] = 0;
         nd.remodeled[i] = 0;
         if (!list_erase(slist, &nd.sup_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
+        count_hit(nd, kHitUnsuper);
     }
     if ((w < p.ACTIVATION_THRESHOLD) && (nd.act[e] == 1)) {
         nd.act[e] = 0;
         if (!list_erase(alist, &nd.act_cnt[i], j)) atomicAdd(&row_warn(nd, i)[2], 1);
+        count_hit(nd, kHitDeactivate);
     }
 }
 
@@ -124,6 +134,7 @@ static __device__ void axon_remodeling(const NetDev& nd, int i)
     for (int k = 0; k < nd.sup_cnt[i]; ++k) nd.act_list[base + k] = slist[k];
     nd.act_cnt[i] = nd.sup_cnt[i];
     nd.remodeled[i] = 1;
+    count_hit(nd, kHitRemodel);
 }
 
 // LTP / LTD of row i at the sync step t1 against the window's post events of
@@ -163,10 +174,13 @@ static __device__ void rr_sync(const NetDev& nd, int i, int t1, const uint64_t* 
             while (q < pc && ring[q % kPreRing] < lim) ++q;
             for (; q < pc; ++q) {
                 double dt = now - ring[q % kPreRing];
-                if (dt <= nd.syn.T_0)
+                if (dt <= nd.syn.T_0) {
+                    count_hit(nd, saturated ? kHitSyncSaturated : kHitSyncLtd);
                     nd.w[e] = ltd(nd.syn, nd.w[e], dt, row_warn(nd, i));
-                else
+                } else {
+                    count_hit(nd, saturated ? kHitSyncSaturated : kHitSyncLtp);
                     nd.w[e] = ltp(nd.syn, nd.w[e], dt, row_warn(nd, i));
+                }
                 update_synapse(nd, i, e);
             }
         }

@@ -251,13 +251,14 @@ class Engine:
             ts[b] = _abi.Trial(int(kw.get("mode", 0)), int(bool(kw.get("training", True))),
                                int(kw.get("trial_index", 0)) & 0xffffffff, int(bool(kw.get("first_trial", False))),
                                float(kw.get("mean_injection_time", 100.0)))
-        st = self.lib.hvc_run_batch(self.ctx, ts, None)
+        outs = (_abi.TrialOut * B)()   # no buffers: only the spike totals, bad_values and warnings come back
+        st = self.lib.hvc_run_batch(self.ctx, ts, outs)
         if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
             self._check(st)
         out = []
         for b, n in enumerate(self.nets):
             out.append(TrialResult(self._spikes(b, 0, n.n_ra), self._spikes(b, 1, n.n_ra),
-                                   self._spikes(b, 2, n.n_i), 0, (0, 0, 0)))
+                                   self._spikes(b, 2, n.n_i), outs[b].bad_values, tuple(outs[b].warnings)))
         return out
 
     def download_synapses(self, net_index=None) -> SynapseState:
@@ -340,6 +341,16 @@ class Engine:
 
     def set_timing(self, on=True):
         self._check(self.lib.hvc_set_timing(self.ctx, int(bool(on))))
+
+    def set_branch_counting(self, on=True):
+        """Count the synapse-state machine's branches from now on (cleared)."""
+        self._check(self.lib.hvc_set_branch_counting(self.ctx, int(bool(on))))
+
+    def branch_counts(self):
+        """{branch name: passes} since set_branch_counting(True)."""
+        c = np.zeros(len(_abi.BRANCH_COUNTERS), np.int32)
+        self._check(self.lib.hvc_branch_counts(self.ctx, _p(c, _abi.c_int_p)))
+        return dict(zip(_abi.BRANCH_COUNTERS, (int(x) for x in c)))
 
     def kernel_timing(self, name):
         ms, n = ctypes.c_double(), ctypes.c_int()
