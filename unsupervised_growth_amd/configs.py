@@ -28,23 +28,32 @@ from .network import (MaturationParams, Network, NoiseParams, SynapticParams, TI
 
 
 def _gaussian_csr(rng, pa, pb, peak, width, gmax, delays, dt, chunk=2048):
-    rows_t, rows_w = [], []
+    """Pairs (a, b) connected with probability peak * exp(-|pa - pb|^2 / (2 width^2)):
+    one uniform per pair of a chunk of rows (row-major), then the weights of the
+    chunk's connections in row order, then the delays of all connections."""
+    tg_chunks, w_chunks, counts = [], [], []
+    den = 2 * width * width
     for a0 in range(0, len(pa), chunk):
         blk = pa[a0:a0 + chunk]
         if len(pb) == 0:
-            rows_t += [np.zeros(0, np.int32)] * len(blk)
-            rows_w += [np.zeros(0)] * len(blk)
+            counts.append(np.zeros(len(blk), np.int64))
             continue
-        d2 = ((blk[:, None, :] - pb[None, :, :]) ** 2).sum(-1)
-        prob = peak * np.exp(-d2 / (2 * width * width))
-        hit = rng.random(prob.shape) < prob
-        for r in range(len(blk)):
-            tg = np.nonzero(hit[r])[0].astype(np.int32)
-            rows_t.append(tg)
-            rows_w.append(rng.random(len(tg)) * gmax)
-    ptr = _csr(rows_t, len(pa))
-    tgt = np.concatenate(rows_t).astype(np.int32) if rows_t else np.zeros(0, np.int32)
-    w = np.concatenate(rows_w) if rows_w else np.zeros(0)
+        dx = blk[:, 0:1] - pb[None, :, 0]
+        d2 = dx * dx
+        dy = blk[:, 1:2] - pb[None, :, 1]
+        d2 += dy * dy
+        dz = blk[:, 2:3] - pb[None, :, 2]
+        d2 += dz * dz
+        hit = rng.random(d2.shape) < peak * np.exp(-d2 / den)
+        rows, cols = np.nonzero(hit)
+        counts.append(np.bincount(rows, minlength=len(blk)))
+        tg_chunks.append(cols.astype(np.int32))
+        w_chunks.append(rng.random(len(cols)) * gmax)
+    ptr = np.zeros(len(pa) + 1, np.int32)
+    if counts:
+        np.cumsum(np.concatenate(counts), out=ptr[1:])
+    tgt = np.concatenate(tg_chunks).astype(np.int32) if tg_chunks else np.zeros(0, np.int32)
+    w = np.concatenate(w_chunks) if w_chunks else np.zeros(0)
     return ptr, tgt, w, _delays(rng, len(tgt), delays, dt)
 
 
