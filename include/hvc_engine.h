@@ -221,6 +221,21 @@ hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const double* erest, c
 hvc_status hvc_trace_i(hvc_ctx* ctx, int n, int n_steps, int first_trial, uint32_t trial,
                        const double* exc_g, const double* inh_g, double* out_state, uint8_t* out_fired);
 
+/* hvc_trace_ra on device arrays, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every array is device memory of the context's device, in hvc_trace_ra's
+ * layouts; results equal hvc_trace_ra's bit for bit.  The trace clock and decay
+ * tables live in a fixed buffer that the library holds on each GPU until the
+ * process ends (about 3.8 MB), so n_steps is at most HVC_NOALLOC_MAX_STEPS
+ * (HVC_E_ARG above).  Every call on one GPU shares that buffer: calls that
+ * overlap on one GPU (from several threads or contexts) race.  The call returns
+ * after the context's stream has finished the trace. */
+#define HVC_NOALLOC_MAX_STEPS 16384
+hvc_status hvc_trace_ra_noalloc(hvc_ctx* ctx, int n, int n_steps, const double* d_erest, const double* d_gca,
+                                uint32_t trial, const double* d_kick, const double* d_exc_w, const double* d_inh_g,
+                                double* d_out_state, uint8_t* d_out_fired);
+
 /* Kernel timing: with timing enabled, HIP events on the engine's stream bracket
  * every 16th window's launches (and the end-of-trial decay); hvc_kernel_timing
  * returns the average device duration (ms) per launch of a kernel class over

@@ -22,7 +22,7 @@ STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: 
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
            "hvc_upload_synapses", "hvc_patch_neurons", "hvc_trial_spikes", "hvc_trace_ra", "hvc_trace_i",
            "hvc_kernel_timing", "hvc_set_timing", "hvc_eval_math", "hvc_upload_batch", "hvc_run_batch",
-           "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses",
+           "hvc_trace_ra_noalloc", "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses",
            "hvc_pool_footprint",
            "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
            "hvc_pool_download_synapses", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
@@ -168,6 +168,8 @@ def load(path=LIB_PATH):
                                  c_double_p, c_double_p, c_double_p, c_double_p, c_u8_p]
     lib.hvc_trace_i.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                 c_double_p, c_double_p, c_double_p, c_u8_p]
+    lib.hvc_trace_ra_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_uint32, vp, vp, vp, vp,
+                                         vp]
     lib.hvc_trial_spikes.argtypes = [vp, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
     lib.hvc_upload_batch.argtypes = [vp, ctypes.c_int, P(NetworkDesc), P(ctypes.c_uint32)]
     lib.hvc_run_batch.argtypes = [vp, P(Trial), P(TrialOut)]

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhvc_engine.so")
-SOURCES = ["hvc_kernels.hip", "hvc_kernels_f32.hip", "hvc_engine.cpp", "hvc_topology.cpp", "hvc_driver.cpp", "hvc_chain.cpp"]
+SOURCES = ["hvc_kernels.hip", "hvc_kernels_f32.hip", "hvc_noalloc.hip", "hvc_engine.cpp", "hvc_topology.cpp", "hvc_driver.cpp", "hvc_chain.cpp"]
 ARCH = os.environ.get("HVC_OFFLOAD_ARCH", "gfx950")
 
 

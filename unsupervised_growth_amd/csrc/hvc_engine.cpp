@@ -45,6 +45,9 @@ void launch_rr_pos(hipStream_t st, const NetDev& nd, RrPos* pos, int E);
 void launch_decay(hipStream_t st, const NetDev& nd);
 void launch_chain_edges(hipStream_t st, const NetDev& nd);
 void launch_eval_math(hipStream_t st, int fn, int n, const double* a, const double* b, double* out);
+hipError_t noalloc_setup(int device);
+void* noalloc_scratch(int device);
+void launch_clock_table(hipStream_t st, double* clock, int n_rows, double dt);
 }  // namespace hvc
 
 using namespace hvc;
@@ -256,6 +259,8 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     *out = ctx;
     HVC_TRY(hipSetDevice(device));
     HVC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    // this device's fixed buffer of hvc_trace_ra_noalloc (looked up once per device)
+    HVC_TRY(noalloc_setup(device));
     // default launch shape for one network: HVC-I in quads in the FP32 form
     // (the DPP exchange shortens DP8's dependent chain), five lanes in FP64
     // (there the extra gate per lane costs more than the exchange saves)
@@ -1792,6 +1797,48 @@ extern "C" hvc_status hvc_trace_ra(hvc_ctx* ctx, int n, int n_steps, const doubl
     HVC_TRY(hipMemcpyAsync(out_state, d_out, sizeof(double) * (in_sz + n) * 12, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipMemcpyAsync(out_fired, d_fired, in_sz, hipMemcpyDeviceToHost, s));
     HVC_TRY(hipStreamSynchronize(s));
+    return HVC_OK;
+}
+
+// true when p is device memory of this context's device (or managed memory)
+static bool on_device(const hvc_ctx* ctx, const void* p)
+{
+    hipPointerAttribute_t a;
+    std::memset(&a, 0, sizeof a);
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type == hipMemoryTypeDevice) return a.device == ctx->device;
+    return a.type == hipMemoryTypeManaged;
+}
+
+extern "C" hvc_status hvc_trace_ra_noalloc(hvc_ctx* ctx, int n, int n_steps, const double* d_erest,
+                                           const double* d_gca, uint32_t trial, const double* d_kick,
+                                           const double* d_exc_w, const double* d_inh_g, double* d_out_state,
+                                           uint8_t* d_out_fired)
+{
+    if (!ctx) return HVC_E_ARG;
+    if (n <= 0 || n_steps <= 0) return fail(ctx, HVC_E_ARG, "n and n_steps must be > 0");
+    if (n_steps > kNoAllocMaxSteps)
+        return fail(ctx, HVC_E_ARG, "n_steps " + std::to_string(n_steps) + " exceeds the fixed trace buffer's " +
+                                        std::to_string(kNoAllocMaxSteps) + " steps");
+    const void* arrays[] = {d_erest, d_gca, d_kick, d_exc_w, d_inh_g, d_out_state, d_out_fired};
+    for (const void* q : arrays)
+        if (!q || !on_device(ctx, q))
+            return fail(ctx, HVC_E_ARG, "hvc_trace_ra_noalloc takes device arrays of the context's device");
+    HVC_TRY(hipSetDevice(ctx->device));
+    unsigned char* base = (unsigned char*)noalloc_scratch(ctx->device);
+    if (!base) return fail(ctx, HVC_E_STATE, "fixed trace buffer not set up on this device");
+    double* d_clock = (double*)base;
+    void* d_decay = base + kNoAllocClockBytes;
+    launch_clock_table(ctx->stream, d_clock, n_steps + 1, ctx->prm.dt);
+    launch_decay_table(ctx->stream, d_clock, ctx->d_offsets, n_steps + 1, d_decay);
+    HVC_TRY(hipGetLastError());
+    hvc_status r = trace_ra_device(ctx, n, n_steps, d_erest, d_gca, trial, d_kick, d_exc_w, d_inh_g, d_out_state,
+                                   d_out_fired, d_clock, d_decay);
+    if (r != HVC_OK) return r;
+    HVC_TRY(hipStreamSynchronize(ctx->stream));
     return HVC_OK;
 }
 

@@ -15,6 +15,14 @@ constexpr int kMaxWindowSteps = 64;  // steps between two synchronisation points
 constexpr int kPostListCap = 8192;  // new post events per window, all networks together
 constexpr int kMaxNets = 4096;      // networks per context (12 bits of a post-event key)
 
+// hvc_trace_ra_noalloc: its clock and decay tables live in a fixed per-device
+// array (hvc_noalloc.hip) sized for this many trace steps
+constexpr int kNoAllocMaxSteps = HVC_NOALLOC_MAX_STEPS;
+constexpr int kNoAllocMaxDevices = 64;
+constexpr size_t kNoAllocDecayRowBytes = 224;   // sizeof(DecayRow)
+constexpr size_t kNoAllocClockBytes = ((size_t)(kNoAllocMaxSteps + 1) * sizeof(double) + 255) / 256 * 256;
+constexpr size_t kNoAllocScratchBytes = kNoAllocClockBytes + (size_t)(kNoAllocMaxSteps + 1) * kNoAllocDecayRowBytes;
+
 // error flags raised by kernels (bit mask)
 enum ErrFlag : uint32_t {
     kErrSpikeLog = 1u,
