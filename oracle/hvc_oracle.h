@@ -24,10 +24,15 @@
  *                       the same mt19937 stream.
  *   trial / STDP      : the reference trial (NGS:8839-9571, 12747-13118) needs
  *                       Boost and libconfig++, which this image lacks, so it is
- *                       not built; the restatement is pinned by an independent
- *                       line-by-line Python restatement on small networks,
- *                       closed-form known answers for LTP/LTD/update_synapse,
- *                       and invariants (see tests/test_oracle_trial.py).
+ *                       not built; the restatement is pinned by known answers
+ *                       derived by hand from the NGS lines on networks of one to
+ *                       six neurons (kick step and size, arrival step and sync of
+ *                       every connection type, zero delay, LTP/LTD in every
+ *                       branch with exact pre-event and sync-clock post-event
+ *                       times, the update_synapse truth table, axon_remodeling,
+ *                       potentiation_decay; tests/test_oracle_pins.py), by an
+ *                       independent Python restatement on small networks, and by
+ *                       invariants (tests/test_oracle_trial.py).
  */
 #ifndef HVC_ORACLE_H
 #define HVC_ORACLE_H
@@ -120,12 +125,25 @@ int ora_set_neuron_params(ora_sim* s, const double* erest, const double* gca);
 int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int first_trial,
                   double mean_injection_time);
 /* Spikes of the last trial: per-neuron counts then times (CSR). */
+int ora_threads(void);   /* threads the neuron steps of a large network run on (OpenMP) */
+void ora_set_threads(int n);
 int ora_spike_counts(const ora_sim* s, int kind, int* counts); /* kind 0 soma,1 dend,2 I; returns total */
 int ora_spike_times(const ora_sim* s, int kind, double* times);
 /* Counters of reference diagnostics (LTP/LTD rejected dt, missing list entries). */
 int ora_warnings(const ora_sim* s, int* out3);
 /* Final neuron states of the last trial (RA: 12 per neuron, I: 7 per neuron). */
 int ora_final_state(const ora_sim* s, double* ra, double* in);
+
+/* Probe for tests: with out != NULL, the next trials write state variable var
+ * of one neuron (kind 0 RA, var 0..11 as ORA_RA_NSTATE; kind 1 HVC-I, var 0..6)
+ * into out[t] after every step t (after that step's synchronisation, if any);
+ * out[0] is the value at the start of the trial.  out holds num_steps values. */
+int ora_set_probe(ora_sim* s, int kind, int neuron, int var, double* out);
+/* The synapse-state machine on the current state, for known-answer tests:
+ * update_synapse(i, j) (NGS:16546-16592) and axon_remodeling(i) (NGS:16363-16415).
+ * Return -1 when (i, j) is not a candidate edge. */
+int ora_update_synapse(ora_sim* s, int i, int j);
+int ora_axon_remodeling(ora_sim* s, int i);
 
 /* STDP primitives, exposed for known-answer tests (NGS:16615-16652). */
 double ora_ltp(const ora_synaptic_params* p, double w, double dt, int* rejected);

@@ -30,6 +30,12 @@
 #include "hvc_oracle.h"
 #include "ora_neuron.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* the neuron steps of a step run on OpenMP threads from this many neurons on */
+#define ORA_OMP_MIN_NEURONS 512
+
 #define STDP_WINDOW 100.0      /* NGS:13 */
 #define G_TRAINING_KICK 3.0    /* NGS:22 */
 #define EVENT_WINDOW 30.0      /* NGS:8841 */
@@ -134,7 +140,33 @@ struct ora_sim {
     dvec* pre_ev;                 /* per candidate edge */
     dvec* post_ev;                /* per RA */
     qitem* scratch;
+    /* probe: one state variable of one neuron after every step (tests) */
+    int probe_kind, probe_neuron, probe_var;
+    double* probe_out;
 };
+
+static double probe_value(const ora_sim* s)
+{
+    double st[ORA_RA_NSTATE];
+    if (s->probe_kind == 0)
+        ora_ra_get_state(&s->ra[s->probe_neuron], st);
+    else
+        ora_i_get_state(&s->in[s->probe_neuron], st);
+    return st[s->probe_var];
+}
+
+int ora_set_probe(ora_sim* s, int kind, int neuron, int var, double* out)
+{
+    if (out && ((kind == 0 && (neuron < 0 || neuron >= s->d.n_ra || var < 0 || var >= ORA_RA_NSTATE)) ||
+                (kind == 1 && (neuron < 0 || neuron >= s->d.n_i || var < 0 || var >= ORA_I_NSTATE)) ||
+                (kind != 0 && kind != 1)))
+        return -1;
+    s->probe_kind = kind;
+    s->probe_neuron = neuron;
+    s->probe_var = var;
+    s->probe_out = out;
+    return 0;
+}
 
 static int edge_of(const ora_sim* s, int i, int j)
 {
@@ -489,10 +521,16 @@ int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int 
     double internal_time = 0;
     double network_time = d->sync_ms;
     const int num_steps = s->num_steps;
+    if (s->probe_out) s->probe_out[0] = probe_value(s);
     for (int t = 1; t < num_steps; ++t) {
         internal_time += d->dt;
 
-        /* ---------------- HVC-RA neurons ---------------- */
+        /* ---------------- HVC-RA neurons ----------------
+         * The steps first: a neuron's step reads only its own state (the
+         * deliveries of this step go to the update_* sums, applied at the
+         * sync), so stepping every neuron before any delivery gives the
+         * reference's values; the threads share nothing. */
+#pragma omp parallel for schedule(static) if (nra >= ORA_OMP_MIN_NEURONS)
         for (int i = 0; i < nra; ++i) {
             ora_ra_neuron* a = &s->ra[i];
             if (training && s->is_training[i] && !injected[i]) {
@@ -502,7 +540,10 @@ int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int 
                 }
             }
             ora_ra_step(a);
-
+        }
+        /* spikes and delivery queues, neuron by neuron in the reference's order */
+        for (int i = 0; i < nra; ++i) {
+            ora_ra_neuron* a = &s->ra[i];
             if (a->fired_soma) {
                 dvec_push(&s->sp_soma[i], internal_time);
                 if (growth) {
@@ -609,10 +650,11 @@ int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int 
             if (a->fired_dend) dvec_push(&s->sp_dend[i], internal_time);
         }
 
-        /* ---------------- HVC-I neurons ---------------- */
+        /* ---------------- HVC-I neurons (steps first, as above) ---------------- */
+#pragma omp parallel for schedule(static) if (ni >= ORA_OMP_MIN_NEURONS)
+        for (int i = 0; i < ni; ++i) ora_i_step(&s->in[i]);
         for (int i = 0; i < ni; ++i) {
             ora_i_neuron* b = &s->in[i];
-            ora_i_step(b);
             if (b->fired) {
                 dvec_push(&s->sp_i[i], internal_time);
                 int k = 0;
@@ -690,6 +732,7 @@ int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int 
             }
             network_time += d->sync_ms;
         }
+        if (s->probe_out) s->probe_out[t] = probe_value(s);
     }
 
     if (growth) {
@@ -700,6 +743,40 @@ int ora_run_trial(ora_sim* s, int mode, int training, uint32_t trial_index, int 
     for (int i = 0; i < nra; ++i)
         if (ora_ra_check_bad_values(&s->ra[i]) == -1) return -1;
     return 0;
+}
+
+int ora_update_synapse(ora_sim* s, int i, int j)
+{
+    if (i < 0 || i >= s->d.n_ra) return -1;
+    int e = edge_of(s, i, j);
+    if (e < 0) return -1;
+    update_synapse(s, i, e);
+    return 0;
+}
+
+int ora_axon_remodeling(ora_sim* s, int i)
+{
+    if (i < 0 || i >= s->d.n_ra) return -1;
+    axon_remodeling(s, i);
+    return 0;
+}
+
+int ora_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ora_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
 }
 
 int ora_spike_counts(const ora_sim* s, int kind, int* counts)

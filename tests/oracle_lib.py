@@ -74,6 +74,13 @@ def oracle():
         lib.ora_num_steps.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.ora_poisson_u32.restype = ctypes.c_uint32
         lib.ora_poisson_u32.argtypes = [ctypes.c_uint32] * 6
+        lib.ora_set_probe.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        lib.ora_update_synapse.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        lib.ora_axon_remodeling.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.ora_threads.restype = ctypes.c_int
+        lib.ora_threads.argtypes = []
+        lib.ora_set_threads.restype = None
+        lib.ora_set_threads.argtypes = [ctypes.c_int]
         _ora = lib
     return _ora
 

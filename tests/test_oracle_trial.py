@@ -9,7 +9,11 @@ is pinned here by
     desaturation, axon remodeling, LTD on delivery, LTP at sync);
   * closed-form known answers of LTP / LTD (NGS:16615-16652);
   * invariants (a network without synapses behaves as isolated neurons; a
-    chain test leaves synapses untouched; zero-delay equals immediate delivery).
+    chain test leaves synapses untouched; the threaded neuron steps equal one
+    thread).
+Known answers derived by hand for tiny networks (arrival steps, zero delay,
+sync application, every STDP branch, update_synapse, remodeling, decay) are in
+test_oracle_pins.py.
 """
 import ctypes
 
@@ -207,3 +211,25 @@ def test_chain_trial_leaves_synapses_unchanged():
     before = a.synapses()
     a.run_trial(mode=1, trial_index=1)
     _cmp_syn(before, a.synapses())
+
+
+def test_threaded_neuron_steps_equal_one_thread():
+    """Above 512 neurons the oracle steps its neurons on OpenMP threads before
+    the step's deliveries; the trial must equal the one-thread run bit for bit."""
+    from unsupervised_growth_amd.configs import sphere_network
+    net = sphere_network(np.random.default_rng(2), 700, 190, 5.0, k_rr=30, n_train=12, spread_max=5.0)
+    lib = oracle()
+    threads = lib.ora_threads()
+    out = []
+    for n in (1, max(threads, 2)):
+        lib.ora_set_threads(n)
+        sim = OracleSim(net, trial_ms=40.0, seed=(4, 2))
+        r = [sim.run_trial(mode=0, trial_index=k, first_trial=(k == 0), mean_injection_time=10.0) for k in range(2)]
+        out.append((r, sim.synapses()))
+        sim.close()
+    lib.ora_set_threads(threads)
+    (ra, sa), (rb, sb) = out
+    assert sum(map(len, ra[1]["soma"])) > 0
+    for x, y in zip(ra, rb):
+        _cmp_trial(x, y)
+    _cmp_syn(sa, sb)
