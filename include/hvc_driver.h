@@ -38,7 +38,10 @@ typedef struct {
     double E_REST_MATURE, E_REST_IMMATURE;
     double GCA_MATURE, GCA_IMMATURE;
     double DEATH_RATE_THRESHOLD;
-    int RATE_WINDOW_LONG;       /* capacity of the activity history (>= 1000, NGS:9586) */
+    int RATE_WINDOW_LONG;       /* capacity of the activity history (>= the activity window) */
+    int ACTIVITY_WINDOW;        /* trials the activity check reads, and the trials after replacement
+                                   before a neuron may be replaced again: 1000 in the reference
+                                   (NGS:9586, 9607); 0 = 1000 */
 } hvc_maturation_params;
 
 /* Positions of the neurons and the distance unit: what the reference keeps in
@@ -146,15 +149,17 @@ void hvc_chain_stats_destroy(hvc_chain_stats* s);
 /* The bookkeeping steps on host arrays, as the driver runs them.
  * history: [n][window] ring shared by all neurons, logical entry 0 = newest at
  * index *head; a push writes at (*head - 1) mod window.  check_neuron_activity
- * (NGS:9573-9627): push this trial's somatic spike count; robustness = fraction
- * of the last 1000 trials with a spike; >= 0.5 switches the scale to driven
- * (1000); < death_rate_threshold after more than 1000 trials since replacement
- * marks the neuron for replacement (to_replace, *n_replace, ascending).
+ * (NGS:9573-9627) with an activity window A (1 <= A <= window; the reference's
+ * is 1000): push this trial's somatic spike count; robustness = fraction of the
+ * last A trials with a spike; >= 0.5 switches the scale to driven (1000);
+ * < death_rate_threshold after more than A trials since replacement marks the
+ * neuron for replacement (to_replace, *n_replace, ascending).
  * trials_after_replacement is read as already advanced for this trial
  * (NGS:5831-5832).  recent (may be NULL): per neuron, the number of the newest
- * 1000 history entries that are > 0, kept up to date across calls so that a
- * call costs O(n) instead of O(1000 n); NULL counts from the history. */
-hvc_status hvc_activity_update(int n, int window, int* history, int* head, int* recent, const int* soma_counts,
+ * A history entries that are > 0, kept up to date across calls so that a call
+ * costs O(n) instead of O(A n); NULL counts from the history. */
+hvc_status hvc_activity_update(int n, int window, int activity_window, int* history, int* head, int* recent,
+                               const int* soma_counts,
                                const int* trials_after_replacement, int* maturation_scale,
                                double death_rate_threshold, int* to_replace, int* n_replace, int* became_driven);
 

@@ -201,18 +201,19 @@ def generate_topology(n_ra, n_i, n_tr, p, seed):
 
 
 # ---------------------------------------------------------------- bookkeeping
-def activity_update(history, counts, trials_after, scale, death_rate_threshold):
-    """check_neuron_activity (NGS:9573-9627).  history: list of deques (newest
-    first, maxlen RATE_WINDOW_LONG, kept full as boost::circular_buffer is);
-    push_front drops the oldest.  Returns (to_replace, newly driven)."""
+def activity_update(history, counts, trials_after, scale, death_rate_threshold, window=1000):
+    """check_neuron_activity (NGS:9573-9627) with its window of 1000 trials
+    (NGS:9586) as a parameter.  history: list of deques (newest first, maxlen
+    RATE_WINDOW_LONG, kept full as boost::circular_buffer is); push_front drops
+    the oldest.  Returns (to_replace, newly driven)."""
     to_replace, driven = [], 0
     for i in range(len(counts)):
         history[i].appendleft(int(counts[i]))
-        robustness = sum(1.0 for j in range(1000) if history[i][j] > 0) / 1000.0
+        robustness = sum(1.0 for j in range(window) if history[i][j] > 0) / float(window)
         if robustness >= 0.5 and scale[i] != DRIVEN:
             scale[i] = DRIVEN
             driven += 1
-        if robustness < death_rate_threshold and trials_after[i] > 1000:
+        if robustness < death_rate_threshold and trials_after[i] > window:
             to_replace.append(i)
     return to_replace, driven
 

@@ -293,3 +293,49 @@ def test_chain_statistics_equal_restatement(tmp_path):
         cs.add([np.array([t, t + 50])], [np.zeros(0)])
     _, a = cs.array()
     assert a[0, :4].tolist() == [1.0, 2.0, 12.0, 2.0]
+
+
+def test_activity_update_short_window_closed_forms():
+    """A 25-trial activity window (the 2 000 + 550 workload replaces silent
+    neurons every 25 trials): driven at >= 13 of the last 25 trials with a
+    spike (12.5 is half), silent neurons replaced only after more than 25
+    trials since their last replacement; the ring form with running counts
+    equals the restatement's deque form over 120 trials."""
+    from unsupervised_growth_amd.driver import activity_update
+    A, W = 25, 60
+    hist = np.zeros((4, W), np.int32)
+    hist[0, :12] = 1       # 12 + this trial = 13 of 25 -> driven
+    hist[1, :11] = 1       # 11 + 1 = 12 -> not driven
+    hist[2, 24:] = 5       # only entries older than the window: silent
+    scale = np.full(4, og.SPONTANEOUS, np.int32)
+    head, rep, nd = activity_update(hist, 0, [1, 1, 0, 0], [5, 5, 26, 25], scale, 0.01, activity_window=A)
+    assert scale.tolist() == [og.DRIVEN, og.SPONTANEOUS, og.SPONTANEOUS, og.SPONTANEOUS] and nd == 1
+    assert rep.tolist() == [2]                 # 26 > 25 trials since replacement; neuron 3 has 25
+    with pytest.raises(ValueError):
+        activity_update(np.zeros((1, 24), np.int32), 0, [0], [0], np.full(1, 1000, np.int32), 0.01,
+                        activity_window=A)
+    rng = np.random.default_rng(3)
+    n = 30
+    rates = np.where(np.arange(n) < 6, 0.0, rng.uniform(0.05, 0.9, n))
+    hist = np.zeros((n, W), np.int32)
+    recent = np.zeros(n, np.int32)
+    scale = np.full(n, og.SPONTANEOUS, np.int32)
+    oh = [deque([0] * W, maxlen=W) for _ in range(n)]
+    oscale = [og.SPONTANEOUS] * n
+    after = np.zeros(n, np.int32)
+    head, seen = 0, 0
+    for t in range(120):
+        after += 1
+        counts = (rng.random(n) < rates) * rng.integers(1, 3, n)
+        head, rep, nd = activity_update(hist, head, counts, after, scale, 0.01, recent=recent, activity_window=A)
+        orep, ond = og.activity_update(oh, counts, after.tolist(), oscale, 0.01, window=A)
+        assert rep.tolist() == orep and nd == ond, t
+        seen += len(rep)
+        after[rep] = 0
+        hist[rep] = 0
+        recent[rep] = 0
+        for r in rep:
+            oh[r].extend([0] * W)
+    np.testing.assert_array_equal(scale, oscale)
+    np.testing.assert_array_equal(recent, (_ring_logical(hist, head)[:, :A] > 0).sum(1))
+    assert seen >= 6 * 4          # the silent neurons are replaced every 26 trials

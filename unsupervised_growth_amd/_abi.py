@@ -123,7 +123,8 @@ class Topology(ctypes.Structure):
 class MaturationParamsC(ctypes.Structure):
     _fields_ = [("E_REST_MATURE", ctypes.c_double), ("E_REST_IMMATURE", ctypes.c_double),
                 ("GCA_MATURE", ctypes.c_double), ("GCA_IMMATURE", ctypes.c_double),
-                ("DEATH_RATE_THRESHOLD", ctypes.c_double), ("RATE_WINDOW_LONG", ctypes.c_int)]
+                ("DEATH_RATE_THRESHOLD", ctypes.c_double), ("RATE_WINDOW_LONG", ctypes.c_int),
+                ("ACTIVITY_WINDOW", ctypes.c_int)]
 
 
 class Geometry(ctypes.Structure):
@@ -221,7 +222,8 @@ def load(path=LIB_PATH):
     lib.hvc_driver_last_error.restype = ctypes.c_char_p
     lib.hvc_driver_destroy.argtypes = [vp]
     lib.hvc_driver_destroy.restype = None
-    lib.hvc_activity_update.argtypes = [ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+    lib.hvc_activity_update.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, c_int_p, c_int_p, c_int_p, c_int_p,
+                                        c_int_p,
                                         c_int_p, ctypes.c_double, c_int_p, c_int_p, c_int_p]
     lib.hvc_maturation_update.argtypes = [ctypes.c_int, P(MaturationParamsC), c_int_p, c_int_p, c_double_p,
                                           c_double_p, c_int_p]

@@ -51,7 +51,8 @@ struct hvc_driver {
     std::vector<int> mature, scale, trials_after;
     std::vector<int> history;   // [n_ra][W] ring, logical entry 0 at head
     int head = 0;
-    std::vector<int> recent;    // per neuron: trials with a spike among the newest 1000 of history
+    std::vector<int> recent;    // per neuron: trials with a spike among the newest activity-window entries
+    int activity_window = 1000; // NGS:9586
     std::vector<double> ra_xyz, i_xyz;
     double mid = 0;
     RefRng rng{0};
@@ -665,14 +666,14 @@ hvc_status write_initial(hvc_driver* d, const std::string& dir)
 // ---------------------------------------------------------------------------
 // bookkeeping on host arrays
 // ---------------------------------------------------------------------------
-extern "C" hvc_status hvc_activity_update(int n, int window, int* history, int* head, int* recent,
-                                          const int* soma_counts, const int* trials_after_replacement,
+extern "C" hvc_status hvc_activity_update(int n, int window, int activity_window, int* history, int* head,
+                                          int* recent, const int* soma_counts, const int* trials_after_replacement,
                                           int* maturation_scale, double death_rate_threshold, int* to_replace,
                                           int* n_replace, int* became_driven)
 {
-    const int kWindow = 1000;           // NGS:9586
-    const double kDriven = 0.50;        // NGS:9588
-    if (n < 0 || window < kWindow || !history || !head || *head < 0 || *head >= window ||
+    const int kWindow = activity_window;   // 1000 in the reference (NGS:9586)
+    const double kDriven = 0.50;           // NGS:9588
+    if (n < 0 || kWindow < 1 || window < kWindow || !history || !head || *head < 0 || *head >= window ||
         (n > 0 && (!soma_counts || !trials_after_replacement || !maturation_scale || !to_replace)) || !n_replace)
         return HVC_E_ARG;
     const int h0 = *head;
@@ -683,7 +684,7 @@ extern "C" hvc_status hvc_activity_update(int n, int window, int* history, int* 
         int* row = history + static_cast<size_t>(i) * window;
         int fired = 0;
         if (recent) {
-            // the entry 999 trials back leaves the 1000-trial window, the new count enters
+            // the entry kWindow - 1 trials back leaves the window, the new count enters
             fired = recent[i] - (row[(h0 + kWindow - 1) % window] > 0 ? 1 : 0) + (soma_counts[i] > 0 ? 1 : 0);
             row[h] = soma_counts[i];
             recent[i] = fired;
@@ -739,8 +740,9 @@ extern "C" hvc_status hvc_driver_create(const hvc_params* params, int device, co
     if (!d) return HVC_E_OOM;
     *out = d;
     if (net->n_ra <= 0 || net->n_i < 0) return fail(d, HVC_E_ARG, "network needs HVC-RA neurons");
-    if (maturation->RATE_WINDOW_LONG < 1000)
-        return fail(d, HVC_E_ARG, "RATE_WINDOW_LONG must be >= 1000: the activity check reads the last 1000 trials (NGS:9586-9599)");
+    const int act_window = maturation->ACTIVITY_WINDOW > 0 ? maturation->ACTIVITY_WINDOW : 1000;
+    if (maturation->RATE_WINDOW_LONG < act_window)
+        return fail(d, HVC_E_ARG, "RATE_WINDOW_LONG must be >= the activity window (1000 in the reference, NGS:9586-9599)");
     if (!geometry->ra_xyz || (net->n_i > 0 && !geometry->i_xyz) || !(geometry->model_interneuron_distance > 0))
         return fail(d, HVC_E_ARG, "geometry needs positions and a model interneuron distance > 0");
     if (!(params->trial_ms > 2 * kWaitingTime)) return fail(d, HVC_E_ARG, "trial must be longer than 2 x WAITING_TIME (NGS:8898)");
@@ -789,9 +791,10 @@ extern "C" hvc_status hvc_driver_create(const hvc_params* params, int device, co
         d->trials_after.assign(resume->trials_after_replacement, resume->trials_after_replacement + nra);
     if (resume && resume->history) d->history.assign(resume->history, resume->history + static_cast<size_t>(nra) * W);
     d->trial_number = resume ? resume->trial_number : 0;
+    d->activity_window = act_window;
     d->recent.assign(nra, 0);
     for (int i = 0; i < nra; ++i)
-        for (int j = 0; j < 1000; ++j) d->recent[i] += d->history[static_cast<size_t>(i) * W + j] > 0 ? 1 : 0;
+        for (int j = 0; j < act_window; ++j) d->recent[i] += d->history[static_cast<size_t>(i) * W + j] > 0 ? 1 : 0;
     for (int i = 0; i < nra; ++i)
         if (d->scale[i] == 0) return fail(d, HVC_E_ARG, "maturation scale 0");
     d->rng = RefRng(seed);
@@ -866,7 +869,8 @@ extern "C" hvc_status hvc_driver_trial(hvc_driver* d, int training, hvc_driver_r
     for (int& a : d->trials_after) a += 1;
     std::vector<int> to_replace(d->n_ra);
     int n_replace = 0;
-    r = hvc_activity_update(d->n_ra, d->mat.RATE_WINDOW_LONG, d->history.data(), &d->head, d->recent.data(),
+    r = hvc_activity_update(d->n_ra, d->mat.RATE_WINDOW_LONG, d->activity_window, d->history.data(), &d->head,
+                            d->recent.data(),
                             d->cnt[0].data(), d->trials_after.data(), d->scale.data(), d->mat.DEATH_RATE_THRESHOLD, to_replace.data(),
                             &n_replace, &rep.became_driven);
     if (r != HVC_OK) return fail(d, r, "activity update");

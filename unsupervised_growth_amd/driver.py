@@ -20,11 +20,15 @@ from .network import MaturationParams, Network
 from .topology import TopologyParams
 
 
-def _mat_c(m: MaturationParams):
+ACTIVITY_WINDOW = 1000   # trials check_neuron_activity reads (NetworkGrowthSimulator.cpp:9586)
+
+
+def _mat_c(m: MaturationParams, activity_window=ACTIVITY_WINDOW):
     c = _abi.MaturationParamsC()
     for k in ("E_REST_MATURE", "E_REST_IMMATURE", "GCA_MATURE", "GCA_IMMATURE", "DEATH_RATE_THRESHOLD"):
         setattr(c, k, float(getattr(m, k)))
     c.RATE_WINDOW_LONG = int(m.RATE_WINDOW_LONG)
+    c.ACTIVITY_WINDOW = int(activity_window)
     return c
 
 
@@ -49,11 +53,15 @@ class GrowthDriver:
     ra_xyz, i_xyz: positions (N x 3); model_interneuron_distance: the distance
     unit of the positions; topology: resampling parameters; resume: optional
     dict with mature, maturation_scale, trials_after_replacement, history
-    (N x RATE_WINDOW_LONG, newest first) and trial_number.
+    (N x RATE_WINDOW_LONG, newest first) and trial_number; activity_window:
+    the trials the activity check reads and the trials after a replacement
+    before the neuron may be replaced again (the reference's 1000; at most
+    RATE_WINDOW_LONG).
     """
 
     def __init__(self, net: Network, ra_xyz, i_xyz, model_interneuron_distance, topology: TopologyParams,
-                 maturation: MaturationParams = None, resume=None, seed=15000, params=None, device=0, **kw):
+                 maturation: MaturationParams = None, resume=None, seed=15000, params=None, device=0,
+                 activity_window=ACTIVITY_WINDOW, **kw):
         self.lib = _abi.load()
         self.params = params if params is not None else make_params(**kw)
         self.maturation = maturation or MaturationParams()
@@ -64,7 +72,8 @@ class GrowthDriver:
         geo = _abi.Geometry(_p(self._xyz[0], _abi.c_double_p), _p(self._xyz[1], _abi.c_double_p),
                             float(model_interneuron_distance))
         top = topology.c()
-        mat = _mat_c(self.maturation)
+        self.activity_window = int(activity_window)
+        mat = _mat_c(self.maturation, self.activity_window)
         res = None
         if resume is not None:
             self._res = {k: _i32(np.asarray(resume[k]).reshape(-1)) if resume.get(k) is not None else None
@@ -205,12 +214,12 @@ class GrowthDriver:
 
 
 def activity_update(history, head, soma_counts, trials_after_replacement, maturation_scale,
-                    death_rate_threshold, recent=None):
+                    death_rate_threshold, recent=None, activity_window=ACTIVITY_WINDOW):
     """check_neuron_activity (NGS:9573-9627) on host arrays; history is the ring
     [n][window] with logical entry 0 at `head`; recent (optional, int32 per
-    neuron) the running count of spiking trials among the newest 1000.  Updates
-    history, recent and maturation_scale in place; returns (new head, neurons
-    to replace, driven count)."""
+    neuron) the running count of spiking trials among the newest
+    activity_window.  Updates history, recent and maturation_scale in place;
+    returns (new head, neurons to replace, driven count)."""
     lib = _abi.load()
     n, window = history.shape
     assert history.dtype == np.int32 and history.flags.c_contiguous
@@ -219,7 +228,8 @@ def activity_update(history, head, soma_counts, trials_after_replacement, matura
     h = ctypes.c_int(head)
     to_rep = np.zeros(max(n, 1), np.int32)
     nr, nd = ctypes.c_int(), ctypes.c_int()
-    st = lib.hvc_activity_update(n, window, _p(history, _abi.c_int_p), ctypes.byref(h), _p(recent, _abi.c_int_p),
+    st = lib.hvc_activity_update(n, window, int(activity_window), _p(history, _abi.c_int_p), ctypes.byref(h),
+                                 _p(recent, _abi.c_int_p),
                                  _p(_i32(soma_counts), _abi.c_int_p), _p(_i32(trials_after_replacement), _abi.c_int_p),
                                  _p(maturation_scale, _abi.c_int_p), float(death_rate_threshold),
                                  _p(to_rep, _abi.c_int_p), ctypes.byref(nr), ctypes.byref(nd))
