@@ -35,6 +35,10 @@ FP32_PEAK_TFLOPS = 157.3
 # WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
 # profiles/integrate_c3_hbm_traffic.txt; keyed by (config, precision, HVC-I layout)
 TRAFFIC_PROFILED = {("c3", 1, 2): 7.167e6, ("c3", 0, 2): 7.782e6}
+# HBM bytes per launch of the delivery kernel (same passes), keyed by config
+DELIVER_TRAFFIC_PROFILED = {"c3": 4.26e6}
+DELIVER_TRAFFIC_NOTE = "profiles/integrate_c3_hbm_traffic.txt"
+HBM_PEAK_GBPS = 8000.0
 
 
 def parse():
@@ -46,9 +50,11 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--ra-model", type=int, default=1, help="0 Debraband (drop-in), 1 RK4 (HH2_buffer)")
     ap.add_argument("--i-model", type=int, default=0, help="0 DP8, 1 RK4")
-    ap.add_argument("--cpu-sample-ms", type=float, default=5.0,
+    ap.add_argument("--cpu-sample-ms", type=float, default=50.0,
                     help="trial length of the CPU-baseline sample (same network)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--dropin-trials", type=int, default=2,
+                    help="timed trials of the drop-in pairing (Debraband + DP8) in FP32 and FP64; 0 skips")
     ap.add_argument("--nets", type=int, default=64, help="c4: number of networks")
     ap.add_argument("--check-nets", type=int, default=4, help="c4: networks re-run alone for comparison")
     ap.add_argument("--trial-ms", type=float, default=None, help="trial length (default: the config's)")
@@ -69,20 +75,29 @@ def parse():
 
 
 def cpu_baseline(net, kw, args):
-    """The oracle (C restatement, -O2, one thread) on the same network, shortened trial."""
+    """The oracle (C restatement of the trial, -O2) on the host cores: the neuron
+    steps of each time step on OpenMP threads (every core the process may use),
+    deliveries and STDP in the reference's order on one thread; the same
+    network, a shortened growth trial whose training kick falls at a fifth of
+    it (as 100 ms does in a 500 ms trial), after an untimed first trial."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import OracleSim
+    from oracle_lib import OracleSim, oracle
     sample_ms = args.cpu_sample_ms
     sim = OracleSim(net, ra_model=args.ra_model, i_model=args.i_model, trial_ms=sample_ms,
                     seed=kw.get("seed", (1, 0)))
-    steps = int(sample_ms / 0.02) - 1
+    steps = int(round(sample_ms / 0.02)) - 1
+    sim.run_trial(mode=0, training=True, trial_index=0, first_trial=True, mean_injection_time=0.2 * sample_ms)
     t = time.perf_counter()
-    sim.run_trial(mode=0, training=True, trial_index=0, first_trial=True, mean_injection_time=100.0)
+    sim.run_trial(mode=0, training=True, trial_index=1, first_trial=False, mean_injection_time=0.2 * sample_ms)
     el = time.perf_counter() - t
     sim.close()
-    return {"value": (net.n_ra + net.n_i) * steps / el, "unit": "neuron-updates/s", "cores": 1, "kind": "port",
-            "sample": "oracle growth trial of %g ms (%d steps) on the same %d RA + %d I network, %.1f s wall"
-                      % (sample_ms, steps, net.n_ra, net.n_i, el)}
+    threads = oracle().ora_threads()
+    return {"value": (net.n_ra + net.n_i) * steps / el, "unit": "neuron-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle growth trial of %g ms (%d steps, kick at %g ms) on the same %d RA + %d I network, "
+                      "%.2f s wall; neuron steps on %d OpenMP threads (%d host CPUs visible), delivery and STDP "
+                      "on one thread, -O2" % (sample_ms, steps, 0.2 * sample_ms, net.n_ra, net.n_i, el, threads,
+                                              os.cpu_count() or 0)}
 
 
 def bench_driver(net, kw, args, precision, tuning):
@@ -109,6 +124,55 @@ def bench_driver(net, kw, args, precision, tuning):
             "spikes_per_trial": sum(r.soma_spikes + r.i_spikes for r in reps) / len(reps),
             "note": "growth loop through the C++ driver (hvc_driver_trial): trial with a random injection time, "
                     "spike read-back, check_neuron_activity, maturation update and the Erest/GCa patch"}
+
+
+def deliver_bytes(net, results, windows, avg_ms, config):
+    """The delivery kernel against its bytes: algorithmic bytes per launch (every
+    edge a spike delivers: 4 B target + 8 B delay + 8 B weight read, 16 B inbox
+    record written) from the timed trials' spikes and the network's out-degrees
+    (unsaturated RA->RA rows deliver to every candidate, NetworkGrowthSimulator.cpp:9010-9030),
+    and the HBM bytes rocprofv3 counted per launch for this workload."""
+    import numpy as np
+    ra_deg = np.diff(net.ra2i_ptr) + np.diff(net.rr_ptr)
+    i_deg = np.diff(net.i2ra_ptr)
+    edges = 0
+    for r in results:
+        edges += int(np.dot(np.asarray([len(x) for x in r.soma]), ra_deg))
+        edges += int(np.dot(np.asarray([len(x) for x in r.interneuron]), i_deg))
+    alg = 36.0 * edges / (len(results) * windows)
+    meas = DELIVER_TRAFFIC_PROFILED.get(config)
+    s = avg_ms / 1000.0
+    out = {"avg_launch_ms": avg_ms, "delivered_edges_per_trial": edges / len(results),
+           "algorithmic_bytes_per_launch": alg, "algorithmic_GBps": alg / s / 1e9 if s > 0 else None,
+           "hbm_bytes_per_launch": meas, "hbm_GBps": meas / s / 1e9 if (meas and s > 0) else None,
+           "peak_GBps": HBM_PEAK_GBPS}
+    if meas:
+        out["note"] = ("HBM bytes from rocprofv3 FETCH_SIZE / WRITE_SIZE per k_deliver dispatch, scaled by a kernel "
+                       "of known traffic: %s" % DELIVER_TRAFFIC_NOTE)
+    return out
+
+
+def bench_dropin(net, kw, args, tuning):
+    """The drop-in pairing of the reference's drivers (HH2_test Debraband for
+    HVC-RA, HHI_buffer DP8 for HVC-I; NetworkGrowthSimulator.cpp:8943, 9223) on
+    the same network, in FP32 and FP64: growth trials per second."""
+    from unsupervised_growth_amd.engine import Engine
+    out = {}
+    steps = int(round(kw["trial_ms"] / 0.02)) - 1
+    for name, prec in (("fp32", 1), ("fp64", 0)):
+        with Engine(ra_model=0, i_model=0, precision=prec, **kw) as e:
+            e.upload(net)
+            e.set_tuning(**dict(tuning, ra_lanes=0))
+            e.run_trial(mode=0, training=True, trial_index=0, first_trial=True, mean_injection_time=100.0)
+            wall = 0.0
+            for k in range(args.dropin_trials):
+                t0 = time.perf_counter()
+                e.run_trial(mode=0, training=True, trial_index=1 + k, mean_injection_time=100.0)
+                wall += time.perf_counter() - t0
+        out[name] = {"value": (net.n_ra + net.n_i) * steps * args.dropin_trials / wall, "unit": "neuron-updates/s",
+                     "ms_per_step": 1000.0 * wall / args.dropin_trials, "trials": args.dropin_trials}
+    out["note"] = "HVC-RA HH2_test::Debraband_step + HVC-I HHI_buffer::DP8_step (what the reference's drivers run)"
+    return out
 
 
 def apply_shape(eng, args):
@@ -195,8 +259,8 @@ def bench_batch(args):
         precision_check = dict(reference="FP64 batched form (bit-exact with the oracle), same trial", dt_ms=0.02, **tot)
 
     windows = int(round(kw["trial_ms"] / 0.25))
-    ra_flops = opcount.flops_per_step("ra", args.ra_model)
-    i_flops = opcount.flops_per_step("i", args.i_model)
+    ra_flops = opcount.flops_per_step("ra", args.ra_model, precision)
+    i_flops = opcount.flops_per_step("i", args.i_model, precision)
     flops_launch = sum(ra_flops * n.n_ra + i_flops * n.n_i for n in nets) * steps_per_trial / windows
     peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
     achieved = flops_launch / (kern["integrate"] / 1000.0) / 1e12 if kern["integrate"] > 0 else 0.0
@@ -213,7 +277,7 @@ def bench_batch(args):
                                   min(n.n_ra for n in nets), max(n.n_ra for n in nets), kw["trial_ms"],
                                   "RK4-3/8" if args.ra_model else "Debraband"),
                    "config": "c4", "network_build_s": round(t_build, 1)},
-        "roofline": {"bound": "mfma", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "flops_per_launch": flops_launch,
                      "avg_launch_ms": kern["integrate"],
                      "kernels_ms_per_trial": {k: v * (1 if k in ("decay", "sync") else windows)
@@ -358,24 +422,26 @@ def main():
     # integrate and deliver run once per window; sync (the last window's) and decay once per trial
     per_trial = {k: v["avg_ms"] * (1 if k in ("decay", "sync") else windows) for k, v in kern.items()}
     dom = max(per_trial, key=per_trial.get)
-    ra_flops = opcount.flops_per_step("ra", args.ra_model)
-    i_flops = opcount.flops_per_step("i", args.i_model)
+    ra_flops = opcount.flops_per_step("ra", args.ra_model, precision)
+    i_flops = opcount.flops_per_step("i", args.i_model, precision)
     steps_per_window = steps_per_trial / windows
     flops_launch = (ra_flops * net.n_ra + i_flops * net.n_i) * steps_per_window
     avg_s = kern["integrate"]["avg_ms"] / 1000.0
     peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
     achieved = flops_launch / avg_s / 1e12 if avg_s > 0 else 0.0
     traffic = TRAFFIC_PROFILED.get((args.config, precision, tuning["i_lanes"]))
-    roofline = {"bound": "mfma", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+    roofline = {"bound": "valu", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic,
                 "traffic_note": None if traffic is None else "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
-                                "(scale x2.0 / x1.0 from a kernel of known traffic), profiles/integrate_c3_hbm_traffic.txt; "
+                                "(scaled by a kernel of known traffic), profiles/integrate_c3_hbm_traffic.txt; "
                                 "algorithmic state traffic ~7.5 MB per launch",
-                "note": "VALU-bound (no dense contraction); peak = MI355X %s vector rate; achieved = "
-                        "algorithmic flops per launch (opcount: %d per RA step, %d per I step, exp = 20) / "
-                        "HIP-event launch duration" % ("FP64" if precision == 0 else "FP32", ra_flops, i_flops),
+                "note": "vector-ALU bound (no dense contraction, no MFMA); peak = MI355X %s vector rate; achieved = "
+                        "algorithmic flops per launch (opcount: %d per RA step, %d per I step, one exp = %d) / "
+                        "HIP-event launch duration" % ("FP64" if precision == 0 else "FP32", ra_flops, i_flops,
+                                                       opcount.FLOPS_PER_EXP[precision]),
                 "flops_per_launch": flops_launch, "avg_launch_ms": kern["integrate"]["avg_ms"],
                 "dominant": dom, "kernels_ms_per_trial": per_trial}
+    deliver = deliver_bytes(net, results, windows, kern["deliver"]["avg_ms"], args.config)
 
     # FP32 form: each timed trial again in the FP64 form (bit-exact with the
     # oracle) from the same synapse state; per-trial spike-time error
@@ -423,9 +489,12 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    line["deliver"] = deliver
     if precision_check is not None:
         line["precision_check"] = precision_check
         line["fp64_form"] = fp64_form
+    if args.dropin_trials > 0:
+        line["dropin"] = bench_dropin(net, kw, args, tuning)
     if growth_loop is not None:
         line["growth_loop"] = growth_loop
     print(json.dumps(line), flush=True)

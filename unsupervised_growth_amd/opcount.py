@@ -5,23 +5,26 @@ The counts are the floating-point operations one neuron-step performs:
 additions, subtractions, multiplications and divisions of the stage and RHS
 evaluations, and calls of exp / log.  The conductance decay factors shared by
 all neurons come from a per-step table and are not counted per neuron.
-``FLOPS_PER_EXP`` is the operation count of the engine's exp (hvc_math.hip.h:
-1 mul for the reduction argument, rint, 2 fma reduction, 1 mul r^2, 5 fma
-polynomial, 1 fma + 1 add reconstruction, 1 mul scale; fma = 2 flops).
+``FLOPS_PER_EXP`` is the operation count of one exp in each precision form:
+FP64 (precision 0) 20, the engine's own exp (hvc_math.hip.h: 1 mul for the
+reduction argument, rint, 2 fma reduction, 1 mul r^2, 5 fma polynomial, 1 fma
++ 1 add reconstruction, 1 mul scale; fma = 2 flops); FP32 (precision 1) 1,
+the hardware exp2 (v_exp_f32) -- the argument's scaling by log2(e) / B is the
+expression's division by B, counted with the divisions.
 
-``flops_per_step(kind, model)`` is what bench.py uses for the roofline.
+``flops_per_step(kind, model, precision)`` is what bench.py uses for the roofline.
 """
 import math
 
-FLOPS_PER_EXP = 20
+FLOPS_PER_EXP = {0: 20, 1: 1}
 
 
 class Counter:
     def __init__(self):
         self.n = {"add": 0, "mul": 0, "div": 0, "exp": 0}
 
-    def flops(self):
-        return self.n["add"] + self.n["mul"] + self.n["div"] + FLOPS_PER_EXP * self.n["exp"]
+    def flops(self, precision=0):
+        return self.n["add"] + self.n["mul"] + self.n["div"] + FLOPS_PER_EXP[precision] * self.n["exp"]
 
 
 class CF:
@@ -239,13 +242,13 @@ def count_i_step(model):
     return cnt
 
 
-def flops_per_step(kind, model):
-    """FP64/FP32 operations of one neuron-step; kind 'ra' or 'i'."""
+def flops_per_step(kind, model, precision=0):
+    """Operations of one neuron-step in a precision form (0 FP64, 1 FP32); kind 'ra' or 'i'."""
     cnt = count_ra_step(model) if kind == "ra" else count_i_step(model)
-    return cnt.flops()
+    return cnt.flops(precision)
 
 
 if __name__ == "__main__":
     for kind, model, name in (("ra", 0, "RA Debraband"), ("ra", 1, "RA RK4"), ("i", 0, "I DP8"), ("i", 1, "I RK4")):
         c = count_ra_step(model) if kind == "ra" else count_i_step(model)
-        print("%-13s %s flops=%d" % (name, c.n, c.flops()))
+        print("%-13s %s flops FP64=%d FP32=%d" % (name, c.n, c.flops(0), c.flops(1)))
