@@ -168,7 +168,8 @@ hvc_status hvc_batch_upload_synapses(hvc_ctx* ctx, int net, const hvc_synapses* 
  * reference. */
 typedef struct {
     int integrate_block;   /* threads per block of the integration kernel: 64, 128, 256 */
-    int i_lanes;           /* HVC-I DP8: 2 = four lanes per neuron, 1 = five lanes, 0 = one thread per neuron */
+    int i_lanes;           /* HVC-I DP8: 3 = eight lanes per neuron, 2 = four lanes, 1 = five lanes,
+                              0 = one thread per neuron */
     int deliver_block;     /* 64, 128, 256 */
     int sync_block;        /* 64 .. 512 */
     int ra_lanes;          /* HVC-RA RK4: 1 = four lanes per neuron, 0 = one thread per neuron */

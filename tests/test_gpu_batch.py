@@ -73,7 +73,9 @@ def test_launch_shapes_do_not_change_results(precision):
               dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512, ra_lanes=0),
               dict(integrate_block=128, i_lanes=2, deliver_block=256, sync_block=128, ra_lanes=0),
               dict(integrate_block=256, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1),
-              dict(integrate_block=64, i_lanes=2, deliver_block=128, sync_block=512, ra_lanes=1)]
+              dict(integrate_block=64, i_lanes=2, deliver_block=128, sync_block=512, ra_lanes=1),
+              dict(integrate_block=128, i_lanes=3, deliver_block=64, sync_block=256, ra_lanes=1),
+              dict(integrate_block=256, i_lanes=3, deliver_block=128, sync_block=256, ra_lanes=0)]
     runs = []
     for shape in shapes:
         # the RK4 form for HVC-RA, where four lanes per neuron exist
@@ -104,7 +106,7 @@ def test_autotune_keeps_a_valid_shape_and_results():
         print("autotune:", shape, ms, flush=True)
         assert shape["integrate_block"] in (64, 128, 256) and shape["deliver_block"] in (64, 128, 256)
         assert shape["sync_block"] in (128, 256, 512) and np.all(ms > 0)
-        assert shape["i_lanes"] in (0, 1, 2) and shape["ra_lanes"] in (0, 1)
+        assert shape["i_lanes"] in (0, 1, 2, 3) and shape["ra_lanes"] in (0, 1)
         again = e.run_batch([_trial(b, 0, mode=1) for b in range(len(nets))])
     for b in range(len(nets)):
         for kind in ("soma", "dend", "interneuron"):

@@ -71,11 +71,12 @@ def _compare_synapses(syn, osyn):
 
 
 @pytest.mark.parametrize("rr,ra_model,ra_lanes,i_lanes", [("dense", 0, 0, 1), (12, 0, 0, 2), ("dense", 1, 1, 2),
-                                                             (12, 1, 0, 1), (12, 1, 1, 0)])
+                                                             (12, 1, 0, 1), (12, 1, 1, 0), ("dense", 0, 0, 3),
+                                                             (12, 1, 1, 3)])
 def test_growth_trials_bit_exact(rr, ra_model, ra_lanes, i_lanes):
     """Full growth trials in every integration layout: HVC-RA (RK4 form) with
-    four lanes or one thread per neuron, HVC-I (DP8) with four or five lanes or
-    one thread per neuron."""
+    four lanes or one thread per neuron, HVC-I (DP8) with four, five or eight
+    lanes or one thread per neuron."""
     net = _small_net(rr=rr)
     trial_ms = 300.0
     print("growth rr=%s ra_model=%d ra_lanes=%d i_lanes=%d n_ra=%d n_i=%d trial_ms=%g" % (

@@ -261,10 +261,10 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
     HVC_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     // this device's fixed buffer of hvc_trace_ra_noalloc (looked up once per device)
     HVC_TRY(noalloc_setup(device));
-    // default launch shape for one network: HVC-I in quads in the FP32 form
-    // (the DPP exchange shortens DP8's dependent chain), five lanes in FP64
-    // (there the extra gate per lane costs more than the exchange saves)
-    ctx->tuning.i_lanes = 2;   // four lanes per HVC-I neuron: the fastest single-network layout in both forms
+    // default launch shape for one network: eight lanes per HVC-I neuron (one
+    // gate rate and one stage combination per lane and DP8 stage: the shortest
+    // dependent chain, so the fastest single-network layout)
+    ctx->tuning.i_lanes = 3;
     ctx->tuning.ra_lanes = 0;
 
     // trial clock and synchronisation points (NGS:8906-8925, 9291, 9563)
@@ -1260,7 +1260,7 @@ static hvc_status check_tuning(hvc_ctx* ctx, const hvc_tuning* t)
 {
     auto ok_block = [](int b, int hi) { return b >= 64 && b <= hi && (b % 64) == 0; };
     if (!ok_block(t->integrate_block, 256) || !ok_block(t->deliver_block, 256) || !ok_block(t->sync_block, 512) ||
-        t->i_lanes < 0 || t->i_lanes > 2 || (t->ra_lanes != 0 && t->ra_lanes != 1))
+        t->i_lanes < 0 || t->i_lanes > 3 || (t->ra_lanes != 0 && t->ra_lanes != 1))
         return fail(ctx, HVC_E_ARG, "tuning out of range (blocks: multiples of 64 up to 256, sync up to 512)");
     return HVC_OK;
 }
@@ -1363,12 +1363,13 @@ extern "C" hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, 
     Tuning tu = ctx->tuning;
     double ms[3], best_ms[3];
     hvc_status r;
-    // integrate: block size and the HVC-I and HVC-RA layouts (five lanes per
-    // HVC-I neuron exist for DP8 only, four per HVC-RA neuron for RK4 only)
+    // integrate: block size and the HVC-I and HVC-RA layouts (four, five and
+    // eight lanes per HVC-I neuron exist for DP8 only, four per HVC-RA neuron
+    // for RK4 only)
     double bi = 1e30;
     Tuning ti = tu;
     for (int ra_lanes = (ctx->prm.ra_model == 1 ? 1 : 0); ra_lanes >= 0; --ra_lanes)
-        for (int lanes = (ctx->prm.i_model == 0 ? 2 : 0); lanes >= 0; --lanes) {
+        for (int lanes = (ctx->prm.i_model == 0 ? 3 : 0); lanes >= 0; --lanes) {
             for (int bs : {64, 128, 256}) {
                 Tuning c = tu;
                 c.integrate_block = bs;

@@ -113,6 +113,15 @@ __device__ __forceinline__ void i_step_quad(IState& s, real& xlo, real& wlo, int
     HVC_GUARDED_STEP(IState, (i_dp8_quad<false>(bad, s, xlo, wlo, role, ga, gb, dt, row)),
                      (xlo = xlo0, wlo = wlo0, i_dp8_quad<true>(bad, s, xlo, wlo, role, ga, gb, dt, row)))
 }
+template <class Row>
+__device__ __forceinline__ void i_step_oct(IState& s, real& xo, real& xlo, int q, const GateSlot& g, real dt,
+                                           const Row& row)
+{
+    // the lane's variable and its low part are state too
+    real xo0 = xo, xlo0 = xlo;
+    HVC_GUARDED_STEP(IState, (i_dp8_oct<false>(bad, s, xo, xlo, q, g, dt, row)),
+                     (xo = xo0, xlo = xlo0, i_dp8_oct<true>(bad, s, xo, xlo, q, g, dt, row)))
+}
 #undef HVC_GUARDED_STEP
 
 // state variables between HBM (FP64) and registers: the FP32 form keeps a
@@ -680,14 +689,79 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     }
 }
 
+constexpr int kINeuronsPerOctWave = 8;       // eight lanes per HVC-I neuron
+
+// HVC-I, eight lanes per neuron (DP8 form); one wave holds 8 neurons
+__device__ void integrate_i_oct(const NetDev& nd, const Win& win, int wave, int t0, int t1)
+{
+    const int lane = __lane_id();
+    const int q = lane & 7;
+    const int i = wave * kINeuronsPerOctWave + (lane >> 3);
+    const bool live = i < nd.n_i;
+    const int ii = live ? i : 0;
+    const GateSlot g = kIOctGates[q];
+    // the lane's own variable: V (lane 0), n (1, 6), m (2, 5), h (3, 4), w (7)
+    const int var = q == 0 ? 0 : (q == 1 || q == 6 ? 1 : (q == 2 || q == 5 ? 2 : (q == 7 ? 4 : 3)));
+    IState s;
+    HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
+    HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    win.stage(nd);
+    // the previous window's sync: HHI raiseE (Ge += G), by the octet's first lane
+    double ge = nd.in[5][ii];
+    if (live && q == 0) {
+        double sum;
+        if (inbox_take(nd, nd.in_ge_i, i, sum)) ge += sum;
+    }
+    ge = q < 4 ? quad_bcast<0>(ge) : quad_bcast<3>(oct_mirror(ge));
+    s.Ge = ge; s.Gi = nd.in[6][ii];
+    int flag = nd.i_flag[ii];
+    NoiseCursor nc;
+    if (nd.i_poisson)
+        nc.open(nd.i_ev + (size_t)ii * nd.ev_cap, nd.i_ev_pos[ii]);
+    else
+        nc.none();
+    real xo = var == 0 ? s.V : (var == 1 ? s.n : (var == 2 ? s.m : (var == 3 ? s.h : s.w)));
+    real xlo = 0;
+#ifdef HVC_FP32
+    xlo = var == 0 ? s.V_lo : (var == 1 ? s.n_lo : (var == 2 ? s.m_lo : (var == 3 ? s.h_lo : s.w_lo)));
+#endif
+    uint64_t mask = 0;
+    for (int t = t0; t <= t1; ++t) {
+        int f = i_state_check(s.V, flag);
+        // HHI_buffer noise checks (Ge, Gi), as listed
+        nc.take(t, [&](int gi, double inc) {
+            s.Ge = gi == 0 ? (real)(s.Ge + inc) : s.Ge;
+            s.Gi = gi == 1 ? (real)(s.Gi + inc) : s.Gi;
+        });
+        i_step_oct(s, xo, xlo, q, g, (real)nd.dt, win.row(t));
+        bool fired = live && f && q == 0;
+        if (fired) mask |= 1ull << (t - t0);
+    }
+    window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
+    if (live && (q < 4 || q == 7)) {
+        // lanes 0-3 write V, n, m, h and lane 7 w, widened with their low parts
+#ifdef HVC_FP32
+        nd.in[var][i] = (double)xo + (double)xlo;
+#else
+        nd.in[var][i] = xo;
+#endif
+    }
+    if (live && q == 0) {
+        nd.in[5][i] = s.Ge; nd.in[6][i] = s.Gi;
+        nd.i_flag[i] = flag;
+        if (nd.i_poisson) nd.i_ev_pos[i] = nc.pos;
+        nd.i_mask[i] = mask;
+    }
+}
+
 // blocks [0, iblocks) integrate HVC-I (five lanes per neuron, 12 neurons per
 // wave; four lanes, 16 per wave; or one thread per neuron), the next rblocks
 // HVC-RA (four lanes per neuron, 16 neurons per wave, or one thread per
 // neuron); any blocks after them run the previous window's sync-point STDP
 // (one row per thread, sa).
 // LANES: bit 0 = HVC-I five lanes, bit 1 = HVC-RA four lanes, bit 2 = HVC-I
-// four lanes (the layouts are template parameters: a kernel holds the register
-// allocation of one layout only)
+// four lanes, bit 3 = HVC-I eight lanes (the layouts are template parameters: a
+// kernel holds the register allocation of one layout only)
 template <int RA_MODEL, int I_MODEL, int LANES>
 __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, int iblocks, int rblocks, StdpArgs sa)
 {
@@ -703,7 +777,9 @@ __global__ __launch_bounds__(256) void k_integrate(NetDev nd, int t0, int t1, in
     const int nsteps = t1 - t0 + 1;
     const Win win{(const WinRow*)win_lds, (const double*)((const WinRow*)win_lds + nsteps), t0, win_lds, nsteps};
     if ((int)blockIdx.x < iblocks) {
-        if constexpr ((LANES & 4) != 0)
+        if constexpr ((LANES & 8) != 0)
+            integrate_i_oct(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
+        else if constexpr ((LANES & 4) != 0)
             integrate_i_quad(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
         else if constexpr ((LANES & 1) != 0)
             integrate_i_lanes(nd, win, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), t0, t1);
@@ -746,11 +822,12 @@ void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, in
                                      const StdpArgs& sa, int stdp)
 {
     const int bs = tu.integrate_block;
-    // HVC-I layout (DP8 only): 0 one thread, 1 five lanes, 2 four lanes per neuron
+    // HVC-I layout (DP8 only): 0 one thread, 1 five lanes, 2 four lanes, 3 eight lanes per neuron
     const int i_layout = nd.i_model == 0 ? tu.i_lanes : 0;
     const int ra_lanes = (nd.ra_model == 1 && tu.ra_lanes) ? 1 : 0;
     const int i_per_block = i_layout == 1 ? (bs / 64) * kINeuronsPerWave
-                            : (i_layout == 2 ? (bs / 64) * kINeuronsPerQuadWave : bs);
+                            : (i_layout == 2 ? (bs / 64) * kINeuronsPerQuadWave
+                                             : (i_layout == 3 ? (bs / 64) * kINeuronsPerOctWave : bs));
     const int ra_per_block = ra_lanes ? (bs / 64) * kRANeuronsPerWave : bs;
     int iblocks = nd.n_i == 0 ? 0 : (nd.n_i + i_per_block - 1) / i_per_block;
     int rblocks = blocks_for(nd.n_ra, ra_per_block);
@@ -760,16 +837,19 @@ void HVC_LAUNCHER(launch_integrate)(hipStream_t st, const NetDev& nd, int t0, in
 #define HVC_K(R, I, L) k_integrate<R, I, L><<<grid, bs, lds, st>>>(nd, t0, t1, iblocks, rblocks, sa)
     if (nd.ra_model == 0) {
         if (nd.i_model == 1) HVC_K(0, 1, 0);
+        else if (i_layout == 3) HVC_K(0, 0, 8);
         else if (i_layout == 2) HVC_K(0, 0, 4);
         else if (i_layout == 1) HVC_K(0, 0, 1);
         else HVC_K(0, 0, 0);
     } else if (ra_lanes) {
         if (nd.i_model == 1) HVC_K(1, 1, 2);
+        else if (i_layout == 3) HVC_K(1, 0, 10);
         else if (i_layout == 2) HVC_K(1, 0, 6);
         else if (i_layout == 1) HVC_K(1, 0, 3);
         else HVC_K(1, 0, 2);
     } else {
         if (nd.i_model == 1) HVC_K(1, 1, 0);
+        else if (i_layout == 3) HVC_K(1, 0, 8);
         else if (i_layout == 2) HVC_K(1, 0, 4);
         else if (i_layout == 1) HVC_K(1, 0, 1);
         else HVC_K(1, 0, 0);

@@ -94,7 +94,7 @@ struct NoiseId {
 // launch shapes of the per-window kernels (chosen by the autotuner)
 struct Tuning {
     int integrate_block = 64;   // threads per block of k_integrate (64, 128, 256)
-    int i_lanes = 2;            // HVC-I DP8: 2 = four lanes per neuron, 1 = five lanes, 0 = one thread
+    int i_lanes = 3;            // HVC-I DP8: 3 = eight lanes per neuron, 2 = four, 1 = five, 0 = one thread
     int ra_lanes = 0;           // HVC-RA RK4: 1 = four lanes per neuron, 0 = one thread per neuron
     int deliver_block = 64;     // threads per block of k_deliver
     int sync_block = 256;       // threads per block of k_sync

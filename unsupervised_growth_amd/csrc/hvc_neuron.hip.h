@@ -466,18 +466,50 @@ constexpr real kI_Ek = -HR(80.0), kI_gKHT = HR(500.0), kI_Ei = -HR(75.0), kI_A =
 #define HVC_EXPLIN(v, d, B)                                                                          \
     __builtin_amdgcn_exp2f(__builtin_fmaf((v), (real)(-1.4426950408889634 / (double)(B)),           \
                                           (real)(-(double)(d) * 1.4426950408889634 / (double)(B))))
-template <bool EXACT>
-__device__ __forceinline__ real i_an(real v, Guard& bad)
+// x / (1 - exp(-x)) by its Taylor series 1 + x/2 + x^2/12 - x^4/720 +
+// x^6/30240 - x^8/1209600 (for |x| <= 1/2 the remainder is below 2e-11)
+__device__ __forceinline__ real xexpm1_series(real x)
 {
-    HVC_FP_FAST return __builtin_fmaf(HR(0.15), v, HR(2.25)) * __builtin_amdgcn_rcpf(1 - HVC_EXPLIN(v, 15, 10));
+    HVC_FP_FAST
+    const real x2 = x * x;
+    real p = __builtin_fmaf(x2, HR(-1.0 / 1209600.0), HR(1.0 / 30240.0));
+    p = __builtin_fmaf(x2, p, HR(-1.0 / 720.0));
+    p = __builtin_fmaf(x2, p, HR(1.0 / 12.0));
+    return __builtin_fmaf(x2, p, __builtin_fmaf(x, HR(0.5), HR(1.0)));
 }
+// c (v + d) / (1 - exp(-(v + d) / e)), the form of the rates an and am
+// (HHI_buffer.h:173-176), away from v = -d.  Both factors vanish at v = -d,
+// where the quotient cancels (and is 0/0 at v = -d exactly): within
+// |v + d| <= e/2 the rate is c e g(x), g = xexpm1_series, x = (v + d) / e
+// (rates_vd_near); v + d is exact there (Sterbenz), so both forms put the
+// zero at the same point.
+__device__ __forceinline__ real rate_vd(real v, real d, real e, real c)
+{
+    HVC_FP_FAST
+    const real vd = v + d;
+    return c * vd * __builtin_amdgcn_rcpf(1 - __builtin_amdgcn_exp2f(vd * (real)(-1.4426950408889634 / (double)e)));
+}
+// an (d 15, c 0.15) and am (d 22, c 1), both with e = 10, near their zeros:
+// one test for the two (|v + 18.5| <= 8.5 covers both ranges), the series
+// evaluated only when a lane of the wave is there
+__device__ __forceinline__ bool rates_vd_near(real v, real& an, real& am)
+{
+    HVC_FP_FAST
+    if (__any(__builtin_fabsf(v + HR(18.5)) <= HR(8.5))) {
+        const real xn = (v + HR(15.0)) * HR(0.1), xm = (v + HR(22.0)) * HR(0.1);
+        const real gn = xexpm1_series(xn), gm = xexpm1_series(xm);
+        if (__builtin_fabsf(xn) <= HR(0.5)) an = HR(1.5) * gn;
+        if (__builtin_fabsf(xm) <= HR(0.5)) am = HR(10.0) * gm;
+        return true;
+    }
+    return false;
+}
+template <bool EXACT>
+__device__ __forceinline__ real i_an(real v, Guard& bad) { return rate_vd(v, HR(15.0), HR(10.0), HR(0.15)); }
 template <bool EXACT>
 __device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * HVC_EXPLIN(v, 25, 80); }
 template <bool EXACT>
-__device__ __forceinline__ real i_am(real v, Guard& bad)
-{
-    HVC_FP_FAST return (v + 22) * __builtin_amdgcn_rcpf(1 - HVC_EXPLIN(v, 22, 10));
-}
+__device__ __forceinline__ real i_am(real v, Guard& bad) { return rate_vd(v, HR(22.0), HR(10.0), HR(1.0)); }
 template <bool EXACT>
 __device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * HVC_EXPLIN(v, 47, 18); }
 template <bool EXACT>
@@ -526,9 +558,16 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
     t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
     k.V = -t;
     auto gate = [](real a, real b, real x) { return __builtin_fmaf(-(a + b), x, a); };   // a (1 - x) - b x
-    k.n = gate(i_an<EXACT>(v, bad), i_bn<EXACT>(v, bad), n1);
-    k.m = gate(i_am<EXACT>(v, bad), i_bm<EXACT>(v, bad), m1);
+    real an = i_an<EXACT>(v, bad), am = i_am<EXACT>(v, bad);
+    const real bn = i_bn<EXACT>(v, bad), bm = i_bm<EXACT>(v, bad);
     k.h = gate(i_ah<EXACT>(v, bad), i_bh<EXACT>(v, bad), h1);
+    k.n = gate(an, bn, n1);
+    k.m = gate(am, bm, m1);
+    // an and am near their zeros: the series, after the rest of the stage
+    if (rates_vd_near(v, an, am)) {
+        k.n = gate(an, bn, n1);
+        k.m = gate(am, bm, m1);
+    }
 #else
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
@@ -738,7 +777,7 @@ struct GateSlot {
     real inv2_e;     // RN(log2(e) / e) (FP32 form, EXPCDV)
 };
 
-template <bool EXACT>
+template <bool EXACT, bool CAN_A = true>
 __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
 {
     HVC_FP_FAST
@@ -748,6 +787,16 @@ __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
     real den = (g.type == 0) ? (1 - E) : (1 + E);
     real q = DIVR(num, den);
     HVC_KEEP(q);
+#ifdef HVC_FP32
+    // form A near v = -d: the series (rate_vd)
+    if constexpr (CAN_A) {
+        const bool near = g.type == 0 && __builtin_fabsf(vd) <= HR(0.5) * g.e;
+        if (__any(near)) {
+            const real sr = (g.c * g.e) * xexpm1_series(vd * g.inv_e);
+            if (near) q = sr;
+        }
+    }
+#endif
     return (g.type == 1) ? g.c * E : q;
 }
 
@@ -773,7 +822,7 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
     real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
                 - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
     real a = gate_rate<EXACT>(bad, kIGates[role][0], v);
-    real b = gate_rate<EXACT>(bad, kIGates[role][1], v);
+    real b = gate_rate<EXACT, false>(bad, kIGates[role][1], v);   // second slots are never form A
     real x = role == 1 ? n1 : (role == 2 ? m1 : (role == 3 ? h1 : w1));
     real kg = a * (1 - x) - b * x;
     real kw = (a - x);
@@ -895,25 +944,35 @@ __constant__ GateSlot kIQuadGates[4][2] = {
 
 // FP32 form of a gate slot: with per-lane constants the three rate forms are
 // one expression, E = exp(-(v + d) / e), rate = cB E + (cA (v + d) + cC) / (1 + sg E)
-// (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects
-// rate = cB E + (cA v + cC) / (1 + sg E) with E = 2^(k1 v + k0): the slot's
-// form A, B or C with v + d folded into the exponent and the numerator
+// (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects.
+// Form A cancels near v = -d (rate_vd): its lanes switch to c e g((v + d) / e)
+// within |v + d| <= e/2 (lim; -1 in the other forms), when a lane of the wave is there.
 struct GateK {
-    real cA, cB, cC, sg, k1, k0;
+    real cA, cB, cC, sg, k1, d, ie, ce, lim;
 };
 __device__ __forceinline__ GateK gate_k(const GateSlot& g)
 {
-    const real cA = g.type == 0 ? g.c : HR(0.0);
-    return GateK{cA, g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : cA * g.d, g.type == 0 ? -HR(1.0) : HR(1.0),
-                 -g.inv2_e, -(g.d * g.inv2_e)};
+    const bool A = g.type == 0;
+    return GateK{A ? g.c : HR(0.0), g.type == 1 ? g.c : HR(0.0), g.type == 2 ? g.c : HR(0.0),
+                 A ? -HR(1.0) : HR(1.0), -g.inv2_e, g.d, g.inv_e, g.c * g.e, A ? HR(0.5) * g.e : -HR(1.0)};
 }
 #ifdef HVC_FP32
+template <bool CAN_A>
 __device__ __forceinline__ real gate_rate_k(const GateK& g, real v)
 {
     HVC_FP_FAST
-    const real E = __builtin_amdgcn_exp2f(__builtin_fmaf(g.k1, v, g.k0));
-    const real q = __builtin_fmaf(g.cA, v, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
-    return __builtin_fmaf(g.cB, E, q);
+    const real vd = v + g.d;
+    const real E = __builtin_amdgcn_exp2f(vd * g.k1);
+    const real q = __builtin_fmaf(g.cA, vd, g.cC) * __builtin_amdgcn_rcpf(__builtin_fmaf(g.sg, E, HR(1.0)));
+    real r = __builtin_fmaf(g.cB, E, q);
+    if constexpr (CAN_A) {
+        const bool near = __builtin_fabsf(vd) <= g.lim;
+        if (__any(near)) {
+            const real s = g.ce * xexpm1_series(vd * g.ie);
+            if (near) r = s;
+        }
+    }
+    return r;
 }
 #endif
 
@@ -938,10 +997,23 @@ __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role
     t = __builtin_fmaf(s.Ge * e2, v, t);
     t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
     const real kV = -t;
-    real a = gate_rate_k(gate_k(ga), v);
-    real b = gate_rate_k(gate_k(gb), v);
+    const GateK gka = gate_k(ga);
+    real a = gate_rate_k<false>(gka, v);
+    const real b = gate_rate_k<false>(gate_k(gb), v);   // second slots are never form A
     kw = a - w1;   // a = wInf in lane 0
-    const real kg = __builtin_fmaf(-(a + b), xo, a);   // a (1 - x) - b x
+    real kg = __builtin_fmaf(-(a + b), xo, a);   // a (1 - x) - b x
+    // form A of the first slot near v = -d (rate_vd): the series, and the
+    // derivatives again, when a lane of the wave is there (after the rest of
+    // the stage, so that its work is not split by the branch)
+    const real vd = v + gka.d;
+    const bool near = __builtin_fabsf(vd) <= gka.lim;
+    if (__any(near)) {
+        const real sa = gka.ce * xexpm1_series(vd * gka.ie);
+        if (near) {
+            a = sa;
+            kg = __builtin_fmaf(-(a + b), xo, a);
+        }
+    }
 #else
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
@@ -1012,6 +1084,138 @@ __device__ __forceinline__ void i_dp8_quad(Guard& bad, IState& s, real& xlo, rea
     s.Gi = s.Gi * row_e5dt(row);
 }
 #undef HVC_BCAST4
+
+// ---- HVC-I, DP8, eight lanes per neuron (octet) ----
+// Lanes q and 7 - q of a neuron's octet form a pair, exchanged by the DPP
+// half-row mirror: (1, 6) carry n and evaluate an / bn, (2, 5) m with am / bm,
+// (3, 4) h with ah / bh.  Each lane of a pair evaluates one of its variable's
+// two rates, takes the other from its partner and advances the variable with
+// the same operations on the same values, so both hold the same bits.  Lane 0
+// owns V, lane 7 owns w and evaluates wInf.  A stage thus evaluates one rate
+// per lane (the quad two) and one stage combination (the quad two); lane 0's
+// membrane derivative gets n, m, h by quad broadcasts and w by the mirror.
+// Every value is the expression tree of HHI_buffer, as in i_dp8.
+__constant__ GateSlot kIOctGates[8] = {
+    {1, HR(0.0), HR(0.0), HR(10.0), HR(0.1), HVC_L2(10.0)},                 // V: no rate (unused)
+    {0, HR(0.15), HR(15.0), HR(10.0), HR(0.1), HVC_L2(10.0)},               // an
+    {0, HR(1.0), HR(22.0), HR(10.0), HR(0.1), HVC_L2(10.0)},                // am
+    {1, HR(0.7), HR(34.0), HR(20.0), HR(1.0) / HR(20.0), HVC_L2(20.0)},     // ah
+    {2, HR(10.0), HR(4.0), HR(10.0), HR(0.1), HVC_L2(10.0)},                // bh
+    {1, HR(40.0), HR(47.0), HR(18.0), HR(1.0) / HR(18.0), HVC_L2(18.0)},    // bm
+    {1, HR(0.2), HR(25.0), HR(80.0), HR(1.0) / HR(80.0), HVC_L2(80.0)},     // bn
+    {2, HR(1.0), HR(0.0), HR(5.0), HR(0.2), HVC_L2(5.0)},                   // wInf
+};
+
+// value of lane 7 - q of the octet (row_half_mirror), in lane q
+__device__ __forceinline__ float oct_mirror(float x)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double oct_mirror(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x141, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x141, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// derivative of the lane's variable at stage state (v, n1, m1, h1, w1); xo =
+// the lane's own variable at the stage; n1, m1, h1, w1 are needed in lane 0 only
+template <bool EXACT>
+__device__ __forceinline__ real i_oct_rhs(Guard& bad, const IState& s, int q, const GateSlot& g, real v, real n1,
+                                          real m1, real h1, real w1, real xo, real e2, real e5)
+{
+    HVC_FP_FAST
+#ifdef HVC_FP32
+    const real n2 = n1 * n1;
+    const real gk = __builtin_fmaf(kI_gKdr, n2 * n2, kI_gKHT * w1);
+    real t = __builtin_fmaf(kI_gL, v, -(kI_gL * kI_El));
+    t = __builtin_fmaf(kI_gNa * h1 * (m1 * m1 * m1), v - kI_Ena, t);
+    t = __builtin_fmaf(gk, v - kI_Ek, t);
+    t = __builtin_fmaf(s.Ge * e2, v, t);
+    t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
+    const real kV = -t;
+    const GateK gk1 = gate_k(g);
+    real r = gate_rate_k<false>(gk1, v);
+    // form A (an, am) near v = -d: the series (rate_vd), when a lane of the wave is there
+    const real vd = v + gk1.d;
+    const bool near = __builtin_fabsf(vd) <= gk1.lim;
+    if (__any(near)) {
+        const real sr = gk1.ce * xexpm1_series(vd * gk1.ie);
+        if (near) r = sr;
+    }
+    const real mr = oct_mirror(r);
+    const real a = q < 4 ? r : mr, b = q < 4 ? mr : r;
+    const real kg = __builtin_fmaf(-(a + b), xo, a);   // a (1 - x) - b x
+#else
+    real n4 = n1 * n1 * n1 * n1;
+    real m3 = m1 * m1 * m1;
+    real kV = -kI_gL * (v - kI_El) - kI_gNa * h1 * m3 * (v - kI_Ena) - kI_gKdr * n4 * (v - kI_Ek)
+                - kI_gKHT * w1 * (v - kI_Ek) - (s.Ge * e2) * v - (s.Gi * e5) * (v - kI_Ei) + 100000 * HR(0.0) / kI_A;
+    const real r = gate_rate<EXACT>(bad, g, v);
+    const real mr = oct_mirror(r);
+    const real a = q < 4 ? r : mr, b = q < 4 ? mr : r;
+    const real kg = a * (1 - xo) - b * xo;
+#endif
+    const real kw = r - xo;   // lane 7: wInf - w
+    return q == 0 ? kV : (q == 7 ? kw : kg);
+}
+
+// the stage values every lane needs: V in all lanes; n, m, h (quad
+// broadcasts) and w (mirror of lane 7) in lane 0
+#define HVC_BCAST8(x)                                            \
+    {                                                            \
+        const real t_ = oct_mirror(x);                           \
+        v = q < 4 ? quad_bcast<0>(x) : quad_bcast<3>(t_);        \
+        n1 = quad_bcast<1>(x);                                   \
+        m1 = quad_bcast<2>(x);                                   \
+        h1 = quad_bcast<3>(x);                                   \
+        w1 = t_;                                                 \
+    }
+
+// HHI_buffer::DP8_step with eight lanes per neuron; xo / xlo: the lane's own
+// variable and (FP32 form) its low part; s.V is current in every lane, s.n,
+// s.m, s.h, s.w in lane 0 only
+template <bool EXACT, class Row>
+__device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real& xlo, int q, const GateSlot& g,
+                                          real dt, const Row& row)
+{
+    HVC_FP_FAST
+    const real x0 = xo;
+    real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
+    real x;
+#define HVC_OSTAGE(N, e2, e5, SX)                                                   \
+    const real k_##N = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, e2, e5); \
+    x = x0 + SX(k_);                                                                \
+    HVC_BCAST8(x)
+    const real k_1 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x0, HR(1.0), HR(1.0));
+    x = x0 + HVC_DP8_S2(k_);
+    HVC_BCAST8(x)
+    HVC_OSTAGE(2, row.e2[kOffDp2], row.e5[kOffDp2], HVC_DP8_S3)
+    HVC_OSTAGE(3, row.e2[kOffDp3], row.e5[kOffDp3], HVC_DP8_S4)
+    HVC_OSTAGE(4, row.e2[kOffDp4], row.e5[kOffDp4], HVC_DP8_S5)
+    HVC_OSTAGE(5, row.e2[kOffDp5], row.e5[kOffDp5], HVC_DP8_S6)
+    HVC_OSTAGE(6, row.e2[kOffDp6], row.e5[kOffDp6], HVC_DP8_S7)
+    HVC_OSTAGE(7, row.e2[kOffDp7], row.e5[kOffDp7], HVC_DP8_S8)
+    HVC_OSTAGE(8, row.e2[kOffDp8], row.e5[kOffDp8], HVC_DP8_S9)
+    HVC_OSTAGE(9, row.e2[kOffDp9], row.e5[kOffDp9], HVC_DP8_S10)
+    HVC_OSTAGE(10, row.e2[kOffDp10], row.e5[kOffDp10], HVC_DP8_S11)
+    HVC_OSTAGE(11, row.e2[kOffDp11], row.e5[kOffDp11], HVC_DP8_S12)
+#undef HVC_OSTAGE
+    const real k_12 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, row.e2[kOffDt], row.e5[kOffDt]);
+    {
+        struct { real x, x_lo; } acc{x0, xlo};
+        HVC_ACC(acc, x, HVC_DP8_FIN(k_))
+        x = acc.x;
+        xlo = acc.x_lo;
+    }
+    xo = x;
+    HVC_BCAST8(x)
+    s.V = v; s.n = n1; s.m = m1; s.h = h1; s.w = w1;
+    s.Ge = s.Ge * row_e2dt(row);
+    s.Gi = s.Gi * row_e5dt(row);
+}
+#undef HVC_BCAST8
 
 }  // namespace p32 / p64
 }  // namespace hvc
