@@ -63,7 +63,8 @@ def parse():
                          "(calibrates the HBM byte counters)")
     ap.add_argument("--tune-windows", type=int, default=400, help="coupling windows per autotuner candidate (0: no tuning)")
     ap.add_argument("--shape", default=None,
-                    help="launch shape integrate_block,i_lanes,deliver_block,sync_block[,ra_lanes] (skips the autotuner)")
+                    help="launch shape integrate_block,i_lanes,deliver_block,sync_block[,ra_lanes[,deliver_lists]] "
+                         "(skips the autotuner)")
     ap.add_argument("--driver-trials", type=int, default=2,
                     help="c3: growth-loop trials through the C++ driver (bookkeeping included); 0 skips")
     ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
@@ -180,7 +181,7 @@ def apply_shape(eng, args):
     if args.shape:
         v = [int(x) for x in args.shape.split(",")]
         eng.set_tuning(integrate_block=v[0], i_lanes=v[1], deliver_block=v[2], sync_block=v[3],
-                       ra_lanes=v[4] if len(v) > 4 else 1)
+                       ra_lanes=v[4] if len(v) > 4 else 1, deliver_lists=v[5] if len(v) > 5 else 1)
     elif args.tune_windows > 0:
         shape, ms = eng.autotune(windows=args.tune_windows)
         eng.tune_ms = ms

@@ -173,6 +173,9 @@ typedef struct {
     int deliver_block;     /* 64, 128, 256 */
     int sync_block;        /* 64 .. 512 */
     int ra_lanes;          /* HVC-RA RK4: 1 = four lanes per neuron, 0 = one thread per neuron */
+    int deliver_lists;     /* 1 = the delivery visits only the neurons that spiked in the window or
+                              still have spikes in flight (lists kept on the device), 0 = it reads
+                              every neuron's queue state each window */
 } hvc_tuning;
 hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* tuning);
 hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* tuning);

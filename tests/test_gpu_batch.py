@@ -65,17 +65,19 @@ def test_batch_equals_each_network_alone(precision):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_launch_shapes_do_not_change_results(precision):
-    """Every launch shape the autotuner may pick gives the same bits (RK4 form)."""
+    """Every launch shape the autotuner may pick gives the same bits (RK4 form),
+    the delivery on its lists or sweeping every neuron."""
     from unsupervised_growth_amd.engine import Engine
     nets = _nets()
     keys = [(100 + b, 7) for b in range(len(nets))]
-    shapes = [dict(integrate_block=64, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1),
-              dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512, ra_lanes=0),
-              dict(integrate_block=128, i_lanes=2, deliver_block=256, sync_block=128, ra_lanes=0),
-              dict(integrate_block=256, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1),
-              dict(integrate_block=64, i_lanes=2, deliver_block=128, sync_block=512, ra_lanes=1),
-              dict(integrate_block=128, i_lanes=3, deliver_block=64, sync_block=256, ra_lanes=1),
-              dict(integrate_block=256, i_lanes=3, deliver_block=128, sync_block=256, ra_lanes=0)]
+    shapes = [dict(integrate_block=64, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1, deliver_lists=1),
+              dict(integrate_block=256, i_lanes=0, deliver_block=128, sync_block=512, ra_lanes=0, deliver_lists=0),
+              dict(integrate_block=128, i_lanes=2, deliver_block=256, sync_block=128, ra_lanes=0, deliver_lists=1),
+              dict(integrate_block=256, i_lanes=1, deliver_block=64, sync_block=256, ra_lanes=1, deliver_lists=0),
+              dict(integrate_block=64, i_lanes=2, deliver_block=128, sync_block=512, ra_lanes=1, deliver_lists=0),
+              dict(integrate_block=128, i_lanes=3, deliver_block=64, sync_block=256, ra_lanes=1, deliver_lists=1),
+              dict(integrate_block=256, i_lanes=3, deliver_block=128, sync_block=256, ra_lanes=0, deliver_lists=0),
+              dict(integrate_block=256, i_lanes=3, deliver_block=256, sync_block=256, ra_lanes=0, deliver_lists=1)]
     runs = []
     for shape in shapes:
         # the RK4 form for HVC-RA, where four lanes per neuron exist

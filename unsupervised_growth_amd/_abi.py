@@ -96,7 +96,7 @@ class PoolCaps(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("integrate_block", ctypes.c_int), ("i_lanes", ctypes.c_int), ("deliver_block", ctypes.c_int),
-                ("sync_block", ctypes.c_int), ("ra_lanes", ctypes.c_int)]
+                ("sync_block", ctypes.c_int), ("ra_lanes", ctypes.c_int), ("deliver_lists", ctypes.c_int)]
 
 
 class Synapses(ctypes.Structure):

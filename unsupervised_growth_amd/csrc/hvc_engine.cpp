@@ -709,6 +709,13 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
     nd.in_gi_ra = mk_inbox(nra, c.cap_gi_ra);
     nd.in_ge_ra = mk_inbox(nra, c.cap_ge_ra);
     nd.in_ge_i = mk_inbox(ni, c.cap_ge_i);
+    const int wl_n[3] = {nra, nra, ni};
+    for (int q = 0; q < 3; ++q) {
+        nd.wl_cap[q] = wl_n[q];
+        nd.wl_list[q] = dalloc<int>(ctx, 2 * (size_t)wl_n[q], &st);
+        nd.wl_mark[q] = dalloc<int>(ctx, wl_n[q], &st);
+    }
+    nd.wl_cnt = dalloc<int>(ctx, kWlCounters, &st);
     nd.err = dalloc<unsigned int>(ctx, 1, &st);
     nd.warn = dalloc<int>(ctx, 3 * (size_t)n_nets + 1, &st);
     if (st != HVC_OK) free_bufs(ctx, first_buf);
@@ -1137,13 +1144,16 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     // the window's view: post-event buffer w % 2.  The sync of window w (the
     // inbox sums into the conductances) runs at the start of window w + 1's
     // integration; after the last window it runs on its own.
+    nd.growth_mode = any_growth && any_chain ? 2 : (any_growth ? 1 : 0);
+    nd.wl_on = ctx->tuning.deliver_lists;
     NetDev ndw[2] = {nd, nd};
     ndw[1].post_list = nd.post_list + kPostListCap;
     ndw[1].post_list_cnt = nd.post_list_cnt + 1;
     ndw[1].post_list_next_cnt = nd.post_list_cnt;
     for (int w = 0; w < nw; ++w) {
         int t0 = ctx->windows[w].first, t1 = ctx->windows[w].second;
-        const NetDev& nv = ndw[w & 1];
+        NetDev nv = ndw[w & 1];
+        nv.window = w;   // the delivery lists' index
         bool tm = ctx->timing && (w % 16 == 7);
         noise_chunk(ctx, nd, w);
         tstart(ctx, kTInt, tm);
@@ -1260,7 +1270,8 @@ static hvc_status check_tuning(hvc_ctx* ctx, const hvc_tuning* t)
 {
     auto ok_block = [](int b, int hi) { return b >= 64 && b <= hi && (b % 64) == 0; };
     if (!ok_block(t->integrate_block, 256) || !ok_block(t->deliver_block, 256) || !ok_block(t->sync_block, 512) ||
-        t->i_lanes < 0 || t->i_lanes > 3 || (t->ra_lanes != 0 && t->ra_lanes != 1))
+        t->i_lanes < 0 || t->i_lanes > 3 || (t->ra_lanes != 0 && t->ra_lanes != 1) ||
+        (t->deliver_lists != 0 && t->deliver_lists != 1))
         return fail(ctx, HVC_E_ARG, "tuning out of range (blocks: multiples of 64 up to 256, sync up to 512)");
     return HVC_OK;
 }
@@ -1273,6 +1284,7 @@ extern "C" hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* t)
     t->ra_lanes = ctx->tuning.ra_lanes;
     t->deliver_block = ctx->tuning.deliver_block;
     t->sync_block = ctx->tuning.sync_block;
+    t->deliver_lists = ctx->tuning.deliver_lists;
     return HVC_OK;
 }
 
@@ -1286,6 +1298,7 @@ extern "C" hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* t)
     ctx->tuning.ra_lanes = t->ra_lanes;
     ctx->tuning.deliver_block = t->deliver_block;
     ctx->tuning.sync_block = t->sync_block;
+    ctx->tuning.deliver_lists = t->deliver_lists;
     return HVC_OK;
 }
 
@@ -1314,6 +1327,7 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
     HVC_TRY(hipMemcpyAsync(ctx->d_net_first, zeros.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
     refresh_rr_pos(ctx, nd);
+    nd.growth_mode = 0;   // chain tests only
     launch_reset(s, nd, ctx->E);
     launch_chain_edges(s, nd);
     const int nw = std::min(windows, (int)ctx->windows.size());
@@ -1326,12 +1340,15 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
         noise_chunk(ctx, nd, w);
         hipEventRecord(ev[6 * w + 0], s);
         const StdpArgs none{nd.post_list, nd.post_list_cnt, 0};
+        NetDev nv = nd;
+        nv.window = w;
+        nv.wl_on = tu.deliver_lists;
         if (ctx->prm.precision == 1)
-            launch_integrate_f32(s, nd, t0, t1, tu, none, 0);
+            launch_integrate_f32(s, nv, t0, t1, tu, none, 0);
         else
-            launch_integrate(s, nd, t0, t1, tu, none, 0);
+            launch_integrate(s, nv, t0, t1, tu, none, 0);
         hipEventRecord(ev[6 * w + 1], s);
-        launch_deliver(s, nd, t0, t1, tu);
+        launch_deliver(s, nv, t0, t1, tu);
         hipEventRecord(ev[6 * w + 2], s);
         hipEventRecord(ev[6 * w + 3], s);
         launch_sync(s, nd, tu);
@@ -1382,12 +1399,14 @@ extern "C" hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, 
     tu = ti;
     double bd = 1e30;
     Tuning td = tu;
-    for (int bs : {64, 128, 256}) {
-        Tuning c = tu;
-        c.deliver_block = bs;
-        if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
-        if (ms[1] < bd) { bd = ms[1]; td = c; }
-    }
+    for (int lists = 1; lists >= 0; --lists)
+        for (int bs : {64, 128, 256}) {
+            Tuning c = tu;
+            c.deliver_block = bs;
+            c.deliver_lists = lists;
+            if ((r = timed_windows(ctx, windows, c, ms)) != HVC_OK) return r;
+            if (ms[1] < bd) { bd = ms[1]; td = c; }
+        }
     tu = td;
     double bsy = 1e30;
     Tuning ts = tu;
@@ -1422,8 +1441,9 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     PoolCost c;
     const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
     c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 16 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +
-           4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise;
-    c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise;
+           4.0 * kMaxStreams * 4 + 4 + kMaxStreams * NssA * 4 + 4 + 2 * inflight + 2 * inbox + 4 + logs + noise +
+           2 * 3 * 4;
+    c.i = 7 * 8 + 4 + 2 * 8 + 4 + 8 + 4 + inflight + inbox + 4 + logs + noise + 3 * 4;
     c.rr = 4 + 8 + 4 + sizeof(RrPos) + 8 + 1 + 1 + 4 + kPreRing * 8 + 4 + 4;
     c.a2i = 4 + 8 + 8;
     c.i2ra = 4 + 8 + 8;

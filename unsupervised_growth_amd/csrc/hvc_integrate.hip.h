@@ -360,6 +360,34 @@ __device__ __forceinline__ void window_append(int2* log, int* cnt, int cap, uint
     }
 }
 
+// A neuron that spiked in the window goes on the role's delivery list for the
+// window (the delivery kernel visits only listed neurons), unless the previous
+// window's delivery listed it already (spikes still in flight); `listed` is
+// the neuron's list mark, loaded when the window began.  One atomic per wave.
+// A list holds each neuron at most once, so it never outgrows its neurons.
+__device__ __forceinline__ void wl_enlist(const NetDev& nd, int role, bool spiked, int listed, int neuron)
+{
+    if (!nd.wl_on) return;
+    const int w = nd.window;
+    const bool pred = spiked && listed != w;
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&nd.wl_cnt[3 * role + w % 3], __popcll(m));
+    base = __shfl(base, leader);
+    if (pred) {
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (pos < nd.wl_cap[role]) {
+            nd.wl_list[role][(size_t)(w & 1) * nd.wl_cap[role] + pos] = neuron;
+            nd.wl_mark[role][neuron] = w;
+        } else {
+            atomicOr(nd.err, (unsigned)kErrWorklist);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // integrate: HVC-RA neurons (NGS:8931-9216 minus delivery) and HVC-I neurons
 // (NGS:9221-9286 minus delivery) through the steps t0..t1 of one window, in one
@@ -387,6 +415,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     const bool live = i < nd.n_ra;
     const int ii = live ? i : 0;
     const int growth = nd.net_growth[nd.ra_net[ii]];
+    const int listed0 = nd.wl_on ? nd.wl_mark[0][ii] : 0, listed1 = nd.wl_on ? nd.wl_mark[1][ii] : 0;
     const RAUniform u = ra_uniform(nd);
     RAState s;
     HVC_LOAD(s, Vs, nd.ra[0][ii]) HVC_LOAD(s, Vd, nd.ra[1][ii]) HVC_LOAD(s, n, nd.ra[2][ii])
@@ -467,6 +496,8 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
     // tested against the events of earlier windows only, as in the step loop)
     window_append(nd.soma_log, &nd.log_cnt[0], nd.log_cap, smask, i, t0, nd.err);
     window_append(nd.dend_log, &nd.log_cnt[1], nd.log_cap, dmask, i, t0, nd.err);
+    wl_enlist(nd, 0, smask != 0, listed0, i);
+    wl_enlist(nd, 1, smask != 0, listed1, i);
     uint64_t pmask = 0;
     if (growth) {
         for (uint64_t m = smask; m; m &= m - 1) {
@@ -515,6 +546,7 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    const int listed = nd.wl_on ? nd.wl_mark[2][ii] : 0;
     win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G)
     double ge = nd.in[5][ii];
@@ -542,6 +574,7 @@ __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int 
         if (fired) mask |= 1ull << (t - t0);
     }
     window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
+    wl_enlist(nd, 2, mask != 0, listed, i);
     if (live) {
         nd.in[0][i] = HVC_WIDE(s, V); nd.in[1][i] = HVC_WIDE(s, n); nd.in[2][i] = HVC_WIDE(s, m);
         nd.in[3][i] = HVC_WIDE(s, h); nd.in[4][i] = HVC_WIDE(s, w);
@@ -572,6 +605,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    const int listed = nd.wl_on ? nd.wl_mark[2][ii] : 0;
     win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G), by the group's first lane
     double ge = nd.in[5][ii];
@@ -605,6 +639,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
         if (fired) mask |= 1ull << (t - t0);
     }
     window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
+    wl_enlist(nd, 2, mask != 0, listed, i);
     if (live) {
         // each lane writes its own variable, widened with its low part
         const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : (role == 3 ? s.h : s.w)));
@@ -636,6 +671,7 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    const int listed = nd.wl_on ? nd.wl_mark[2][ii] : 0;
     win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G), by the quad's first lane
     double ge = nd.in[5][ii];
@@ -670,6 +706,7 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
         if (fired) mask |= 1ull << (t - t0);
     }
     window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
+    wl_enlist(nd, 2, mask != 0, listed, i);
     if (live) {
         // each lane writes its own variable, widened with its low part; lane 0 also w
         const real xr = role == 0 ? s.V : (role == 1 ? s.n : (role == 2 ? s.m : s.h));
@@ -705,6 +742,7 @@ __device__ void integrate_i_oct(const NetDev& nd, const Win& win, int wave, int 
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
     HVC_LOAD(s, h, nd.in[3][ii]) HVC_LOAD(s, w, nd.in[4][ii])
+    const int listed = nd.wl_on ? nd.wl_mark[2][ii] : 0;
     win.stage(nd);
     // the previous window's sync: HHI raiseE (Ge += G), by the octet's first lane
     double ge = nd.in[5][ii];
@@ -738,6 +776,7 @@ __device__ void integrate_i_oct(const NetDev& nd, const Win& win, int wave, int 
         if (fired) mask |= 1ull << (t - t0);
     }
     window_append(nd.i_log, &nd.log_cnt[2], nd.log_cap, mask, i, t0, nd.err);
+    wl_enlist(nd, 2, mask != 0, listed, i);
     if (live && (q < 4 || q == 7)) {
         // lanes 0-3 write V, n, m, h and lane 7 w, widened with their low parts
 #ifdef HVC_FP32

@@ -14,6 +14,7 @@ constexpr int kMaxInflight = 16; // in-flight spikes per neuron for fixed-weight
 constexpr int kMaxWindowSteps = 64;  // steps between two synchronisation points
 constexpr int kPostListCap = 8192;  // new post events per window, all networks together
 constexpr int kMaxNets = 4096;      // networks per context (12 bits of a post-event key)
+constexpr int kWlCounters = 9;      // NetDev::wl_cnt: 3 roles x 3 list counts
 
 // hvc_trace_ra_noalloc: its clock and decay tables live in a fixed per-device
 // array (hvc_noalloc.hip) sized for this many trace steps
@@ -32,6 +33,7 @@ enum ErrFlag : uint32_t {
     kErrInbox = 16u,
     kErrPostRing = 32u,
     kErrNoise = 64u,
+    kErrWorklist = 128u,
 };
 
 // counters of the synapse-state machine's branches (hvc_set_branch_counting):
@@ -97,6 +99,7 @@ struct Tuning {
     int i_lanes = 3;            // HVC-I DP8: 3 = eight lanes per neuron, 2 = four, 1 = five, 0 = one thread
     int ra_lanes = 0;           // HVC-RA RK4: 1 = four lanes per neuron, 0 = one thread per neuron
     int deliver_block = 64;     // threads per block of k_deliver
+    int deliver_lists = 1;      // 1: the delivery visits listed neurons only, 0: every neuron's slot table
     int sync_block = 256;       // threads per block of k_sync
 };
 
@@ -240,6 +243,18 @@ struct NetDev {
     int* chain_e;                // chain test: row i's active edges sorted by (delay, target) at rr_ptr[i]
 
     Inbox in_gi_ra, in_ge_ra, in_ge_i;
+
+    // delivery worklists, per role (0 RA->RA rows, 1 RA->I pushes, 2 I->RA
+    // pushes): the presynaptic neurons the window's delivery visits.  The
+    // delivery of window w - 1 lists those with spikes still in flight, the
+    // integration of window w adds those that spike in it.
+    int* wl_list[3];             // [2][wl_cap[r]], list of window w in half w % 2
+    int* wl_mark[3];             // [wl_cap[r]] window the neuron is listed for (-1: none)
+    int* wl_cnt;                 // [3 roles][3] list counts, by window % 3
+    int wl_cap[3];
+    int window;                  // index of the launch's window in the trial
+    int wl_on;                   // the delivery runs on the lists (the integration fills them)
+    int growth_mode;             // 1: every network runs a growth trial, 0: every one a chain test, 2: mixed
 
     unsigned int* err;
     int* warn;                   // [3 n_nets]

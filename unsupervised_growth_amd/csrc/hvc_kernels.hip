@@ -22,6 +22,7 @@
 // so the FP64 form reproduces the CPU results bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "hvc_internal.h"
@@ -84,6 +85,8 @@ __global__ void k_reset(NetDev nd)
         fl_clear(nd.fl_rr, i);
         nd.in_gi_ra.cnt[i] = 0;
         nd.in_ge_ra.cnt[i] = 0;
+        nd.wl_mark[0][i] = -1;
+        nd.wl_mark[1][i] = -1;
     }
     if (i < nd.n_i) {
         nd.in[0][i] = -66.0; nd.in[1][i] = 0.125; nd.in[2][i] = 0.0; nd.in[3][i] = 0.99; nd.in[4][i] = 0.0;
@@ -99,6 +102,7 @@ __global__ void k_reset(NetDev nd)
         nd.i_draw[i] = ps.draw;
         fl_clear(nd.fl_i2ra, i);
         nd.in_ge_i.cnt[i] = 0;
+        nd.wl_mark[2][i] = -1;
     }
     if (i == 0) {
         nd.log_cnt[0] = 0; nd.log_cnt[1] = 0; nd.log_cnt[2] = 0;
@@ -107,6 +111,7 @@ __global__ void k_reset(NetDev nd)
         *nd.err = 0;
     }
     if (i < 3 * nd.n_nets) nd.warn[i] = 0;
+    if (i < kWlCounters) nd.wl_cnt[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -311,9 +316,10 @@ __device__ __forceinline__ double group_min(double x)
 // records in the reference's order whatever order they are written in, so
 // lane r of the group takes edges cursor + r, cursor + r + G, ... (the records'
 // atomics of a spike overlap) and the group agrees on the new cursor, the
-// first edge not due.  New spikes take the free slots in slot order.
+// first edge not due.  New spikes take the free slots in slot order.  Returns
+// the number of spikes still in flight after the window (in every lane).
 template <int G>
-__device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin,
+__device__ __forceinline__ int push_fixed(const NetDev& nd, const double* wclk, int pre, int r, int row_begin,
                                            int row_end, const int* edge_ids, const int* tgt_arr, const double* w_arr,
                                            const double* d_arr, const InFlight& fl, FlHead h,
                                            uint64_t new_mask, int t0, int t1, const Inbox& ib)
@@ -321,7 +327,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
     const int deg = row_end - row_begin;
     FlHead* hp = fl.head + pre;
     const int held = h.count;
-    if (deg == 0 || (held == 0 && new_mask == 0)) return;
+    if (deg == 0 || (held == 0 && new_mask == 0)) return held;
     FlSlot* more = fl.more + (size_t)pre * (kMaxInflight - kHeadSlots);
     auto slot = [&](int k) -> FlSlot* { return k < kHeadSlots ? &hp->s[k] : more + (k - kHeadSlots); };
     const double tend = wclk[t1 - t0 + 1];
@@ -427,6 +433,7 @@ __device__ __forceinline__ void push_fixed(const NetDev& nd, const double* wclk,
         }
     }
     if (r == 0) hp->count = now_held;
+    return now_held;
 }
 
 // next item of stream sidx of row i from the row's stream state in memory:
@@ -463,6 +470,7 @@ __device__ __forceinline__ bool stream_peek(const NetDev& nd, int i, int sidx, d
 }
 
 constexpr int kQuadPushNeurons = 65536;   // neurons per context up to which the pushes take a quad per neuron
+constexpr int kDeliverRoleWaves = 256;    // waves per role of a delivery launch, at most
 
 // A row's first kRowHeads streams while a window drains them, one column of the
 // block's LDS per thread: the head item (T, target, edge) and the stream's
@@ -670,7 +678,8 @@ __device__ __forceinline__ void rr_item(const NetDev& nd, int i, int e, int tgt,
 
 // Growth-mode RA->RA row of presynaptic neuron i for the window t0..t1
 // (NGS:8991-9030 enqueue, 9069-9191 drain with LTD, 9375-9554 at the sync).
-__device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int i, int t0, int t1, StreamHeads hd)
+// Returns the number of the row's spikes still in flight after the window.
+__device__ __forceinline__ int rr_row(const NetDev& nd, const double* wclk, int i, int t0, int t1, StreamHeads hd)
 {
     const int Nss = nd.syn.Nss;
     uint64_t mask = nd.soma_mask[i];
@@ -679,7 +688,7 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
     // at a step before it, with no spike to enqueue, the drain and the
     // exhausted-stream pass change nothing, so the step is skipped
     double next_T = sc > 0 ? nd.st_next[i] : HUGE_VAL;
-    if (mask == 0 && !(wclk[t1 - t0 + 1] >= next_T)) return;
+    if (mask == 0 && !(wclk[t1 - t0 + 1] >= next_T)) return sc;
     int* st_step = nd.st_step + (size_t)i * kMaxStreams;
     int* st_mode = nd.st_mode + (size_t)i * kMaxStreams;
     int* st_cur = nd.st_cursor + (size_t)i * kMaxStreams;
@@ -853,27 +862,31 @@ __device__ __forceinline__ void rr_row(const NetDev& nd, const double* wclk, int
     if (cached) heads_store(nd, i, sc, hd);
     if (lastp != lastp0) nd.last_pre[i] = lastp;
     nd.st_next[i] = next_T;
+    return sc;
 }
 
 // ---------------------------------------------------------------------------
-// deliver: one thread per presynaptic neuron and role
+// deliver, sweep form: one lane (group) per presynaptic neuron and role
 //   role 0: RA->RA rows (growth) or chain-test RA->RA push
 //   role 1: RA->I push        role 2: I->RA push
+// Every neuron's row state or slot-table head is read, so a push starts one
+// memory round trip after the kernel does; the bytes grow with the network
+// (128 per neuron and fixed-weight role), which the list form below avoids.
 // ---------------------------------------------------------------------------
 // two waves per SIMD: the register budget of the fixed-weight pushes and the
 // row drain (its LTD steps) is held to 256 so that a launch's ~2 waves per SIMD
 // are resident at once
 // GA / GB lanes per presynaptic neuron of the RA->I / I->RA pushes
 template <int GA, int GB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deliver(NetDev nd, int t0, int t1,
-                                                                                       int b0, int b1)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deliver_sweep(NetDev nd, int t0,
+                                                                                             int t1, int b0, int b1)
 {
     const int b = blockIdx.x, bs = blockDim.x, tid = threadIdx.x;
     __shared__ double wclk[kMaxWindowSteps + 1];
     extern __shared__ double heads_lds[];   // StreamHeads of the block's rows
     // the window's clock, and for a fixed-weight push its row, the window's
     // spikes and the slot-table head: all issued before the block waits on the
-    // clock, so the push starts one memory round trip after the kernel does
+    // clock
     const int nclk = t1 - t0 + 2;
     const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
     const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
@@ -904,9 +917,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         // RA->RA: one lane per row
         const int i = b * blockDim.x + threadIdx.x;
         if (i >= nd.n_ra) return;
-        const int bs = blockDim.x;
         const StreamHeads hd{heads_lds + tid, (int*)(heads_lds + kHeadDoubles * kRowHeads * bs) + tid, bs};
-        if (nd.net_growth[nd.ra_net[i]])
+        const int growth = nd.growth_mode == 2 ? nd.net_growth[nd.ra_net[i]] : nd.growth_mode;
+        if (growth)
             rr_row(nd, wclk, i, t0, t1, hd);
         else
             push_fixed<1>(nd, wclk, i, 0, nd.rr_ptr[i], nd.rr_ptr[i] + nd.act_cnt[i], nd.chain_e, nd.rr_tgt, nd.w,
@@ -915,14 +928,88 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     // fixed-weight pushes: G lanes per presynaptic neuron
     if (!push) return;
-    if constexpr (GA == GB) {
-        push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1,
-                       ra ? nd.in_ge_i : nd.in_gi_ra);
-    } else {
-        if (ra)
-            push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_ge_i);
-        else
-            push_fixed<GB>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_gi_ra);
+    if (ra)
+        push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_ge_i);
+    else
+        push_fixed<GB>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_gi_ra);
+}
+
+// ---------------------------------------------------------------------------
+// deliver, list form: the presynaptic neurons with work in the window, per
+// role (as above).  The window visits the neurons on the role's list only: those that spiked in
+// it (listed by the integration) and those with spikes still in flight after
+// the previous window (listed here), so it reads the rows, slot tables and
+// edges of its active neurons and nothing else.  Item k of a role goes to wave
+// k mod W (W = the role's waves): up to W items run one per wave, side by side.
+// ---------------------------------------------------------------------------
+template <int GA, int GB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_deliver_list(NetDev nd, int t0,
+                                                                                            int t1, int b0, int b1)
+{
+    const int b = blockIdx.x, bs = blockDim.x, tid = threadIdx.x;
+    const int w = nd.window;
+    __shared__ double wclk[kMaxWindowSteps + 1];
+    extern __shared__ double heads_lds[];   // StreamHeads of the block's rows
+    const int role = b < b0 ? 0 : (b < b0 + b1 ? 1 : 2);
+    const int G = role == 0 ? 1 : (role == 1 ? GA : GB);
+    const int wpb = bs >> 6;
+    const int nwaves = (role == 0 ? b0 : (role == 1 ? b1 : (int)gridDim.x - b0 - b1)) * wpb;
+    const int wave = (b - (role == 0 ? 0 : (role == 1 ? b0 : b0 + b1))) * wpb + (tid >> 6);
+    const int lane = tid & 63, r = lane % G;
+    const int cap = nd.wl_cap[role];
+    const int* list_now = nd.wl_list[role] + (size_t)(w & 1) * cap;
+    // the window's clock, the list's count and the lane's first item: all
+    // issued before the block waits on the clock
+    const int nclk = t1 - t0 + 2;
+    const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
+    const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
+    const int k0 = (lane / G) * nwaves + wave;
+    const int n_items = nd.wl_cnt[3 * role + w % 3];
+    const int first = k0 < cap ? list_now[k0] : 0;
+    if (tid < nclk) wclk[tid] = c0;
+    if (tid + bs < nclk) wclk[tid + bs] = c1;
+    for (int k = tid + 2 * bs; k < nclk; k += bs) wclk[k] = nd.clock[t0 - 1 + k];
+    if (b == 0 && tid == 0) {
+        // the other post-event buffer was read by the previous window's STDP (in
+        // this window's integration); the next window's integration fills it
+        *nd.post_list_next_cnt = 0;
+        // the lists of window w + 2 start empty (window w - 1's, done with)
+        for (int q = 0; q < 3; ++q) nd.wl_cnt[3 * q + (w + 2) % 3] = 0;
+    }
+    __syncthreads();
+    int* list_next = nd.wl_list[role] + (size_t)((w + 1) & 1) * cap;
+    int* cnt_next = nd.wl_cnt + 3 * role + (w + 1) % 3;
+    const StreamHeads hd{heads_lds + tid, (int*)(heads_lds + kHeadDoubles * kRowHeads * bs) + tid, bs};
+    for (int k = k0; k < n_items; k += nwaves * (64 / G)) {
+        const int i = k == k0 ? first : list_now[k];
+        int left;
+        if (role == 0) {
+            const int growth = nd.growth_mode == 2 ? nd.net_growth[nd.ra_net[i]] : nd.growth_mode;
+            if (growth)
+                left = rr_row(nd, wclk, i, t0, t1, hd);
+            else
+                left = push_fixed<1>(nd, wclk, i, 0, nd.rr_ptr[i], nd.rr_ptr[i] + nd.act_cnt[i], nd.chain_e,
+                                     nd.rr_tgt, nd.w, nd.rr_d, nd.fl_rr, nd.fl_rr.head[i], nd.soma_mask[i], t0, t1,
+                                     nd.in_ge_ra);
+        } else if (role == 1) {
+            left = push_fixed<GA>(nd, wclk, i, r, nd.ra2i.ptr[i], nd.ra2i.ptr[i + 1], nullptr, nd.ra2i.tgt,
+                                  nd.ra2i.w, nd.ra2i.d, nd.fl_ra2i, nd.fl_ra2i.head[i], nd.soma_mask[i], t0, t1,
+                                  nd.in_ge_i);
+        } else {
+            left = push_fixed<GB>(nd, wclk, i, r, nd.i2ra.ptr[i], nd.i2ra.ptr[i + 1], nullptr, nd.i2ra.tgt,
+                                  nd.i2ra.w, nd.i2ra.d, nd.fl_i2ra, nd.fl_i2ra.head[i], nd.i_mask[i], t0, t1,
+                                  nd.in_gi_ra);
+        }
+        // still in flight: listed for the next window
+        if (left > 0 && r == 0) {
+            const int pos = atomicAdd(cnt_next, 1);
+            if (pos < cap) {
+                list_next[pos] = i;
+                nd.wl_mark[role][i] = w + 1;
+            } else {
+                atomicOr(nd.err, (unsigned)kErrWorklist);
+            }
+        }
     }
 }
 
@@ -1028,6 +1115,7 @@ void launch_reset(hipStream_t st, const NetDev& nd, int E)
 {
     int n = nd.n_ra > nd.n_i ? nd.n_ra : nd.n_i;
     if (3 * nd.n_nets > n) n = 3 * nd.n_nets;
+    if (kWlCounters > n) n = kWlCounters;
     k_reset<<<blocks_for(n, 256), 256, 0, st>>>(nd);
     if (E > 0) k_clear_pre<<<blocks_for(E, 256), 256, 0, st>>>(nd.pre_cnt, E);
 }
@@ -1036,16 +1124,31 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuni
 {
     const int bs = tu.deliver_block;
     const size_t lds = (size_t)kRowHeads * bs * (kHeadDoubles * sizeof(double) + kHeadInts * sizeof(int));
+    // blocks of a role: enough waves for one lane (group) per neuron, at most
+    // kDeliverRoleWaves (the items then loop over the role's waves)
+    auto role_blocks = [&](long long lanes) {
+        const long long waves = tu.deliver_lists ? std::min<long long>((lanes + 63) / 64, kDeliverRoleWaves)
+                                                 : (lanes + 63) / 64;
+        return (int)((waves * 64 + bs - 1) / bs);
+    };
     // a quad per HVC-I neuron shortens the chain of inbox atomics of its ~30
     // I->RA edges when few waves are in flight (one network); an HVC-RA neuron's
     // few RA->I edges take one lane.  With many networks the extra waves cost
     // more than they save, and every push takes one lane.
     if (nd.n_ra + nd.n_i <= kQuadPushNeurons) {
-        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(nd.n_ra, bs), b2 = blocks_for(4 * nd.n_i, bs);
-        k_deliver<1, 4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        const int b0 = role_blocks(nd.n_ra), b1 = role_blocks(nd.n_ra), b2 = role_blocks(4ll * nd.n_i);
+        if (b0 + b1 + b2 == 0) return;
+        if (tu.deliver_lists)
+            k_deliver_list<1, 4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        else
+            k_deliver_sweep<1, 4><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
     } else {
-        const int b0 = blocks_for(nd.n_ra, bs), b1 = blocks_for(nd.n_ra, bs), b2 = blocks_for(nd.n_i, bs);
-        k_deliver<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        const int b0 = role_blocks(nd.n_ra), b1 = role_blocks(nd.n_ra), b2 = role_blocks(nd.n_i);
+        if (b0 + b1 + b2 == 0) return;
+        if (tu.deliver_lists)
+            k_deliver_list<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        else
+            k_deliver_sweep<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
     }
 }
 

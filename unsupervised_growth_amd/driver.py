@@ -201,9 +201,9 @@ class GrowthDriver:
         d["num_test_trials"] = num_trials
         return d
 
-    def set_tuning(self, integrate_block, i_lanes, deliver_block, sync_block, ra_lanes=0):
+    def set_tuning(self, integrate_block, i_lanes, deliver_block, sync_block, ra_lanes=0, deliver_lists=1):
         """Launch shapes of the driver's engine (kept across rebuilds)."""
-        t = _abi.Tuning(integrate_block, i_lanes, deliver_block, sync_block, ra_lanes)
+        t = _abi.Tuning(integrate_block, i_lanes, deliver_block, sync_block, ra_lanes, deliver_lists)
         st = self.lib.hvc_set_tuning(self.engine_ctx(), ctypes.byref(t))
         if st != _abi.HVC_OK:
             raise EngineError(st, "hvc_set_tuning")
