@@ -67,9 +67,17 @@ def parse():
                          "(skips the autotuner)")
     ap.add_argument("--driver-trials", type=int, default=2,
                     help="c3: growth-loop trials through the C++ driver (bookkeeping included); 0 skips")
-    ap.add_argument("--requests", type=int, default=16, help="c5: number of requests")
+    ap.add_argument("--requests", type=int, default=16, help="c5: requests of the first load point")
     ap.add_argument("--trials", type=int, default=50, help="c5: chain-test trials per request")
-    ap.add_argument("--rate", type=float, default=20.0, help="c5: request arrivals per second")
+    ap.add_argument("--rate", type=float, default=20.0, help="c5: arrivals per second of the first load point")
+    ap.add_argument("--sustain-requests", type=int, default=100,
+                    help="c5: requests of the second load point (0 skips it)")
+    ap.add_argument("--sustain-rate", type=float, default=1.0,
+                    help="c5: arrivals per second of the second load point (one the device sustains)")
+    ap.add_argument("--serve", default="async", choices=["async", "rounds"],
+                    help="c5: hvc_submit / hvc_poll (async) or rounds driven by the caller (hvc_pool_run)")
+    ap.add_argument("--check-requests", type=int, default=3, help="c5: requests re-run alone in FP64")
+    ap.add_argument("--check-trials", type=int, default=10, help="c5: trials per checked request")
     ap.add_argument("--variants", type=int, default=2, help="c5: networks built per size")
     ap.add_argument("--hbm-gib", type=float, default=8.0, help="c5: device memory of the serving pool")
     return ap.parse_args()
@@ -302,7 +310,6 @@ def bench_serve(args):
 
     t_build = time.perf_counter()
     nets = configs.c5_networks(variants=args.variants)
-    reqs0 = configs.c5_requests(nets, n_requests=args.requests, rate=args.rate, trials=args.trials)
     t_build = time.perf_counter() - t_build
     all_nets = [n for v in nets.values() for n in v]
     sizes = [network_sizes(n) for n in all_nets]
@@ -328,29 +335,85 @@ def bench_serve(args):
     max_ra = int(args.hbm_gib * (1 << 30) / (unit.value / 1000.0))
     caps = pool_caps(64, max_ra, int(max_ra * per_ra["i"]), int(max_ra * per_ra["rr"]), int(max_ra * per_ra["a2i"]),
                      int(max_ra * per_ra["i2ra"]), indeg + 8, 128)
-    out = {}
-    with Pool(caps, **kw) as pool:
-        reqs = [Request(rid=k, arrival=a, net=n, key=key, onsets=on, mode=1, first_trial_index=0)
-                for k, (a, n, key, on) in enumerate(reqs0)]
-        stats = Server(pool).run(reqs, realtime=True, keep_results=False)
-        out = dict(stats, pool_gib=pool.pool_bytes() / (1 << 30))
-    lat = sorted(r.finished - r.arrival for r in reqs)
+    from unsupervised_growth_amd.serve import AsyncServer
+    from unsupervised_growth_amd.spikes import compare_results
+
+    def serve(rate, n_req, seed):
+        reqs0 = configs.c5_requests(nets, n_requests=n_req, rate=rate, trials=args.trials, seed=seed)
+        with Pool(caps, **kw) as pool:
+            reqs = [Request(rid=k, arrival=a, net=n, key=key, onsets=on, mode=1, first_trial_index=0)
+                    for k, (a, n, key, on) in enumerate(reqs0)]
+            if args.serve == "async":
+                stats = AsyncServer(pool).run(reqs, keep_results=True)
+            else:
+                stats = Server(pool).run(reqs, realtime=True, keep_results=True)
+            stats["pool_gib"] = pool.pool_bytes() / (1 << 30)
+        lat = sorted(r.finished - r.arrival for r in reqs)
+        pt = {"offered_rate": rate, "requests": n_req, "requests_per_s": n_req / stats["wall_s"],
+              "neuron_updates_per_s": stats["neuron_updates_per_s"], "wall_s": stats["wall_s"],
+              "rounds": stats["rounds"],
+              "latency_s": {"p50": stats["latency_p50"], "p99": stats["latency_p99"], "max": stats["latency_max"],
+                            "min": lat[0]}}
+        return reqs, pt, stats
+
+    points = []
+    reqs_sus = None
+    for rate, n_req, seed in ((args.rate, args.requests, 7), (args.sustain_rate, args.sustain_requests, 8)):
+        if n_req <= 0:
+            continue
+        reqs, pt, stats = serve(rate, n_req, seed)
+        points.append(pt)
+        if seed == 8 or reqs_sus is None:
+            reqs_sus, pool_gib = reqs, stats["pool_gib"]
+
+    # FP32 served results against the FP64 form of the same request alone (one
+    # request of each network size, its first check_trials trials)
+    precision_check = None
+    if precision == 1 and args.check_requests > 0:
+        from unsupervised_growth_amd.engine import Engine
+        tot = {"max_abs_dt_ms": 0.0, "unpaired_spikes": 0, "paired_within_one_step": 0, "spikes": 0,
+               "requests": 0, "trials_per_request": 0}
+        picked, seen = [], set()
+        for r in reqs_sus:
+            if r.net.n_ra not in seen and len(picked) < args.check_requests:
+                seen.add(r.net.n_ra)
+                picked.append(r)
+        ntr = min(args.check_trials, args.trials)
+        for r in picked:
+            with Engine(precision=0, trial_ms=500.0, ra_model=args.ra_model, i_model=args.i_model,
+                        seed=r.key) as e:
+                e.upload(r.net)
+                for k in range(ntr):
+                    ref = e.run_trial(**r.trial_kw(k))
+                    c = compare_results(ref, r.results[k], window=1.0)
+                    tot["max_abs_dt_ms"] = max(tot["max_abs_dt_ms"], max(v["max_abs_dt_ms"] for v in c.values()))
+                    tot["unpaired_spikes"] += sum(v["unpaired"] for v in c.values())
+                    tot["paired_within_one_step"] += sum(v["paired_within_one_step"] for v in c.values())
+                    tot["spikes"] += sum(v["spikes_ref"] for v in c.values())
+            tot["requests"] += 1
+        tot["trials_per_request"] = ntr
+        tot["network_sizes"] = [r.net.n_ra for r in picked]
+        precision_check = dict(reference="FP64 form of the same request run alone (bit-exact with the oracle)",
+                               dt_ms=0.02, **tot)
+    main_pt = points[-1]
     line = {
         "metric": "served chain-test requests: p50/p99 latency and neuron-updates/s",
-        "value": out["neuron_updates_per_s"], "unit": "neuron-updates/s", "n_gpus": 1,
-        "steps": args.requests, "warmup": 0, "ms_per_step": 1000.0 * out["wall_s"] / args.requests,
+        "value": main_pt["neuron_updates_per_s"], "unit": "neuron-updates/s", "n_gpus": 1,
+        "steps": main_pt["requests"], "warmup": 0, "ms_per_step": 1000.0 * main_pt["wall_s"] / main_pt["requests"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if precision == 0 else "f32",
         "data": "synthetic grown chain networks, Poisson arrivals (seeded)",
-        "config": {"workload": "%d chain-test requests x %d trials of 500 ms, Poisson arrivals %g/s, N_RA 1000/4000/20000 "
-                               "(25/50/25 %%), N_I = 0.275 N_RA, continuous batching on a %.1f GiB pool"
-                               % (args.requests, args.trials, args.rate, out["pool_gib"]),
+        "config": {"workload": "chain-test requests x %d trials of 500 ms, Poisson arrivals, N_RA 1000/4000/20000 "
+                               "(25/50/25 %%), N_I = 0.275 N_RA, continuous batching on a %.1f GiB pool (%s)"
+                               % (args.trials, pool_gib, "hvc_submit / hvc_poll" if args.serve == "async"
+                                  else "rounds driven by the caller"),
                    "config": "c5", "network_build_s": round(t_build, 1)},
-        "latency_s": {"p50": out["latency_p50"], "p99": out["latency_p99"], "max": out["latency_max"],
-                      "min": lat[0]},
-        "requests_per_s": args.requests / out["wall_s"],
-        "rounds": out["rounds"], "busy_fraction": out["busy_s"] / out["wall_s"],
+        "latency_s": main_pt["latency_s"],
+        "requests_per_s": main_pt["requests_per_s"],
+        "load_points": points,
     }
+    if precision_check is not None:
+        line["precision_check"] = precision_check
     print(json.dumps(line), flush=True)
 
 

@@ -30,8 +30,9 @@ typedef enum {
     HVC_E_HIP = 3,      /* HIP runtime error */
     HVC_E_NUMERIC = 4,  /* check_bad_values flagged an HVC-RA neuron (HH2_test.cpp:244-260) */
     HVC_E_STATE = 5,    /* call out of order (e.g. run before upload) */
-    HVC_E_CAPACITY = 6  /* an engine buffer overflowed; the trial's spikes are discarded (no spike query
+    HVC_E_CAPACITY = 6, /* an engine buffer overflowed; the trial's spikes are discarded (no spike query
                            answers until the next trial) and its synapse state is undefined */
+    HVC_PENDING = 7     /* hvc_poll: the submitted trial has not finished yet (not an error) */
 } hvc_status;
 
 /* SynapticParameters, c++/include/ConfigurationNetworkGrowth.h:11-34 */
@@ -212,6 +213,31 @@ hvc_status hvc_pool_release(hvc_ctx* ctx, int handle);
 hvc_status hvc_pool_run(hvc_ctx* ctx, int n, const int* handles, const hvc_trial* trials, hvc_trial_out* outs);
 hvc_status hvc_pool_spikes(hvc_ctx* ctx, int handle, int kind, int* counts, double* times, int cap, int* total);
 hvc_status hvc_pool_download_synapses(hvc_ctx* ctx, int handle, hvc_synapses* syn);
+
+/* Asynchronous serving on a pool: the serving entry points of the build's
+ * boundary, for the chain-test requests the reference serves one at a time
+ * (test_chain, NGS:1313-1482, trial at NGS:1406) and for growth trials.
+ * hvc_submit queues one trial of resident network `handle` and returns at
+ * once with a ticket.  A host thread of the context (started by the first
+ * submission, stopped by hvc_destroy) runs rounds: a round takes the oldest
+ * queued trial of every network that has one and runs them together, as
+ * hvc_pool_run would; networks with nothing queued sit the round out
+ * unchanged.  A network's trials run in submission order, one per round; a
+ * trial submitted while a round runs joins a later round.  Results equal the
+ * same trials run alone (hvc_run_trial on the network with the same key).
+ * hvc_poll returns HVC_PENDING until the ticket's trial is done, with *round
+ * (may be NULL) -1 while queued, else the index of the round that runs it;
+ * then the trial's status (HVC_OK, HVC_E_NUMERIC as hvc_run_trial, or the
+ * round's error) with out filled as by hvc_run_trial (counts per neuron of the
+ * network; times, when given, need *_cap >= the totals, else HVC_E_ARG with
+ * the totals set and the ticket kept), and the ticket is released.  With out
+ * NULL the call reports the status only and keeps the ticket.  hvc_submit and hvc_poll may be called from any thread while a round
+ * runs; hvc_pool_admit, _release, _run, _spikes and _download_synapses wait
+ * for the round boundary; hvc_pool_release refuses a network whose submitted
+ * trials have not finished.  A handle that is not resident when its round
+ * comes fails that ticket with HVC_E_STATE.  No counterpart in the reference. */
+hvc_status hvc_submit(hvc_ctx* ctx, int handle, const hvc_trial* trial, uint64_t* ticket);
+hvc_status hvc_poll(hvc_ctx* ctx, uint64_t ticket, hvc_trial_out* out, long long* round);
 
 /* Single-neuron traces on the device: the neuron seam of the reference
  * (HH2_test::Debraband_step_no_target_update HH2_test.cpp:535-545,

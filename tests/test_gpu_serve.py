@@ -7,6 +7,8 @@ Each request's spikes (and, for a growth request, its final synapse state) must
 equal those of the same network and trials run alone, bit for bit, in FP64 and
 in FP32.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -71,3 +73,82 @@ def test_served_requests_equal_alone(precision):
             for f in ("w", "act_cnt", "act_ids", "sup_cnt", "sup_ids", "remodeled"):
                 np.testing.assert_array_equal(getattr(r.synapses, f), getattr(syn, f), err_msg=f)
     assert spikes > 0
+
+
+def test_submit_mid_round_joins_next_round_and_equals_alone():
+    """hvc_submit / hvc_poll: a trial submitted while a round runs joins the
+    next round; every ticket's result equals its network's trial run alone
+    (FP64, bit for bit), and a network's queued trials run in order."""
+    import ctypes
+    from unsupervised_growth_amd.engine import Pool, pool_caps
+    reqs = _requests()
+    a, b, g = reqs[1], reqs[0], reqs[3]          # chain tests on 100 and 60 RA, growth on 50 RA
+    trial_ms = 200.0
+    caps = pool_caps(max_nets=3, max_ra=400, max_i=120, max_rr=4000, max_ra2i=20000, max_i2ra=20000,
+                     max_in_degree=200, spikes_per_neuron=128)
+    with Pool(caps, precision=0, trial_ms=trial_ms) as pool:
+        ha, hb, hg = pool.admit(a.net, a.key), pool.admit(b.net, b.key), pool.admit(g.net, g.key)
+        print("admitted", ha, hb, hg, flush=True)
+        ta = pool.submit(ha, **a.trial_kw(0))
+        rnd = ctypes.c_longlong(-1)
+        deadline = time.monotonic() + 60
+        while rnd.value < 0:                      # until the serving thread has taken a's trial
+            assert time.monotonic() < deadline, "the serving thread never took the trial"
+            pool.lib.hvc_poll(pool.ctx, ta, None, ctypes.byref(rnd))
+            time.sleep(0.0002)
+        tb = pool.submit(hb, **b.trial_kw(0))
+        tg = [pool.submit(hg, **g.trial_kw(k)) for k in range(3)]
+        got = {}
+        deadline = time.monotonic() + 120
+        for t in [ta, tb] + tg:
+            while t not in got:
+                assert time.monotonic() < deadline, "ticket %d never finished" % t
+                r = pool.poll(t)
+                if r is not None:
+                    got[t] = r
+                else:
+                    time.sleep(0.001)
+        print({t: r[0] for t, r in got.items()}, flush=True)
+        assert got[tb][0] > got[ta][0]
+        assert [got[t][0] for t in tg] == sorted(got[t][0] for t in tg)
+        assert len(set(got[t][0] for t in tg)) == 3
+        syn_g = pool.download_synapses(hg)
+        with pytest.raises(Exception):
+            pool.poll(ta)                          # a polled ticket is gone
+    for req, res in ((a, [got[ta][1]]), (b, [got[tb][1]]), (g, [got[t][1] for t in tg])):
+        n = len(res)
+        req.onsets = req.onsets[:n]
+        alone, syn = _alone(req, 0, trial_ms)
+        for k, (x, y) in enumerate(zip(alone, res)):
+            for kind in ("soma", "dend", "interneuron"):
+                for u, v in zip(getattr(x, kind), getattr(y, kind)):
+                    np.testing.assert_array_equal(u, v, err_msg="request %d trial %d %s" % (req.rid, k, kind))
+        if syn is not None:
+            for f in ("w", "act_cnt", "act_ids", "sup_cnt", "sup_ids", "remodeled"):
+                np.testing.assert_array_equal(getattr(syn_g, f), getattr(syn, f), err_msg=f)
+
+
+def test_async_server_equals_alone():
+    """AsyncServer (arrivals on the wall clock, trials through hvc_submit):
+    every request's results equal the same network's trials run alone."""
+    from unsupervised_growth_amd.engine import Pool, pool_caps
+    from unsupervised_growth_amd.serve import AsyncServer
+    trial_ms = 200.0
+    reqs = _requests()
+    for k, r in enumerate(reqs):
+        r.arrival = 0.02 * r.arrival              # rounds -> a few tens of ms apart
+    caps = pool_caps(max_nets=3, max_ra=400, max_i=120, max_rr=4000, max_ra2i=20000, max_i2ra=20000,
+                     max_in_degree=200, spikes_per_neuron=128)
+    with Pool(caps, precision=0, trial_ms=trial_ms) as pool:
+        stats = AsyncServer(pool).run(reqs, timeout_s=120)
+    print(stats, flush=True)
+    for r in reqs:
+        alone, syn = _alone(r, 0, trial_ms)
+        assert len(r.results) == len(alone)
+        for k, (x, y) in enumerate(zip(alone, r.results)):
+            for kind in ("soma", "dend", "interneuron"):
+                for u, v in zip(getattr(x, kind), getattr(y, kind)):
+                    np.testing.assert_array_equal(u, v, err_msg="request %d trial %d %s" % (r.rid, k, kind))
+        if syn is not None:
+            for f in ("w", "act_cnt", "act_ids", "sup_cnt", "sup_ids", "remodeled"):
+                np.testing.assert_array_equal(getattr(r.synapses, f), getattr(syn, f), err_msg=f)

@@ -14,9 +14,9 @@ c_int_p = ctypes.POINTER(ctypes.c_int)
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_u8_p = ctypes.POINTER(ctypes.c_uint8)
 
-HVC_OK, HVC_E_ARG, HVC_E_OOM, HVC_E_HIP, HVC_E_NUMERIC, HVC_E_STATE, HVC_E_CAPACITY = range(7)
+HVC_OK, HVC_E_ARG, HVC_E_OOM, HVC_E_HIP, HVC_E_NUMERIC, HVC_E_STATE, HVC_E_CAPACITY, HVC_PENDING = range(8)
 STATUS_NAMES = {0: "HVC_OK", 1: "HVC_E_ARG", 2: "HVC_E_OOM", 3: "HVC_E_HIP", 4: "HVC_E_NUMERIC",
-                5: "HVC_E_STATE", 6: "HVC_E_CAPACITY"}
+                5: "HVC_E_STATE", 6: "HVC_E_CAPACITY", 7: "HVC_PENDING"}
 
 # every entry point declared in include/hvc_engine.h
 EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_synapses",
@@ -25,7 +25,7 @@ EXPORTS = ["hvc_create", "hvc_upload_network", "hvc_run_trial", "hvc_download_sy
            "hvc_trace_ra_noalloc", "hvc_batch_spikes", "hvc_batch_download_synapses", "hvc_batch_upload_synapses",
            "hvc_pool_footprint",
            "hvc_pool_create", "hvc_pool_admit", "hvc_pool_release", "hvc_pool_run", "hvc_pool_spikes",
-           "hvc_pool_download_synapses", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
+           "hvc_pool_download_synapses", "hvc_submit", "hvc_poll", "hvc_get_tuning", "hvc_set_tuning", "hvc_autotune", "hvc_destroy",
            "hvc_last_error", "hvc_set_branch_counting", "hvc_branch_counts"]
 BRANCH_COUNTERS = ("activate", "super", "unsuper", "deactivate", "remodel", "ltd_delivery", "ltd_delivery_saturated",
                    "desaturation", "sync_ltp", "sync_ltd", "sync_saturated")
@@ -185,6 +185,8 @@ def load(path=LIB_PATH):
     lib.hvc_pool_run.argtypes = [vp, ctypes.c_int, c_int_p, P(Trial), P(TrialOut)]
     lib.hvc_pool_spikes.argtypes = [vp, ctypes.c_int, ctypes.c_int, c_int_p, c_double_p, ctypes.c_int, c_int_p]
     lib.hvc_pool_download_synapses.argtypes = [vp, ctypes.c_int, P(Synapses)]
+    lib.hvc_submit.argtypes = [vp, ctypes.c_int, P(Trial), P(ctypes.c_uint64)]
+    lib.hvc_poll.argtypes = [vp, ctypes.c_uint64, P(TrialOut), P(ll)]
     lib.hvc_get_tuning.argtypes = [vp, P(Tuning)]
     lib.hvc_set_tuning.argtypes = [vp, P(Tuning)]
     lib.hvc_autotune.argtypes = [vp, ctypes.c_int, P(Tuning), c_double_p]

@@ -8,12 +8,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "hvc_internal.h"
@@ -112,6 +119,31 @@ struct PoolNet {
     std::vector<double> spread;
 };
 
+// Asynchronous serving on a pool (hvc_submit / hvc_poll): one trial of one
+// resident network per ticket, run by the context's serving thread in rounds.
+struct ServeTicket {
+    int handle = 0;
+    hvc_trial trial{};
+    long long round = -1;          // -1 while queued, else the round that runs it
+    bool done = false;
+    hvc_status status = HVC_OK;
+    std::string err;
+    int bad_values = 0, warnings[3] = {0, 0, 0};
+    std::vector<int> counts[3];    // soma, dendrite, HVC-I: per neuron of the network
+    std::vector<double> times[3];
+};
+
+struct Serving {
+    std::mutex q_mu;               // guards everything below (never held while waiting on api_mu)
+    std::condition_variable cv;    // the serving thread waits on it for work or stop
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_ticket = 1;
+    long long rounds = 0;
+    std::deque<uint64_t> queue;    // queued tickets in submission order
+    std::unordered_map<uint64_t, ServeTicket> tickets;
+};
+
 struct hvc_ctx {
     hvc_params prm;
     int device = 0;
@@ -162,6 +194,11 @@ struct hvc_ctx {
 
     // serving pool
     bool pool = false;
+    std::mutex api_mu;               // held by the pool calls and by the serving thread through a round
+    std::atomic<int> api_waiters{0}; // pool calls waiting for api_mu (the serving thread lets them in first)
+    std::unique_ptr<Serving> serve;  // hvc_submit / hvc_poll
+    int* d_net_live = nullptr;       // per network: 1 = runs this round (serving rounds only)
+    std::vector<int> h_live;
     std::vector<PoolNet> pnets;
     int next_handle = 1;
     long long a_tot = 0, b_tot = 0;
@@ -188,6 +225,21 @@ struct hvc_ctx {
     std::vector<std::pair<int, size_t>> ev_pending;  // (class, index of start event)
     double t_sum[kNumTimers] = {};
     int t_cnt[kNumTimers] = {};
+};
+
+// api_mu for a pool call: announced first, so that the serving thread lets the
+// call in at the next round boundary instead of starting another round
+struct ApiLock {
+    hvc_ctx* c;
+    explicit ApiLock(hvc_ctx* ctx) : c(ctx)
+    {
+        c->api_waiters.fetch_add(1);
+        c->api_mu.lock();
+        c->api_waiters.fetch_sub(1);
+    }
+    ~ApiLock() { c->api_mu.unlock(); }
+    ApiLock(const ApiLock&) = delete;
+    ApiLock& operator=(const ApiLock&) = delete;
 };
 
 #define HVC_TRY(expr)                                                                          \
@@ -336,6 +388,14 @@ extern "C" hvc_status hvc_create(const hvc_params* prm, int device, hvc_ctx** ou
 extern "C" void hvc_destroy(hvc_ctx* ctx)
 {
     if (!ctx) return;
+    if (ctx->serve) {
+        {
+            std::lock_guard<std::mutex> lq(ctx->serve->q_mu);
+            ctx->serve->stop = true;
+        }
+        ctx->serve->cv.notify_all();
+        if (ctx->serve->worker.joinable()) ctx->serve->worker.join();
+    }
     if (ctx->stream) {
         hipStreamSynchronize(ctx->stream);
         free_bufs(ctx);
@@ -1099,7 +1159,11 @@ static void noise_chunk(hvc_ctx* ctx, const NetDev& nd, int w)
     launch_noise_events(ctx->stream, nd, ctx->windows[w].first, ctx->windows[last].second, gen_ra, gen_i);
 }
 
-static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs)
+// One trial of every resident network.  live (host, one flag per network, or
+// nullptr: all) marks the networks that run; the others sit the trial out
+// (their neurons are integrated but nothing of them is stored, so they neither
+// spike nor change).
+static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out* outs, const int* live = nullptr)
 {
     const int B = ctx->n_nets;
     int any_growth = 0, any_chain = 0;
@@ -1137,6 +1201,12 @@ static hvc_status run_impl(hvc_ctx* ctx, const hvc_trial* trials, hvc_trial_out*
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, ctx->h_net_trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_first, ctx->h_net_first.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
     HVC_TRY(hipMemcpyAsync(ctx->d_net_growth, ctx->h_net_growth.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+    nd.net_live = nullptr;
+    if (live) {
+        ctx->h_live.assign(live, live + B);
+        HVC_TRY(hipMemcpyAsync(ctx->d_net_live, ctx->h_live.data(), sizeof(int) * B, hipMemcpyHostToDevice, s));
+        nd.net_live = ctx->d_net_live;
+    }
     refresh_rr_pos(ctx, nd);
     launch_reset(s, nd, ctx->E);
     if (any_chain) launch_chain_edges(s, nd);
@@ -1328,6 +1398,7 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
     HVC_TRY(hipMemcpyAsync(ctx->d_net_trial, trial.data(), sizeof(uint32_t) * B, hipMemcpyHostToDevice, s));
     refresh_rr_pos(ctx, nd);
     nd.growth_mode = 0;   // chain tests only
+    nd.net_live = nullptr;
     launch_reset(s, nd, ctx->E);
     launch_chain_edges(s, nd);
     const int nw = std::min(windows, (int)ctx->windows.size());
@@ -1485,6 +1556,10 @@ extern "C" hvc_status hvc_pool_create(const hvc_params* prm, int device, const h
     const long long spn = caps->spikes_per_neuron > 0 ? caps->spikes_per_neuron : 256;
     c.log_cap = spn * ((long long)caps->max_ra + caps->max_i);
     if ((r = alloc_device(ctx, c)) != HVC_OK) return r;
+    hvc_status st = HVC_OK;
+    ctx->d_net_live = dalloc<int>(ctx, (size_t)std::max(caps->max_nets, 1), &st);
+    if (st != HVC_OK) return st;
+    ctx->serve.reset(new Serving());
     ctx->pool = true;
     ctx->n_ra = ctx->n_i = ctx->E = 0;
     ctx->n_nets = 0;
@@ -1540,6 +1615,7 @@ extern "C" hvc_status hvc_pool_admit(hvc_ctx* ctx, const hvc_network* net, uint3
 {
     if (!ctx || !net || !handle) return HVC_E_ARG;
     if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    ApiLock lk(ctx);
     hvc_status r = check_network(ctx, net, "network");
     if (r != HVC_OK) return r;
     const int n = net->n_ra, ni = net->n_i;
@@ -1642,8 +1718,15 @@ extern "C" hvc_status hvc_pool_release(hvc_ctx* ctx, int handle)
 {
     if (!ctx) return HVC_E_ARG;
     if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    ApiLock lk(ctx);
     const int p = pool_position(ctx, handle);
     if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
+    {
+        std::lock_guard<std::mutex> lq(ctx->serve->q_mu);
+        for (const auto& kv : ctx->serve->tickets)
+            if (kv.second.handle == handle && !kv.second.done)
+                return fail(ctx, HVC_E_STATE, "the network has submitted trials that have not finished");
+    }
     const PoolNet gone = ctx->pnets[p];
     const int dn = gone.n_ra, di = gone.n_i, dE = gone.E, dA = gone.A, dB = gone.Bn;
     const int NssA = std::max(ctx->prm.syn.Nss, 1);
@@ -1712,6 +1795,7 @@ extern "C" hvc_status hvc_pool_run(hvc_ctx* ctx, int n, const int* handles, cons
 {
     if (!ctx || (n > 0 && (!handles || !trials))) return HVC_E_ARG;
     if (!ctx->pool) return fail(ctx, HVC_E_STATE, "not a pool context (hvc_pool_create)");
+    ApiLock lk(ctx);
     if (n != ctx->n_nets) return fail(ctx, HVC_E_ARG, "hvc_pool_run needs one trial for every resident network");
     if (n == 0) return HVC_OK;
     std::vector<hvc_trial> tp(n);
@@ -1734,6 +1818,7 @@ extern "C" hvc_status hvc_pool_spikes(hvc_ctx* ctx, int handle, int kind, int* c
                                       int* total)
 {
     if (!ctx) return HVC_E_ARG;
+    ApiLock lk(ctx);
     const int p = pool_position(ctx, handle);
     if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
     return hvc_batch_spikes(ctx, p, kind, counts, times, cap, total);
@@ -1742,9 +1827,198 @@ extern "C" hvc_status hvc_pool_spikes(hvc_ctx* ctx, int handle, int kind, int* c
 extern "C" hvc_status hvc_pool_download_synapses(hvc_ctx* ctx, int handle, hvc_synapses* syn)
 {
     if (!ctx) return HVC_E_ARG;
+    ApiLock lk(ctx);
     const int p = pool_position(ctx, handle);
     if (p < 0) return fail(ctx, HVC_E_ARG, "unknown network handle");
     return hvc_batch_download_synapses(ctx, p, syn);
+}
+
+// ---------------------------------------------------------------------------
+// asynchronous serving: hvc_submit / hvc_poll
+// ---------------------------------------------------------------------------
+// One round of the serving thread (api_mu held): the picked tickets' trials
+// run together, the other resident networks sit the round out; results go
+// into the tickets.
+static void serve_round(hvc_ctx* ctx, const std::vector<uint64_t>& pick, std::vector<ServeTicket>& res)
+{
+    const int B = ctx->n_nets;
+    std::vector<hvc_trial> tp(B, hvc_trial{1, 0, 0u, 0, 100.0});
+    std::vector<int> live(B, 0), at(pick.size(), -1);
+    for (size_t k = 0; k < pick.size(); ++k) {
+        const int p = pool_position(ctx, res[k].handle);
+        if (p < 0) {
+            res[k].status = HVC_E_STATE;
+            res[k].err = "the network is not resident";
+            continue;
+        }
+        tp[p] = res[k].trial;
+        live[p] = 1;
+        at[k] = p;
+    }
+    if (std::find(live.begin(), live.end(), 1) == live.end()) return;
+    std::vector<hvc_trial_out> op(B, hvc_trial_out{});
+    hvc_status r = run_impl(ctx, tp.data(), op.data(), live.data());
+    for (size_t k = 0; k < pick.size(); ++k) {
+        const int p = at[k];
+        if (p < 0) continue;
+        ServeTicket& t = res[k];
+        if (r != HVC_OK && r != HVC_E_NUMERIC) {
+            t.status = r;
+            t.err = ctx->err;
+            continue;
+        }
+        const int r0 = ctx->ra_off[p], r1 = ctx->ra_off[p + 1], i0 = ctx->i_off[p], i1 = ctx->i_off[p + 1];
+        const int lo[3] = {r0, r0, i0}, hi[3] = {r1, r1, i1};
+        hvc_status rs = HVC_OK;
+        for (int kind = 0; kind < 3 && rs == HVC_OK; ++kind) {
+            int total = 0;
+            t.counts[kind].assign(hi[kind] - lo[kind], 0);
+            rs = spikes_range(ctx, kind, lo[kind], hi[kind], nullptr, nullptr, 0, &total);
+            if (rs != HVC_OK) break;
+            t.times[kind].assign(total, 0.0);
+            rs = spikes_range(ctx, kind, lo[kind], hi[kind], t.counts[kind].data(), t.times[kind].data(), total,
+                              &total);
+        }
+        if (rs != HVC_OK) {
+            t.status = rs;
+            t.err = ctx->err;
+            continue;
+        }
+        t.bad_values = op[p].bad_values;
+        for (int q = 0; q < 3; ++q) t.warnings[q] = op[p].warnings[q];
+        t.status = t.bad_values ? HVC_E_NUMERIC : HVC_OK;
+        if (t.bad_values) t.err = "check_bad_values: HVC-RA voltage out of [-200, 100] or NaN";
+    }
+}
+
+static void serve_loop(hvc_ctx* ctx)
+{
+    Serving& sv = *ctx->serve;
+    const hipError_t dev = hipSetDevice(ctx->device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lq(sv.q_mu);
+            sv.cv.wait(lq, [&] { return sv.stop || !sv.queue.empty(); });
+            if (sv.stop) return;
+        }
+        // pool calls waiting at the round boundary go first (admissions, releases)
+        while (ctx->api_waiters.load() > 0) std::this_thread::yield();
+        std::vector<uint64_t> pick;
+        std::vector<ServeTicket> res;
+        std::unique_lock<std::mutex> la(ctx->api_mu);
+        {
+            // the oldest queued trial of every network that has one
+            std::lock_guard<std::mutex> lq(sv.q_mu);
+            std::vector<int> nets;
+            for (auto it = sv.queue.begin(); it != sv.queue.end();) {
+                ServeTicket& t = sv.tickets[*it];
+                if (std::find(nets.begin(), nets.end(), t.handle) == nets.end()) {
+                    nets.push_back(t.handle);
+                    t.round = sv.rounds;
+                    pick.push_back(*it);
+                    res.push_back(ServeTicket{});
+                    res.back().handle = t.handle;
+                    res.back().trial = t.trial;
+                    it = sv.queue.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            if (!pick.empty()) ++sv.rounds;
+        }
+        if (pick.empty()) continue;
+        if (dev != hipSuccess) {
+            for (auto& t : res) {
+                t.status = HVC_E_HIP;
+                t.err = std::string("hipSetDevice: ") + hipGetErrorString(dev);
+            }
+        } else {
+            serve_round(ctx, pick, res);
+        }
+        la.unlock();
+        std::lock_guard<std::mutex> lq(sv.q_mu);
+        for (size_t k = 0; k < pick.size(); ++k) {
+            auto it = sv.tickets.find(pick[k]);
+            if (it == sv.tickets.end()) continue;
+            ServeTicket& t = it->second;
+            t.status = res[k].status;
+            t.err = std::move(res[k].err);
+            t.bad_values = res[k].bad_values;
+            for (int q = 0; q < 3; ++q) {
+                t.warnings[q] = res[k].warnings[q];
+                t.counts[q] = std::move(res[k].counts[q]);
+                t.times[q] = std::move(res[k].times[q]);
+            }
+            t.done = true;
+        }
+    }
+}
+
+extern "C" hvc_status hvc_submit(hvc_ctx* ctx, int handle, const hvc_trial* trial, uint64_t* ticket)
+{
+    if (!ctx || !trial || !ticket) return HVC_E_ARG;
+    if (!ctx->pool || !ctx->serve) return HVC_E_STATE;
+    if (trial->mode != 0 && trial->mode != 1) return HVC_E_ARG;
+    Serving& sv = *ctx->serve;
+    {
+        std::lock_guard<std::mutex> lq(sv.q_mu);
+        if (sv.stop) return HVC_E_STATE;
+        if (!sv.worker.joinable()) {
+            try {
+                sv.worker = std::thread(serve_loop, ctx);
+            } catch (...) {
+                return HVC_E_OOM;
+            }
+        }
+        const uint64_t id = sv.next_ticket++;
+        ServeTicket& t = sv.tickets[id];
+        t.handle = handle;
+        t.trial = *trial;
+        sv.queue.push_back(id);
+        *ticket = id;
+    }
+    sv.cv.notify_all();
+    return HVC_OK;
+}
+
+extern "C" hvc_status hvc_poll(hvc_ctx* ctx, uint64_t ticket, hvc_trial_out* out, long long* round)
+{
+    if (!ctx) return HVC_E_ARG;
+    if (!ctx->pool || !ctx->serve) return HVC_E_STATE;
+    Serving& sv = *ctx->serve;
+    std::lock_guard<std::mutex> lq(sv.q_mu);
+    auto it = sv.tickets.find(ticket);
+    if (it == sv.tickets.end()) return HVC_E_ARG;
+    ServeTicket& t = it->second;
+    if (round) *round = t.round;
+    if (!t.done) return HVC_PENDING;
+    if (!out) return t.status;   // status only: the ticket stays
+    if (t.status != HVC_OK && t.status != HVC_E_NUMERIC) {
+        const hvc_status st = t.status;
+        sv.tickets.erase(it);
+        return st;
+    }
+    {
+        int* cnt[3] = {out->soma_counts, out->dend_counts, out->i_counts};
+        double* tm[3] = {out->soma_times, out->dend_times, out->i_times};
+        const int cap[3] = {out->soma_cap, out->dend_cap, out->i_cap};
+        int* tot[3] = {&out->soma_total, &out->dend_total, &out->i_total};
+        bool small = false;
+        for (int k = 0; k < 3; ++k) {
+            *tot[k] = (int)t.times[k].size();
+            if (tm[k] && (int)t.times[k].size() > cap[k]) small = true;
+        }
+        if (small) return HVC_E_ARG;   // the ticket stays; totals give the sizes needed
+        for (int k = 0; k < 3; ++k) {
+            if (cnt[k] && !t.counts[k].empty()) std::memcpy(cnt[k], t.counts[k].data(), sizeof(int) * t.counts[k].size());
+            if (tm[k] && !t.times[k].empty()) std::memcpy(tm[k], t.times[k].data(), sizeof(double) * t.times[k].size());
+        }
+        out->bad_values = t.bad_values;
+        for (int k = 0; k < 3; ++k) out->warnings[k] = t.warnings[k];
+    }
+    const hvc_status st = t.status;
+    sv.tickets.erase(it);
+    return st;
 }
 
 // ---------------------------------------------------------------------------

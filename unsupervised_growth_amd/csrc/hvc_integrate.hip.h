@@ -412,7 +412,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
         gs[0] = kRAGates[2 * (lane & 3)];
         gs[1] = kRAGates[2 * (lane & 3) + 1];
     }
-    const bool live = i < nd.n_ra;
+    const bool live = i < nd.n_ra && (!nd.net_live || nd.net_live[nd.ra_net[i]]);
     const int ii = live ? i : 0;
     const int growth = nd.net_growth[nd.ra_net[ii]];
     const int listed0 = nd.wl_on ? nd.wl_mark[0][ii] : 0, listed1 = nd.wl_on ? nd.wl_mark[1][ii] : 0;
@@ -541,7 +541,7 @@ __device__ void integrate_ra(const NetDev& nd, const Win& win, int slot, int t0,
 template <int MODEL>
 __device__ void integrate_i_thread(const NetDev& nd, const Win& win, int i, int t0, int t1)
 {
-    const bool live = i < nd.n_i;
+    const bool live = i < nd.n_i && (!nd.net_live || nd.net_live[nd.i_net[i]]);
     const int ii = live ? i : 0;
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
@@ -600,7 +600,7 @@ __device__ void integrate_i_lanes(const NetDev& nd, const Win& win, int wave, in
     const int group = spare ? kINeuronsPerWave - 1 : lane / kILanes;
     const int base = group * kILanes;
     const int i = wave * kINeuronsPerWave + group;
-    const bool live = (i < nd.n_i) && !spare;
+    const bool live = (i < nd.n_i) && !spare && (!nd.net_live || nd.net_live[nd.i_net[i]]);
     const int ii = (i < nd.n_i) ? i : 0;
     IState s;
     HVC_LOAD(s, V, nd.in[0][ii]) HVC_LOAD(s, n, nd.in[1][ii]) HVC_LOAD(s, m, nd.in[2][ii])
@@ -665,7 +665,7 @@ __device__ void integrate_i_quad(const NetDev& nd, const Win& win, int wave, int
     const int lane = __lane_id();
     const int role = lane & 3;
     const int i = wave * kINeuronsPerQuadWave + (lane >> 2);
-    const bool live = i < nd.n_i;
+    const bool live = i < nd.n_i && (!nd.net_live || nd.net_live[nd.i_net[i]]);
     const int ii = live ? i : 0;
     const GateSlot ga = kIQuadGates[role][0], gb = kIQuadGates[role][1];
     IState s;
@@ -734,7 +734,7 @@ __device__ void integrate_i_oct(const NetDev& nd, const Win& win, int wave, int 
     const int lane = __lane_id();
     const int q = lane & 7;
     const int i = wave * kINeuronsPerOctWave + (lane >> 3);
-    const bool live = i < nd.n_i;
+    const bool live = i < nd.n_i && (!nd.net_live || nd.net_live[nd.i_net[i]]);
     const int ii = live ? i : 0;
     const GateSlot g = kIOctGates[q];
     // the lane's own variable: V (lane 0), n (1, 6), m (2, 5), h (3, 4), w (7)

@@ -171,6 +171,7 @@ struct NetDev {
     const uint32_t* net_trial;   // [n_nets] trial index of the running trial
     const int* net_first;        // [n_nets] first trial after initialisation (NGS:849-850)
     const int* net_growth;       // [n_nets] 1: growth trial (STDP), 0: chain-test trial
+    const int* net_live;         // [n_nets] 0: the network sits the trial out (nullptr: every network runs)
 
     // RA state (SoA)
     double* ra[12];              // Vs Vd n l h r c Ca Ge_s Gi_s Ge_d Gi_d

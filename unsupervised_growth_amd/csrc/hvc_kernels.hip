@@ -87,6 +87,8 @@ __global__ void k_reset(NetDev nd)
         nd.in_ge_ra.cnt[i] = 0;
         nd.wl_mark[0][i] = -1;
         nd.wl_mark[1][i] = -1;
+        nd.soma_mask[i] = 0;   // (a network sitting the trial out never writes its masks)
+        nd.dend_mask[i] = 0;
     }
     if (i < nd.n_i) {
         nd.in[0][i] = -66.0; nd.in[1][i] = 0.125; nd.in[2][i] = 0.0; nd.in[3][i] = 0.99; nd.in[4][i] = 0.0;
@@ -103,6 +105,7 @@ __global__ void k_reset(NetDev nd)
         fl_clear(nd.fl_i2ra, i);
         nd.in_ge_i.cnt[i] = 0;
         nd.wl_mark[2][i] = -1;
+        nd.i_mask[i] = 0;
     }
     if (i == 0) {
         nd.log_cnt[0] = 0; nd.log_cnt[1] = 0; nd.log_cnt[2] = 0;

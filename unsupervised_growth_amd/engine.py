@@ -386,6 +386,7 @@ class Pool(Engine):
         self.net = None
         self.nets = []
         self.resident = {}      # handle -> Network
+        self._tickets = {}      # submitted ticket -> handle
 
     def footprint(self, net: Network):
         """Device bytes the network occupies in this pool."""
@@ -426,6 +427,12 @@ class Pool(Engine):
                                              len(times), ctypes.byref(tot)))
         return SpikeTrains(c, times[:tot.value]) if n else []
 
+    @staticmethod
+    def _trial(kw):
+        return _abi.Trial(int(kw.get("mode", 1)), int(bool(kw.get("training", True))),
+                          int(kw.get("trial_index", 0)) & 0xffffffff, int(bool(kw.get("first_trial", False))),
+                          float(kw.get("mean_injection_time", 100.0)))
+
     def run(self, trials):
         """trials: {handle: run_trial keyword dict} for every resident network;
         returns {handle: TrialResult}."""
@@ -435,10 +442,7 @@ class Pool(Engine):
         ts = (_abi.Trial * max(n, 1))()
         outs = (_abi.TrialOut * max(n, 1))()
         for k, h in enumerate(hs):
-            kw = trials[h]
-            ts[k] = _abi.Trial(int(kw.get("mode", 1)), int(bool(kw.get("training", True))),
-                               int(kw.get("trial_index", 0)) & 0xffffffff, int(bool(kw.get("first_trial", False))),
-                               float(kw.get("mean_injection_time", 100.0)))
+            ts[k] = self._trial(trials[h])
         st = self.lib.hvc_pool_run(self.ctx, n, _p(handles, _abi.c_int_p), ts, outs)
         if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
             self._check(st)
@@ -448,6 +452,44 @@ class Pool(Engine):
             res[h] = TrialResult(self._pool_spikes(h, 0, net.n_ra), self._pool_spikes(h, 1, net.n_ra),
                                  self._pool_spikes(h, 2, net.n_i), outs[k].bad_values, tuple(outs[k].warnings))
         return res
+
+    def submit(self, handle, **kw):
+        """Queue one trial of resident network `handle` (hvc_submit); returns the
+        ticket at once.  The pool's serving thread runs it in a later round."""
+        t = ctypes.c_uint64()
+        tr = self._trial(kw)
+        self._check(self.lib.hvc_submit(self.ctx, int(handle), ctypes.byref(tr), ctypes.byref(t)))
+        self._tickets[t.value] = handle
+        return t.value
+
+    def poll(self, ticket):
+        """hvc_poll: None while the ticket's trial is queued or running, else
+        (round, TrialResult); the ticket is then released.  Raises EngineError
+        when the trial failed (HVC_E_NUMERIC gives the result with bad_values)."""
+        net = self.resident[self._tickets[ticket]]
+        sizes = (net.n_ra, net.n_ra, net.n_i)
+        counts = [np.zeros(max(n, 1), np.int32) for n in sizes]
+        times = [np.zeros(1) for _ in sizes]
+        rnd = ctypes.c_longlong()
+        for attempt in range(2):
+            o = _abi.TrialOut()
+            o.soma_counts, o.dend_counts, o.i_counts = (_p(c, _abi.c_int_p) for c in counts)
+            o.soma_times, o.dend_times, o.i_times = (_p(t, _abi.c_double_p) for t in times)
+            o.soma_cap, o.dend_cap, o.i_cap = (len(t) if attempt else 0 for t in times)
+            st = self.lib.hvc_poll(self.ctx, ticket, ctypes.byref(o), ctypes.byref(rnd))
+            if st == _abi.HVC_PENDING:
+                return None
+            if st != _abi.HVC_E_ARG or attempt:
+                break
+            # spike-time buffers sized from the totals, then the same ticket again
+            times = [np.zeros(max(tot, 1)) for tot in (o.soma_total, o.dend_total, o.i_total)]
+        if st not in (_abi.HVC_OK, _abi.HVC_E_NUMERIC):
+            self._tickets.pop(ticket, None)
+            raise EngineError(st, "hvc_poll")
+        self._tickets.pop(ticket, None)
+        tots = (o.soma_total, o.dend_total, o.i_total)
+        trains = [SpikeTrains(counts[k][:sizes[k]], times[k][:tots[k]]) if sizes[k] else [] for k in range(3)]
+        return rnd.value, TrialResult(trains[0], trains[1], trains[2], o.bad_values, tuple(o.warnings))
 
     def download_synapses(self, handle) -> SynapseState:
         net = self.resident[handle]

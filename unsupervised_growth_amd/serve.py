@@ -9,7 +9,10 @@ releases the networks whose requests are complete.  A network stays on the
 device from admission to release, so a growth request's synapse state evolves
 in place; a request that arrives while others run joins at the next round.
 Admission is first come, first served: a request that does not fit keeps the
-ones behind it waiting, so large networks are not starved.
+ones behind it waiting, so large networks are not starved.  Server drives the
+rounds itself (hvc_pool_run); AsyncServer hands the trials to the pool's own
+serving thread (hvc_submit / hvc_poll), so that arrivals join mid-round work
+at the next round without waiting for the caller.
 
 Each request's result equals what the same network and trials give alone (the
 pool keys every network's noise by its own key and ids and sums over its own
@@ -126,6 +129,97 @@ class Server:
             "rounds": rounds,
             "wall_s": wall,
             "busy_s": busy,
+            "neuron_updates": updates,
+            "neuron_updates_per_s": updates / wall if wall > 0 else 0.0,
+            "latency_p50": float(np.percentile(lat, 50)),
+            "latency_p99": float(np.percentile(lat, 99)),
+            "latency_max": float(lat.max()),
+        }
+
+
+class AsyncServer:
+    """Requests served through hvc_submit / hvc_poll.
+
+    An arriving request's network is admitted as soon as it fits (first come,
+    first served) and all of its trials are submitted at once; the pool's
+    serving thread runs rounds of whatever is queued (one trial per network
+    and round, a network's trials in order), so a request that arrives while a
+    round runs has its first trial in the next round, and networks of finished
+    requests leave at the next round boundary.  The caller's thread only admits,
+    submits, polls and releases."""
+
+    def __init__(self, pool: Pool):
+        self.pool = pool
+
+    def run(self, requests, keep_results=True, poll_s=0.0005, log=None, timeout_s=None):
+        """Serve the requests (arrival times in seconds on the wall clock);
+        returns statistics.  timeout_s bounds the whole run (TimeoutError)."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        active = {}            # handle -> (request, tickets)
+        t0 = time.perf_counter()
+        updates = 0
+        last_round = -1
+        steps = int(round(self.pool.params.trial_ms / self.pool.params.dt)) - 1
+
+        def now():
+            return time.perf_counter() - t0
+
+        while pending or waiting or active:
+            t = now()
+            if timeout_s is not None and t > timeout_s:
+                raise TimeoutError("%d requests unfinished after %g s" % (len(pending) + len(waiting) + len(active),
+                                                                         timeout_s))
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            while waiting:
+                req = waiting[0]
+                h = self.pool.admit(req.net, req.key)
+                if h is None:
+                    if not active:
+                        raise RuntimeError("request %d does not fit into an empty pool" % req.rid)
+                    break
+                waiting.popleft()
+                req.handle, req.admitted = h, now()
+                req.results = []
+                req._chain = ChainStats(req.net.n_ra) if req.mode == 1 else None
+                tickets = deque(self.pool.submit(h, **req.trial_kw(k)) for k in range(req.n_trials))
+                active[h] = (req, tickets)
+            progressed = False
+            for h, (r, tickets) in list(active.items()):
+                while tickets:
+                    got = self.pool.poll(tickets[0])
+                    if got is None:
+                        break
+                    tickets.popleft()
+                    progressed = True
+                    rnd, res = got
+                    last_round = max(last_round, rnd)
+                    r.results.append(res if keep_results else None)
+                    if r._chain is not None:
+                        r._chain.add_result(res)
+                    updates += (r.net.n_ra + r.net.n_i) * steps
+                if not tickets:
+                    if r.mode == 0:
+                        r.synapses = self.pool.download_synapses(h)
+                    else:
+                        r.stats = r._chain.result()
+                        r._chain.close()
+                        r._chain = None
+                    self.pool.release(h)
+                    r.finished = now()
+                    del active[h]
+                    if log:
+                        log("request %d done: %d resident, %d waiting" % (r.rid, len(active), len(waiting)))
+            if not progressed:
+                nxt = pending[0].arrival - now() if (pending and not active and not waiting) else poll_s
+                time.sleep(max(poll_s, min(nxt, 0.05)))
+        wall = now()
+        lat = np.array([r.finished - r.arrival for r in requests])
+        return {
+            "requests": len(requests),
+            "rounds": last_round + 1,
+            "wall_s": wall,
             "neuron_updates": updates,
             "neuron_updates_per_s": updates / wall if wall > 0 else 0.0,
             "latency_p50": float(np.percentile(lat, 50)),
