@@ -80,6 +80,11 @@ def parse():
     ap.add_argument("--check-trials", type=int, default=10, help="c5: trials per checked request")
     ap.add_argument("--variants", type=int, default=2, help="c5: networks built per size")
     ap.add_argument("--hbm-gib", type=float, default=8.0, help="c5: device memory of the serving pool")
+    ap.add_argument("--c2-trials", type=int, default=100, help="c2: growth trials through the driver")
+    ap.add_argument("--c2-fp64", type=int, default=1, help="c2: 1 runs the FP64 form (the config's), 0 follows "
+                                                          "--precision")
+    ap.add_argument("--activity-window", type=int, default=25,
+                    help="c2: trials of the activity check (and replacement cadence)")
     return ap.parse_args()
 
 
@@ -133,6 +138,59 @@ def bench_driver(net, kw, args, precision, tuning):
             "spikes_per_trial": sum(r.soma_spikes + r.i_spikes for r in reps) / len(reps),
             "note": "growth loop through the C++ driver (hvc_driver_trial): trial with a random injection time, "
                     "spike read-back, check_neuron_activity, maturation update and the Erest/GCa patch"}
+
+
+def bench_growth_c2(args):
+    """The HVC-scale growth run (configs.c2, FP64 by default): the dense 2000 RA
+    + 550 I network grown trial after trial through the C++ driver (random
+    injection times, check_neuron_activity over a window of 25 trials,
+    maturation, replacement of silent neurons with resampled connections, the
+    Erest / GCa patch), every step of the loop inside the timed region."""
+    from unsupervised_growth_amd import configs
+    from unsupervised_growth_amd.driver import GrowthDriver
+    from unsupervised_growth_amd.engine import make_params
+    from unsupervised_growth_amd.network import MaturationParams
+    t_build = time.perf_counter()
+    net, kw = configs.c2()
+    ra, ii, mid, top = configs.c2_geometry()
+    t_build = time.perf_counter() - t_build
+    if args.trial_ms:
+        kw["trial_ms"] = args.trial_ms
+    precision = 0 if args.c2_fp64 else (0 if args.precision == "fp64" else 1)
+    params = make_params(ra_model=args.ra_model, i_model=args.i_model, precision=precision, **kw)
+    n = args.c2_trials
+    with GrowthDriver(net, ra, ii, mid, top, MaturationParams(), params=params,
+                      activity_window=args.activity_window) as drv:
+        if args.shape:
+            v = [int(x) for x in args.shape.split(",")]
+            drv.set_tuning(v[0], v[1], v[2], v[3], v[4] if len(v) > 4 else 1, v[5] if len(v) > 5 else 1)
+        reps = []
+        t0 = time.perf_counter()
+        for k in range(n):
+            reps.append(drv.trial())
+            if (k + 1) % 10 == 0:
+                print("c2: %d trials, %.1f s" % (k + 1, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+    steps = int(round(kw["trial_ms"] / 0.02)) - 1
+    replaced = [(k, r.replaced) for k, r in enumerate(reps) if r.replaced]
+    line = {
+        "metric": "RK4 neuron-updates/s, HVC-scale growth through the driver (replacement included)",
+        "value": (net.n_ra + net.n_i) * steps * n / wall, "unit": "neuron-updates/s", "n_gpus": 1,
+        "steps": n, "warmup": 0, "ms_per_step": 1000.0 * wall / n, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if precision == 0 else "f32",
+        "data": "synthetic network (distance rules, seeded), counter-based noise streams",
+        "config": {"workload": "growth of %d HVC-RA + %d HVC-I, dense RA->RA candidates (%d), lognormal delays "
+                               "(median 2 ms, sigma 0.5, <= 15 ms), %d trials of %g ms, RA %s + I DP8, "
+                               "activity window and replacement cadence %d trials"
+                               % (net.n_ra, net.n_i, net.n_candidates, n, kw["trial_ms"],
+                                  "RK4-3/8" if args.ra_model else "Debraband", args.activity_window),
+                   "config": "c2", "network_build_s": round(t_build, 1)},
+        "replaced_by_trial": replaced,
+        "first_replacement_trial": replaced[0][0] if replaced else None,
+        "spikes_per_trial": sum(r.soma_spikes + r.i_spikes for r in reps) / max(len(reps), 1),
+        "matured": sum(r.matured for r in reps),
+    }
+    print(json.dumps(line), flush=True)
 
 
 def deliver_bytes(net, results, windows, avg_ms, config):
@@ -428,6 +486,9 @@ def main():
         return
     if args.config == "c5":
         bench_serve(args)
+        return
+    if args.config == "c2":
+        bench_growth_c2(args)
         return
     from unsupervised_growth_amd import configs, opcount
     from unsupervised_growth_amd.engine import Engine

@@ -283,3 +283,48 @@ def test_growth_runs_end_to_end(tmp_path):
     got = F.read_jitter(str(out3 / "jitter.bin"))
     np.testing.assert_array_equal(got[2], stats["probability_soma_spike"])
     assert os.path.exists(str(out3 / "test_spike_times_soma_2.bin"))
+
+
+def test_replacement_every_25_trials_equals_oracle(tmp_path):
+    """The cadence of the 2000-neuron growth workload: with an activity window
+    of 25 trials (check_neuron_activity's 1000, NetworkGrowthSimulator.cpp:9586,
+    as a parameter), no neuron is replaced before its 26th trial after the
+    start, silent neurons are replaced at that trial, and the driver's state
+    after it equals the oracle's bookkeeping (oracle/ora_growth.py with the same
+    window and generator draws) applied to the files written at that trial."""
+    from unsupervised_growth_amd import formats as F
+    from unsupervised_growth_amd.driver import GrowthDriver
+    from unsupervised_growth_amd.engine import make_params
+    W = 25
+    t, net, p, mat = _setup()
+    d = str(tmp_path)
+    params = make_params(precision=0, trial_ms=TRIAL_MS, seed=(7, 0))
+    print("activity window %d: n_ra=%d n_i=%d" % (W, net.n_ra, net.n_i), flush=True)
+    reports = []
+    with GrowthDriver(net, t.ra_xyz, t.i_xyz, t.model_interneuron_distance, p, mat, seed=SEED, params=params,
+                      activity_window=W) as drv:
+        drv.write_state(d, what="initial")
+        for k in range(W + 1):
+            if k == W:
+                drv.set_output(d, save_freq_short=1, save_freq_long=1)
+            reports.append(drv.trial(training=True))
+        drv.write_state(d, "_after", "full")
+        replaced_ids, replaced_trials = drv.replaced()
+    assert all(r.replaced == 0 for r in reports[:W]), [r.replaced for r in reports]
+    g = og.Mt19937(SEED)
+    for r in reports:
+        assert r.mean_injection_time == og.WAITING_TIME + g.random(TRIAL_MS - 2 * og.WAITING_TIME)
+    s = _oracle_state(d, "_%d" % W)
+    _, _, soma = F.read_spike_times(os.path.join(d, "spike_times_soma_%d.bin" % W))
+    counts = [len(x) for x in soma]
+    s["trials_after"] = [a + 1 for a in s["trials_after"]]
+    R, driven = og.activity_update(s["history"], counts, s["trials_after"], s["scale"], mat.DEATH_RATE_THRESHOLD,
+                                   window=W)
+    matured = og.maturation_update(_mat_dict(mat), s["scale"], s["mature"], s["erest"], s["gca"])
+    og.replace_neurons(s, R, g, TOPO, _mat_dict(mat))
+    last = reports[W]
+    print("trial %d: replaced %d, driven %d, matured %d" % (W, last.replaced, last.became_driven, last.matured),
+          flush=True)
+    assert last.replaced == len(R) > 0 and last.became_driven == driven and last.matured == matured
+    assert replaced_ids[:len(R)].tolist() == R and (replaced_trials[:len(R)] == W).all()
+    _assert_state_equal(d, "_after", s)

@@ -105,6 +105,22 @@ def c2(seed=7, trial_ms=1000.0):
     return net, dict(trial_ms=trial_ms, seed=(seed, 0))
 
 
+def c2_geometry(seed=7):
+    """Positions of the c2 network (its first draws) and the topology
+    parameters that resample a replaced neuron by the c2 rule (Gaussian
+    distance rule, peak 0.3, width 0.1 on the sphere), in the reference's units
+    (distance on the sphere x 40 / MODEL_INTERNEURON_DISTANCE microns):
+    (ra_xyz, i_xyz, model_interneuron_distance, TopologyParams).  Replaced
+    neurons' new connections take distance-proportional delays."""
+    from .topology import TopologyParams, sphere_model_interneuron_distance
+    rng = np.random.default_rng(seed)
+    pr, pi = sphere_positions(rng, 2000), sphere_positions(rng, 550)
+    mid = sphere_model_interneuron_distance(550)
+    width = 0.1 * 40.0 / mid
+    return pr, pi, mid, TopologyParams(arrangement="sphere", A_RA2I=0.3, SIGMA_RA2I=width, B_I2RA=0.3,
+                                       SIGMA_I2RA=width, Gei_max=0.25, Gie_max=0.5, delay_constant=1.0)
+
+
 def sphere_network(rng, n_ra, n_i, delay_max, k_rr=100, n_train=None, spread_max=20.0):
     """RA and I uniform on the unit sphere, Gaussian distance rule (peak 0.3,
     width 0.1) for RA->I and I->RA, the k_rr nearest RA as RA->RA candidates,
