@@ -299,6 +299,7 @@ def bench_batch(args):
     for b in check_nets:
         with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=precision, seed=keys[b], **kw) as e1:
             e1.upload(nets[b])
+            e1.set_tuning(**tuning)          # the batch's launch shape: the FP32 rounding follows the layout
             e1.upload_synapses(syn0[b])
             r1 = e1.run_trial(mode=0, training=True, trial_index=first_timed, first_trial=(first_timed == 0),
                               mean_injection_time=100.0)
