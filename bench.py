@@ -34,9 +34,9 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the integration kernel, from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
 # profiles/integrate_c3_hbm_traffic.txt; keyed by (config, precision, HVC-I layout)
-TRAFFIC_PROFILED = {("c3", 1, 2): 7.167e6, ("c3", 0, 2): 7.782e6}
-# HBM bytes per launch of the delivery kernel (same passes), keyed by config
-DELIVER_TRAFFIC_PROFILED = {"c3": 4.26e6}
+TRAFFIC_PROFILED = {("c3", 1, 3): 7.245e6, ("c3", 0, 3): 7.705e6}
+# HBM bytes per launch of the delivery kernel (same passes), keyed by (config, delivery form: 0 sweep, 1 lists)
+DELIVER_TRAFFIC_PROFILED = {("c3", 0): 4.331e6, ("c3", 1): 1.873e6}
 DELIVER_TRAFFIC_NOTE = "profiles/integrate_c3_hbm_traffic.txt"
 HBM_PEAK_GBPS = 8000.0
 
@@ -193,7 +193,7 @@ def bench_growth_c2(args):
     print(json.dumps(line), flush=True)
 
 
-def deliver_bytes(net, results, windows, avg_ms, config):
+def deliver_bytes(net, results, windows, avg_ms, key):
     """The delivery kernel against its bytes: algorithmic bytes per launch (every
     edge a spike delivers: 4 B target + 8 B delay + 8 B weight read, 16 B inbox
     record written) from the timed trials' spikes and the network's out-degrees
@@ -207,7 +207,7 @@ def deliver_bytes(net, results, windows, avg_ms, config):
         edges += int(np.dot(np.asarray([len(x) for x in r.soma]), ra_deg))
         edges += int(np.dot(np.asarray([len(x) for x in r.interneuron]), i_deg))
     alg = 36.0 * edges / (len(results) * windows)
-    meas = DELIVER_TRAFFIC_PROFILED.get(config)
+    meas = DELIVER_TRAFFIC_PROFILED.get(key)
     s = avg_ms / 1000.0
     out = {"avg_launch_ms": avg_ms, "delivered_edges_per_trial": edges / len(results),
            "algorithmic_bytes_per_launch": alg, "algorithmic_GBps": alg / s / 1e9 if s > 0 else None,
@@ -567,7 +567,7 @@ def main():
                                                        opcount.FLOPS_PER_EXP[precision]),
                 "flops_per_launch": flops_launch, "avg_launch_ms": kern["integrate"]["avg_ms"],
                 "dominant": dom, "kernels_ms_per_trial": per_trial}
-    deliver = deliver_bytes(net, results, windows, kern["deliver"]["avg_ms"], args.config)
+    deliver = deliver_bytes(net, results, windows, kern["deliver"]["avg_ms"], (args.config, tuning["deliver_lists"]))
 
     # FP32 form: each timed trial again in the FP64 form (bit-exact with the
     # oracle) from the same synapse state; per-trial spike-time error
