@@ -750,7 +750,8 @@ __device__ void integrate_i_oct(const NetDev& nd, const Win& win, int wave, int 
         double sum;
         if (inbox_take(nd, nd.in_ge_i, i, sum)) ge += sum;
     }
-    ge = q < 4 ? quad_bcast<0>(ge) : quad_bcast<3>(oct_mirror(ge));
+    const double ge_lo = quad_bcast<0>(ge), ge_hi = quad_bcast<3>(oct_mirror(ge));
+    ge = q < 4 ? ge_lo : ge_hi;
     s.Ge = ge; s.Gi = nd.in[6][ii];
     int flag = nd.i_flag[ii];
     NoiseCursor nc;

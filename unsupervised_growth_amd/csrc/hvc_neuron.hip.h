@@ -1163,10 +1163,14 @@ __device__ __forceinline__ real i_oct_rhs(Guard& bad, const IState& s, int q, co
 
 // the stage values every lane needs: V in all lanes; n, m, h (quad
 // broadcasts) and w (mirror of lane 7) in lane 0
+// (both broadcasts of V run in every lane and a select picks one: a DPP move
+// under a lane-dependent condition would split the wave's exec mask)
 #define HVC_BCAST8(x)                                            \
     {                                                            \
         const real t_ = oct_mirror(x);                           \
-        v = q < 4 ? quad_bcast<0>(x) : quad_bcast<3>(t_);        \
+        const real v_lo_ = quad_bcast<0>(x);                     \
+        const real v_hi_ = quad_bcast<3>(t_);                    \
+        v = q < 4 ? v_lo_ : v_hi_;                               \
         n1 = quad_bcast<1>(x);                                   \
         m1 = quad_bcast<2>(x);                                   \
         h1 = quad_bcast<3>(x);                                   \
