@@ -1188,6 +1188,23 @@ __device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real&
     const real x0 = xo;
     real v = s.V, n1 = s.n, m1 = s.m, h1 = s.h, w1 = s.w;
     real x;
+#ifdef HVC_FP32
+    // the step's decay factors of stages 3-13, read from LDS together before
+    // the stages (each stage's read would otherwise wait on its own round trip)
+    real e2r[11], e5r[11];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        e2r[k] = row.e2[kOffDp2 + k];
+        e5r[k] = row.e5[kOffDp2 + k];
+    }
+    e2r[10] = row.e2[kOffDt];
+    e5r[10] = row.e5[kOffDt];
+#define HVC_OE2(k, off) e2r[k]
+#define HVC_OE5(k, off) e5r[k]
+#else
+#define HVC_OE2(k, off) row.e2[off]
+#define HVC_OE5(k, off) row.e5[off]
+#endif
 #define HVC_OSTAGE(N, e2, e5, SX)                                                   \
     const real k_##N = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, e2, e5); \
     x = x0 + SX(k_);                                                                \
@@ -1195,18 +1212,20 @@ __device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real&
     const real k_1 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x0, HR(1.0), HR(1.0));
     x = x0 + HVC_DP8_S2(k_);
     HVC_BCAST8(x)
-    HVC_OSTAGE(2, row.e2[kOffDp2], row.e5[kOffDp2], HVC_DP8_S3)
-    HVC_OSTAGE(3, row.e2[kOffDp3], row.e5[kOffDp3], HVC_DP8_S4)
-    HVC_OSTAGE(4, row.e2[kOffDp4], row.e5[kOffDp4], HVC_DP8_S5)
-    HVC_OSTAGE(5, row.e2[kOffDp5], row.e5[kOffDp5], HVC_DP8_S6)
-    HVC_OSTAGE(6, row.e2[kOffDp6], row.e5[kOffDp6], HVC_DP8_S7)
-    HVC_OSTAGE(7, row.e2[kOffDp7], row.e5[kOffDp7], HVC_DP8_S8)
-    HVC_OSTAGE(8, row.e2[kOffDp8], row.e5[kOffDp8], HVC_DP8_S9)
-    HVC_OSTAGE(9, row.e2[kOffDp9], row.e5[kOffDp9], HVC_DP8_S10)
-    HVC_OSTAGE(10, row.e2[kOffDp10], row.e5[kOffDp10], HVC_DP8_S11)
-    HVC_OSTAGE(11, row.e2[kOffDp11], row.e5[kOffDp11], HVC_DP8_S12)
+    HVC_OSTAGE(2, HVC_OE2(0, kOffDp2), HVC_OE5(0, kOffDp2), HVC_DP8_S3)
+    HVC_OSTAGE(3, HVC_OE2(1, kOffDp3), HVC_OE5(1, kOffDp3), HVC_DP8_S4)
+    HVC_OSTAGE(4, HVC_OE2(2, kOffDp4), HVC_OE5(2, kOffDp4), HVC_DP8_S5)
+    HVC_OSTAGE(5, HVC_OE2(3, kOffDp5), HVC_OE5(3, kOffDp5), HVC_DP8_S6)
+    HVC_OSTAGE(6, HVC_OE2(4, kOffDp6), HVC_OE5(4, kOffDp6), HVC_DP8_S7)
+    HVC_OSTAGE(7, HVC_OE2(5, kOffDp7), HVC_OE5(5, kOffDp7), HVC_DP8_S8)
+    HVC_OSTAGE(8, HVC_OE2(6, kOffDp8), HVC_OE5(6, kOffDp8), HVC_DP8_S9)
+    HVC_OSTAGE(9, HVC_OE2(7, kOffDp9), HVC_OE5(7, kOffDp9), HVC_DP8_S10)
+    HVC_OSTAGE(10, HVC_OE2(8, kOffDp10), HVC_OE5(8, kOffDp10), HVC_DP8_S11)
+    HVC_OSTAGE(11, HVC_OE2(9, kOffDp11), HVC_OE5(9, kOffDp11), HVC_DP8_S12)
 #undef HVC_OSTAGE
-    const real k_12 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, row.e2[kOffDt], row.e5[kOffDt]);
+    const real k_12 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, HVC_OE2(10, kOffDt), HVC_OE5(10, kOffDt));
+#undef HVC_OE2
+#undef HVC_OE5
     {
         struct { real x, x_lo; } acc{x0, xlo};
         HVC_ACC(acc, x, HVC_DP8_FIN(k_))
