@@ -89,3 +89,63 @@ def test_latency_statistics():
     lat = [r.finished - r.arrival for r in reqs]
     assert stats["latency_p50"] == np.percentile(lat, 50)
     assert stats["latency_max"] == max(lat)
+
+
+class FakeAsyncPool(FakePool):
+    """submit / poll on top of FakePool: a poll runs the next round (the oldest
+    queued trial of every handle that has one) when the polled ticket is not
+    done yet; results carry the round that ran them."""
+
+    def __init__(self, capacity):
+        super().__init__(capacity)
+        self.queue = []
+        self.done = {}
+        self.ticket = 0
+        self.round = 0
+
+    def submit(self, h, **kw):
+        assert h in self.resident
+        self.ticket += 1
+        self.queue.append((self.ticket, h, kw))
+        return self.ticket
+
+    def _round(self):
+        picked, seen = [], set()
+        for item in self.queue:
+            if item[1] not in seen:
+                seen.add(item[1])
+                picked.append(item)
+        for item in picked:
+            self.queue.remove(item)
+        res = self.run({h: kw for _, h, kw in picked})
+        for t, h, _ in picked:
+            self.done[t] = (self.round, res[h])
+        self.round += 1
+
+    def poll(self, t):
+        if t not in self.done and self.queue:
+            self._round()
+        return self.done.pop(t, None)
+
+    def release(self, h):
+        assert not any(item[1] == h for item in self.queue), "released with trials queued"
+        super().release(h)
+
+
+def test_async_server_submits_in_order_and_releases_when_done():
+    from unsupervised_growth_amd.serve import AsyncServer
+    pool = FakeAsyncPool(capacity=10)
+    reqs = [_req(0, 0.0, 6, 3), _req(1, 0.0, 6, 2), _req(2, 0.0, 3, 2), _req(3, 0.0, 2, 1, mode=0)]
+    stats = AsyncServer(pool).run(reqs, poll_s=0.0001, timeout_s=30)
+    assert stats["requests"] == 4 and pool.used == 0 and not pool.resident
+    for r in reqs:
+        # every trial of a request, in order, with consecutive trial indices
+        assert [x.tag[2] for x in r.results] == [r.first_trial_index + k for k in range(r.n_trials)]
+        assert r.finished is not None and r.finished >= r.admitted
+    # request 1 (6 neurons) does not fit beside request 0: it and request 2 behind it wait (first come, first served)
+    assert reqs[1].admitted >= reqs[0].finished
+    assert reqs[2].admitted >= reqs[1].admitted
+    # a network never ran two trials in one round
+    assert all(len(r) == len(set(r)) for r in pool.rounds)
+    assert reqs[3].synapses == ("syn", reqs[3].handle)
+    assert reqs[0].stats["num_test_trials"] == 3
