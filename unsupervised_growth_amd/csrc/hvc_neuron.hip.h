@@ -166,6 +166,12 @@ template <bool EXACT>
 __device__ __forceinline__ real rInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, 5, 10); }
 template <bool EXACT>
 __device__ __forceinline__ real cInf(real v, Guard& bad) { HVC_FP_FAST return HVC_SIGM(v, -1, -10, 7); }
+// the denominators 1 + exp(...) of nInf, tauN, hInf, tauH, for the one-thread
+// steppers: (x_inf - x) / tau then takes one reciprocal per gate instead of three
+__device__ __forceinline__ real nInfDen(real v) { HVC_FP_FAST return 1 + HVC_EXPLIN(v, -1, 35, 10); }
+__device__ __forceinline__ real tauNDen(real v) { HVC_FP_FAST return 1 + HVC_EXPLIN(v, 1, 27, 15); }
+__device__ __forceinline__ real hInfDen(real v) { HVC_FP_FAST return 1 + HVC_EXPLIN(v, 1, 45, 7); }
+__device__ __forceinline__ real tauHDen(real v) { HVC_FP_FAST return 1 + HVC_EXPLIN(v, 1, 40.5, 6); }
 #undef HVC_SIGM
 #undef HVC_EXPLIN
 #else
@@ -196,15 +202,27 @@ struct RAGates {
     real m, nI, tN, hI, tH, lI, rI, cI;
 };
 
-template <int MODEL, bool EXACT>
+// RAW (FP32 form, one thread per neuron): nI, tN, hI, tH hold the
+// denominators of their sigmoids (nInfDen ...), which ra_rhs_g<..., true> expects
+template <int MODEL, bool EXACT, bool RAW = false>
 __device__ __forceinline__ RAGates ra_gates(Guard& bad, real vs, real vd)
 {
     RAGates g;
     g.m = mInf<EXACT>(vs, bad);
-    g.nI = nInf<EXACT>(vs, bad);
-    g.tN = tauN<EXACT>(vs, bad);
-    g.hI = hInf<EXACT>(vs, bad);
-    g.tH = tauH<EXACT>(vs, bad);
+#ifdef HVC_FP32
+    if constexpr (RAW) {
+        g.nI = nInfDen(vs);
+        g.tN = tauNDen(vs);
+        g.hI = hInfDen(vs);
+        g.tH = tauHDen(vs);
+    } else
+#endif
+    {
+        g.nI = nInf<EXACT>(vs, bad);
+        g.tN = tauN<EXACT>(vs, bad);
+        g.hI = hInf<EXACT>(vs, bad);
+        g.tH = tauH<EXACT>(vs, bad);
+    }
     g.lI = (MODEL == 0) ? lInf<EXACT>(vs, bad) : HR(0.0);
     g.rI = rInf<EXACT>(vd, bad);
     g.cI = cInf<EXACT>(vd, bad);
@@ -212,7 +230,7 @@ __device__ __forceinline__ RAGates ra_gates(Guard& bad, real vs, real vd)
 }
 
 // all eight derivatives at one stage from its gate values; E = decay factor of the stage time
-template <int MODEL, bool EXACT>
+template <int MODEL, bool EXACT, bool RAW = false>
 __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, real GCa, const RAUniform& u,
                                         const RAGates& g, real vs, real vd, real n, real h, real l, real r,
                                         real c, real ca, real E)
@@ -240,9 +258,17 @@ __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, r
     d = __builtin_fmaf(s.Ge_d * E, vd, d);
     d = __builtin_fmaf(s.Gi_d * E, vd - kRA_Egaba, d);
     k.Vd = __builtin_fmaf((real)(100000.0 / 550000.0), vs - vd, u.mu_term_d - d);
-    k.n = DIVR(g.nI - n, g.tN);
+    if constexpr (RAW) {
+        // (1/D - x) / (a + c/T) = (1 - x D) T / (D (a T + c)), D, T the sigmoids' denominators
+        k.n = (__builtin_fmaf(-n, g.nI, HR(1.0)) * g.tN) *
+              __builtin_amdgcn_rcpf(g.nI * __builtin_fmaf(HR(0.1), g.tN, HR(0.5)));
+        k.h = (__builtin_fmaf(-h, g.hI, HR(1.0)) * g.tH) *
+              __builtin_amdgcn_rcpf(g.hI * __builtin_fmaf(HR(0.1), g.tH, HR(0.75)));
+    } else {
+        k.n = DIVR(g.nI - n, g.tN);
+        k.h = DIVR(g.hI - h, g.tH);
+    }
     k.l = (MODEL == 0) ? CD(g.lI - l, 10) : HR(0.0);
-    k.h = DIVR(g.hI - h, g.tH);
     k.r = (g.rI - r);  // divided by tauR = 1 (exact)
     k.c = (MODEL == 0) ? CD(g.cI - c, 15) : CD(g.cI - c, 10);
     k.Ca = __builtin_fmaf(-HR(0.1), ica, -HR(0.02) * ca);
@@ -273,8 +299,13 @@ __device__ __forceinline__ RAK ra_rhs(Guard& bad, const RAState& s, real EL, rea
                                       real vs, real vd, real n, real h, real l, real r,
                                       real c, real ca, real E)
 {
+#ifdef HVC_FP32
+    return ra_rhs_g<MODEL, EXACT, true>(bad, s, EL, GCa, u, ra_gates<MODEL, EXACT, true>(bad, vs, vd), vs, vd, n, h,
+                                        l, r, c, ca, E);
+#else
     return ra_rhs_g<MODEL, EXACT>(bad, s, EL, GCa, u, ra_gates<MODEL, EXACT>(bad, vs, vd), vs, vd, n, h, l, r, c,
                                   ca, E);
+#endif
 }
 
 // ---- HVC-RA, RK4 form, four lanes per neuron ----
@@ -343,7 +374,12 @@ __device__ __forceinline__ RAK ra_rhs4(Guard& bad, const RAState& s, real EL, re
         g.cI = quad_bcast<3>(g0);
         g.lI = HR(0.0);
     } else {
+#ifdef HVC_FP32
+        return ra_rhs_g<1, EXACT, true>(bad, s, EL, GCa, u, ra_gates<1, EXACT, true>(bad, vs, vd), vs, vd, n, h,
+                                        HR(0.0), r, c, ca, E);
+#else
         g = ra_gates<1, EXACT>(bad, vs, vd);
+#endif
     }
     return ra_rhs_g<1, EXACT>(bad, s, EL, GCa, u, g, vs, vd, n, h, HR(0.0), r, c, ca, E);
 }
