@@ -513,18 +513,10 @@ __device__ __forceinline__ real xexpm1_series(real x)
     p = __builtin_fmaf(x2, p, HR(1.0 / 12.0));
     return __builtin_fmaf(x2, p, __builtin_fmaf(x, HR(0.5), HR(1.0)));
 }
-// c (v + d) / (1 - exp(-(v + d) / e)), the form of the rates an and am
-// (HHI_buffer.h:173-176), away from v = -d.  Both factors vanish at v = -d,
-// where the quotient cancels (and is 0/0 at v = -d exactly): within
-// |v + d| <= e/2 the rate is c e g(x), g = xexpm1_series, x = (v + d) / e
-// (rates_vd_near); v + d is exact there (Sterbenz), so both forms put the
-// zero at the same point.
-__device__ __forceinline__ real rate_vd(real v, real d, real e, real c)
-{
-    HVC_FP_FAST
-    const real vd = v + d;
-    return c * vd * __builtin_amdgcn_rcpf(1 - __builtin_amdgcn_exp2f(vd * (real)(-1.4426950408889634 / (double)e)));
-}
+// The rates an and am (HHI_buffer.h:173-176) have the form
+// c (v + d) / (1 - exp(-(v + d) / e)).  Both factors vanish at v = -d, where
+// the quotient cancels (and is 0/0 at v = -d exactly): within |v + d| <= e/2
+// the rate is c e g(x), g = xexpm1_series, x = (v + d) / e (rates_vd_near).
 // an (d 15, c 0.15) and am (d 22, c 1), both with e = 10, near their zeros:
 // one test for the two (|v + 18.5| <= 8.5 covers both ranges), the series
 // evaluated only when a lane of the wave is there
@@ -541,19 +533,33 @@ __device__ __forceinline__ bool rates_vd_near(real v, real& an, real& am)
     return false;
 }
 template <bool EXACT>
-__device__ __forceinline__ real i_an(real v, Guard& bad) { return rate_vd(v, HR(15.0), HR(10.0), HR(0.15)); }
-template <bool EXACT>
 __device__ __forceinline__ real i_bn(real v, Guard& bad) { HVC_FP_FAST return HR(0.2) * HVC_EXPLIN(v, 25, 80); }
 template <bool EXACT>
-__device__ __forceinline__ real i_am(real v, Guard& bad) { return rate_vd(v, HR(22.0), HR(10.0), HR(1.0)); }
-template <bool EXACT>
 __device__ __forceinline__ real i_bm(real v, Guard& bad) { HVC_FP_FAST return 40 * HVC_EXPLIN(v, 47, 18); }
-template <bool EXACT>
-__device__ __forceinline__ real i_ah(real v, Guard& bad) { HVC_FP_FAST return HR(0.7) * HVC_EXPLIN(v, 34, 20); }
-template <bool EXACT>
-__device__ __forceinline__ real i_bh(real v, Guard& bad) { HVC_FP_FAST return 10 * __builtin_amdgcn_rcpf(1 + HVC_EXPLIN(v, 4, 10)); }
-template <bool EXACT>
-__device__ __forceinline__ real i_wInf(real v, Guard& bad) { HVC_FP_FAST return __builtin_amdgcn_rcpf(1 + HVC_EXPLIN(v, 0, 5)); }
+// an, am, bh, ah and wInf of one neuron (one thread per neuron) from two
+// exponentials: with E = exp(-(v + 15) / 10) and F = exp(-(v + 34) / 20),
+// exp(-(v + 22) / 10) = E e^-0.7, exp(-(v + 4) / 10) = E e^1.1 and
+// exp(-v / 5) = F^4 e^6.8 (one exp2 each instead of five).  an and am as
+// quotients hold away from their zeros only: within 5 mV of v = -15 and
+// v = -22 rates_vd_near replaces them by their series.
+struct IRates {
+    real an, am, bh, ah, wInf;
+};
+__device__ __forceinline__ IRates i_rates_shared(real v)
+{
+    HVC_FP_FAST
+    IRates q;
+    const real vn = v + HR(15.0);
+    const real e = __builtin_amdgcn_exp2f(vn * (real)(-1.4426950408889634 / 10.0));
+    q.an = HR(0.15) * vn * __builtin_amdgcn_rcpf(1 - e);
+    q.am = (v + HR(22.0)) * __builtin_amdgcn_rcpf(__builtin_fmaf(-e, HR(0.4965853037914095), HR(1.0)));  // e^-0.7
+    q.bh = 10 * __builtin_amdgcn_rcpf(__builtin_fmaf(e, HR(3.0041660239464334), HR(1.0)));              // e^1.1
+    const real f = HVC_EXPLIN(v, 34, 20);
+    q.ah = HR(0.7) * f;
+    const real f2 = f * f;
+    q.wInf = __builtin_amdgcn_rcpf(__builtin_fmaf(f2 * f2, HR(897.8472916504176), HR(1.0)));          // e^6.8
+    return q;
+}
 #undef HVC_EXPLIN
 #else
 template <bool EXACT>
@@ -594,16 +600,17 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
     t = __builtin_fmaf(s.Gi * e5, v - kI_Ei, t);
     k.V = -t;
     auto gate = [](real a, real b, real x) { return __builtin_fmaf(-(a + b), x, a); };   // a (1 - x) - b x
-    real an = i_an<EXACT>(v, bad), am = i_am<EXACT>(v, bad);
+    IRates q = i_rates_shared(v);
     const real bn = i_bn<EXACT>(v, bad), bm = i_bm<EXACT>(v, bad);
-    k.h = gate(i_ah<EXACT>(v, bad), i_bh<EXACT>(v, bad), h1);
-    k.n = gate(an, bn, n1);
-    k.m = gate(am, bm, m1);
+    k.h = gate(q.ah, q.bh, h1);
+    k.n = gate(q.an, bn, n1);
+    k.m = gate(q.am, bm, m1);
     // an and am near their zeros: the series, after the rest of the stage
-    if (rates_vd_near(v, an, am)) {
-        k.n = gate(an, bn, n1);
-        k.m = gate(am, bm, m1);
+    if (rates_vd_near(v, q.an, q.am)) {
+        k.n = gate(q.an, bn, n1);
+        k.m = gate(q.am, bm, m1);
     }
+    k.w = q.wInf - w1;   // divided by tauW = 1 (exact)
 #else
     real n4 = n1 * n1 * n1 * n1;
     real m3 = m1 * m1 * m1;
@@ -612,8 +619,8 @@ __device__ __forceinline__ IK i_rhs(Guard& bad, const IState& s, real v, real n1
     k.n = i_an<EXACT>(v, bad) * (1 - n1) - i_bn<EXACT>(v, bad) * n1;
     k.m = i_am<EXACT>(v, bad) * (1 - m1) - i_bm<EXACT>(v, bad) * m1;
     k.h = i_ah<EXACT>(v, bad) * (1 - h1) - i_bh<EXACT>(v, bad) * h1;
-#endif
     k.w = (i_wInf<EXACT>(v, bad) - w1);  // divided by tauW = 1 (exact)
+#endif
     return k;
 }
 
@@ -824,7 +831,7 @@ __device__ __forceinline__ real gate_rate(Guard& bad, const GateSlot& g, real v)
     real q = DIVR(num, den);
     HVC_KEEP(q);
 #ifdef HVC_FP32
-    // form A near v = -d: the series (rate_vd)
+    // form A near v = -d: the series (as rates_vd_near)
     if constexpr (CAN_A) {
         const bool near = g.type == 0 && __builtin_fabsf(vd) <= HR(0.5) * g.e;
         if (__any(near)) {
@@ -981,7 +988,7 @@ __constant__ GateSlot kIQuadGates[4][2] = {
 // FP32 form of a gate slot: with per-lane constants the three rate forms are
 // one expression, E = exp(-(v + d) / e), rate = cB E + (cA (v + d) + cC) / (1 + sg E)
 // (A: cA = c, sg = -1; B: cB = c; C: cC = c), no lane-dependent selects.
-// Form A cancels near v = -d (rate_vd): its lanes switch to c e g((v + d) / e)
+// Form A cancels near v = -d (rates_vd_near): its lanes switch to c e g((v + d) / e)
 // within |v + d| <= e/2 (lim; -1 in the other forms), when a lane of the wave is there.
 struct GateK {
     real cA, cB, cC, sg, k1, d, ie, ce, lim;
@@ -1038,7 +1045,7 @@ __device__ __forceinline__ void i_quad_rhs(Guard& bad, const IState& s, int role
     const real b = gate_rate_k<false>(gate_k(gb), v);   // second slots are never form A
     kw = a - w1;   // a = wInf in lane 0
     real kg = __builtin_fmaf(-(a + b), xo, a);   // a (1 - x) - b x
-    // form A of the first slot near v = -d (rate_vd): the series, and the
+    // form A of the first slot near v = -d (rates_vd_near): the series, and the
     // derivatives again, when a lane of the wave is there (after the rest of
     // the stage, so that its work is not split by the branch)
     const real vd = v + gka.d;
@@ -1173,7 +1180,7 @@ __device__ __forceinline__ real i_oct_rhs(Guard& bad, const IState& s, int q, co
     const real kV = -t;
     const GateK gk1 = gate_k(g);
     real r = gate_rate_k<false>(gk1, v);
-    // form A (an, am) near v = -d: the series (rate_vd), when a lane of the wave is there
+    // form A (an, am) near v = -d: the series (rates_vd_near), when a lane of the wave is there
     const real vd = v + gk1.d;
     const bool near = __builtin_fabsf(vd) <= gk1.lim;
     if (__any(near)) {
