@@ -239,19 +239,19 @@ __device__ __forceinline__ RAK ra_rhs_g(Guard& bad, const RAState& s, real EL, r
     RAK k;
     real m = g.m;
 #ifdef HVC_FP32
-    // one chain of multiply-adds per compartment: the potassium conductances
-    // share (vs - E_K), the coupling terms use RN(100000 / R) as one factor,
-    // c / (1 + 6 / [Ca]) is c [Ca] / ([Ca] + 6), and the calcium current is
-    // shared by the dendrite and the calcium equation
+    // one chain of multiply-adds per compartment (the soma's started from minus
+    // its mean drive): the potassium conductances share (vs - E_K), the coupling terms
+    // use RN(100000 / R) as one factor, c / (1 + 6 / [Ca]) is c [Ca] / ([Ca] + 6),
+    // and the calcium current is shared by the dendrite and the calcium equation
     const real n2 = n * n;
     real gk = kRA_GsK * (n2 * n2);
     if (MODEL == 0) gk = __builtin_fmaf(kRA_GsKLT, l, gk);
-    real t = kRA_GsL * (vs - EL);
+    real t = __builtin_fmaf(kRA_GsL, vs - EL, -u.mu_term_s);
     t = __builtin_fmaf(kRA_GsNa * (m * m * m) * h, vs - kRA_EsNa, t);
     t = __builtin_fmaf(gk, vs - kRA_EsK, t);
     t = __builtin_fmaf(s.Gi_s * E, vs - kRA_Egaba, t);
     t = __builtin_fmaf(s.Ge_s * E, vs, t);
-    k.Vs = __builtin_fmaf((real)(100000.0 / 275000.0), vd - vs, u.mu_term_s - t);   // divided by cm = 1.0
+    k.Vs = __builtin_fmaf((real)(100000.0 / 275000.0), vd - vs, -t);   // divided by cm = 1.0
     const real ica = GCa * (r * r) * (vd - kRA_EdCa);
     real d = __builtin_fmaf(kRA_GdL, vd - EL, ica);
     d = __builtin_fmaf(kRA_GdCaK * (vd - kRA_EdK), c * ca * __builtin_amdgcn_rcpf(ca + 6), d);
@@ -440,10 +440,19 @@ __device__ __forceinline__ void ra_runge4(Guard& bad, RAState& s, real EL, real 
 {
     HVC_FP_FAST
     const real dt = u.dt;
+#ifdef HVC_FP32
+    // dt / 3 and dt / 8 once per step; k (dt / 8) is (dt k) / 8 exactly
+    const real dt3 = dt * (real)(1.0 / 3.0), dt8 = dt * HR(0.125);
+#define HVC_S2(x, k) __builtin_fmaf(k1.k, dt3, x)
+#define HVC_F(k) ((k1.k + 3 * k2.k + 3 * k3.k + k4.k) * dt8)
+#else
+#define HVC_S2(x, k) (x + CD(dt * k1.k, 3))
+#define HVC_F(k) (dt * (k1.k + 3 * k2.k + 3 * k3.k + k4.k) * HR(0.125))
+#endif
     RAK k1 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs, s.Vd, s.n, s.h, s.r, s.c, s.Ca, HR(1.0));
-    RAK k2 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, s.Vs + CD(dt * k1.Vs, 3), s.Vd + CD(dt * k1.Vd, 3),
-                                   s.n + CD(dt * k1.n, 3), s.h + CD(dt * k1.h, 3), s.r + CD(dt * k1.r, 3),
-                                   s.c + CD(dt * k1.c, 3), s.Ca + CD(dt * k1.Ca, 3), E_third);
+    RAK k2 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, HVC_S2(s.Vs, Vs), HVC_S2(s.Vd, Vd), HVC_S2(s.n, n),
+                                   HVC_S2(s.h, h), HVC_S2(s.r, r), HVC_S2(s.c, c), HVC_S2(s.Ca, Ca), E_third);
+#undef HVC_S2
 #define HVC_S3(x, k) (x + dt * (CD(-k1.k, 3) + k2.k))
     RAK k3 = ra_rhs4<EXACT, LANES>(bad, s, EL, GCa, u, sl, HVC_S3(s.Vs, Vs), HVC_S3(s.Vd, Vd), HVC_S3(s.n, n),
                                    HVC_S3(s.h, h), HVC_S3(s.r, r), HVC_S3(s.c, c), HVC_S3(s.Ca, Ca), E_twothird);
@@ -456,7 +465,6 @@ __device__ __forceinline__ void ra_runge4(Guard& bad, RAState& s, real EL, real 
     s.Ge_s = s.Ge_s * E_dt;
     s.Gi_d = s.Gi_d * E_dt;
     s.Gi_s = s.Gi_s * E_dt;
-#define HVC_F(k) (dt * (k1.k + 3 * k2.k + 3 * k3.k + k4.k) * HR(0.125))
     HVC_ACC(s, Vs, HVC_F(Vs))
     HVC_ACC(s, n, HVC_F(n))
     HVC_ACC(s, h, HVC_F(h))
