@@ -400,6 +400,14 @@ def bench_serve(args):
     def serve(rate, n_req, seed):
         reqs0 = configs.c5_requests(nets, n_requests=n_req, rate=rate, trials=args.trials, seed=seed)
         with Pool(caps, **kw) as pool:
+            # the pool's launch shape (its networks change: no autotuner); FP32 by
+            # default one thread per neuron, the fastest layout once the pool holds
+            # more neurons than fill the chip at eight lanes per HVC-I neuron
+            if args.shape:
+                apply_shape(pool, args)
+            elif precision == 1:
+                pool.set_tuning(integrate_block=64, i_lanes=0, deliver_block=64, sync_block=256, ra_lanes=0,
+                                deliver_lists=1)
             reqs = [Request(rid=k, arrival=a, net=n, key=key, onsets=on, mode=1, first_trial_index=0)
                     for k, (a, n, key, on) in enumerate(reqs0)]
             if args.serve == "async":
@@ -407,6 +415,7 @@ def bench_serve(args):
             else:
                 stats = Server(pool).run(reqs, realtime=True, keep_results=True)
             stats["pool_gib"] = pool.pool_bytes() / (1 << 30)
+            stats["tuning"] = pool.tuning()
         lat = sorted(r.finished - r.arrival for r in reqs)
         pt = {"offered_rate": rate, "requests": n_req, "requests_per_s": n_req / stats["wall_s"],
               "neuron_updates_per_s": stats["neuron_updates_per_s"], "wall_s": stats["wall_s"],
@@ -423,7 +432,7 @@ def bench_serve(args):
         reqs, pt, stats = serve(rate, n_req, seed)
         points.append(pt)
         if seed == 8 or reqs_sus is None:
-            reqs_sus, pool_gib = reqs, stats["pool_gib"]
+            reqs_sus, pool_gib, shape_used = reqs, stats["pool_gib"], stats["tuning"]
 
     # FP32 served results against the FP64 form of the same request alone (one
     # request of each network size, its first check_trials trials)
@@ -470,6 +479,7 @@ def bench_serve(args):
         "latency_s": main_pt["latency_s"],
         "requests_per_s": main_pt["requests_per_s"],
         "load_points": points,
+        "tuning": shape_used,
     }
     if precision_check is not None:
         line["precision_check"] = precision_check

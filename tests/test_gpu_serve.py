@@ -32,17 +32,25 @@ def _requests():
     return reqs
 
 
-def _alone(req, precision, trial_ms):
+def _alone(req, precision, trial_ms, shape=None):
     from unsupervised_growth_amd.engine import Engine
     with Engine(precision=precision, trial_ms=trial_ms, seed=req.key) as e:
         e.upload(req.net)
+        if shape:
+            e.set_tuning(**shape)
         res = [e.run_trial(**req.trial_kw(k)) for k in range(req.n_trials)]
         syn = e.download_synapses() if req.mode == 0 else None
     return res, syn
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_served_requests_equal_alone(precision):
+# the context default (eight lanes per HVC-I neuron) and one thread per neuron,
+# the shape a pool under load runs fastest with; in FP32 the rounding follows
+# the layout, so the request alone runs with the pool's shape
+SHAPES = [None, dict(integrate_block=64, i_lanes=0, deliver_block=64, sync_block=256, ra_lanes=0, deliver_lists=1)]
+
+
+@pytest.mark.parametrize("precision,shape", [(0, None), (1, None), (1, SHAPES[1])], ids=["fp64", "fp32", "fp32-one-thread"])
+def test_served_requests_equal_alone(precision, shape):
     from unsupervised_growth_amd.engine import Pool, pool_caps
     from unsupervised_growth_amd.serve import Server
     trial_ms = 200.0
@@ -51,6 +59,8 @@ def test_served_requests_equal_alone(precision):
                      max_in_degree=200, spikes_per_neuron=128)
     log = []
     with Pool(caps, precision=precision, trial_ms=trial_ms) as pool:
+        if shape:
+            pool.set_tuning(**shape)
         stats = Server(pool).run(reqs, realtime=False, log=log.append)
     print("precision=%d %s" % (precision, stats), flush=True)
     for line in log:
@@ -60,7 +70,7 @@ def test_served_requests_equal_alone(precision):
     assert any(int(l.split()[4]) > 0 for l in log)        # and requests had to wait
     spikes = 0
     for r in reqs:
-        alone, syn = _alone(r, precision, trial_ms)
+        alone, syn = _alone(r, precision, trial_ms, shape)
         assert len(r.results) == len(alone)
         for k, (a, b) in enumerate(zip(alone, r.results)):
             for kind in ("soma", "dend", "interneuron"):

@@ -1361,6 +1361,7 @@ extern "C" hvc_status hvc_get_tuning(hvc_ctx* ctx, hvc_tuning* t)
 extern "C" hvc_status hvc_set_tuning(hvc_ctx* ctx, const hvc_tuning* t)
 {
     if (!ctx || !t) return HVC_E_ARG;
+    ApiLock lk(ctx);   // a pool's serving thread reads the shape at every round
     hvc_status r = check_tuning(ctx, t);
     if (r != HVC_OK) return r;
     ctx->tuning.integrate_block = t->integrate_block;
@@ -1446,6 +1447,7 @@ static hvc_status timed_windows(hvc_ctx* ctx, int windows, const Tuning& tu, dou
 extern "C" hvc_status hvc_autotune(hvc_ctx* ctx, int windows, hvc_tuning* best, double* ms_out)
 {
     if (!ctx || windows <= 0) return HVC_E_ARG;
+    ApiLock lk(ctx);   // between a pool's serving rounds
     if (!ctx->uploaded || ctx->n_nets == 0) return fail(ctx, HVC_E_STATE, "upload networks first");
     HVC_TRY(hipSetDevice(ctx->device));
     Tuning tu = ctx->tuning;
