@@ -215,6 +215,8 @@ struct hvc_ctx {
     bool trial_done = false;
     int log_counts[3] = {0, 0, 0};
     std::vector<int2> h_log[3];
+    std::vector<int2> h_log_raw;     // a spike log as copied from the device
+    std::vector<int> h_log_start;    // counting-sort offsets per neuron
     bool h_log_valid[3] = {false, false, false};
 
     // timing: HIP events recorded on the engine stream around sampled launches,
@@ -1039,9 +1041,27 @@ static hvc_status sorted_log(hvc_ctx* ctx, int kind, const std::vector<int2>** o
     if (!ctx->h_log_valid[kind]) {
         const int2* d_log = kind == 0 ? ctx->nd.soma_log : (kind == 1 ? ctx->nd.dend_log : ctx->nd.i_log);
         const int count = std::min(ctx->log_counts[kind], ctx->nd.log_cap);
+        std::vector<int2>& raw = ctx->h_log_raw;
+        raw.resize(count);
+        if (count > 0) HVC_TRY(hipMemcpy(raw.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
+        // counting sort by neuron (stable), then by step within each neuron; the
+        // log is appended window by window, so a neuron's entries usually come
+        // in step order already and the second pass finds nothing to move
+        const int n = kind == 2 ? ctx->n_i : ctx->n_ra;
+        std::vector<int>& start = ctx->h_log_start;
+        start.assign((size_t)n + 1, 0);
+        for (const int2& v : raw) {
+            if (v.x < 0 || v.x >= n) return fail(ctx, HVC_E_HIP, "spike log holds a neuron id out of range");
+            ++start[v.x + 1];
+        }
+        for (int i = 0; i < n; ++i) start[i + 1] += start[i];
         log.resize(count);
-        if (count > 0) HVC_TRY(hipMemcpy(log.data(), d_log, sizeof(int2) * count, hipMemcpyDeviceToHost));
-        std::sort(log.begin(), log.end(), [](const int2& a, const int2& b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+        for (const int2& v : raw) log[start[v.x]++] = v;
+        for (int i = 0, b = 0; i < n; b = start[i], ++i) {
+            auto lo = log.begin() + b, hi = log.begin() + start[i];
+            if (hi - lo > 1 && !std::is_sorted(lo, hi, [](const int2& a, const int2& c) { return a.y < c.y; }))
+                std::sort(lo, hi, [](const int2& a, const int2& c) { return a.y < c.y; });
+        }
         ctx->h_log_valid[kind] = true;
     }
     *out = &log;
