@@ -161,9 +161,12 @@ def bench_growth_c2(args):
     n = args.c2_trials
     with GrowthDriver(net, ra, ii, mid, top, MaturationParams(), params=params,
                       activity_window=args.activity_window) as drv:
+        tuning = None
         if args.shape:
             v = [int(x) for x in args.shape.split(",")]
             drv.set_tuning(v[0], v[1], v[2], v[3], v[4] if len(v) > 4 else 1, v[5] if len(v) > 5 else 1)
+        elif args.tune_windows > 0:
+            tuning = drv.autotune(windows=args.tune_windows)   # before the timed region
         reps = []
         t0 = time.perf_counter()
         for k in range(n):
@@ -189,6 +192,7 @@ def bench_growth_c2(args):
         "first_replacement_trial": replaced[0][0] if replaced else None,
         "spikes_per_trial": sum(r.soma_spikes + r.i_spikes for r in reps) / max(len(reps), 1),
         "matured": sum(r.matured for r in reps),
+        "tuning": tuning,
     }
     print(json.dumps(line), flush=True)
 

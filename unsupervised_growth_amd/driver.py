@@ -208,6 +208,16 @@ class GrowthDriver:
         if st != _abi.HVC_OK:
             raise EngineError(st, "hvc_set_tuning")
 
+    def autotune(self, windows=400):
+        """Time the launch-shape candidates on the driver's network and keep the
+        fastest (kept across rebuilds); returns the shape."""
+        t = _abi.Tuning()
+        ms = (ctypes.c_double * 3)()
+        st = self.lib.hvc_autotune(self.engine_ctx(), windows, ctypes.byref(t), ms)
+        if st != _abi.HVC_OK:
+            raise EngineError(st, "hvc_autotune")
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
     def engine_ctx(self):
         """The current engine context (valid until the next trial that replaces neurons)."""
         return self.lib.hvc_driver_engine(self.d)
