@@ -30,6 +30,14 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+# the FP32 vector rate of unpacked instructions (one FMA per lane), the ceiling of
+# kernels built without packed FP32 code generation, as the engine's are
+FP32_UNPACKED_TFLOPS = 78.6
+
+
+def unpacked(achieved, precision):
+    """FP32: the achieved rate against the unpacked ceiling (None for FP64)."""
+    return None if precision == 0 else {"ceiling": FP32_UNPACKED_TFLOPS, "frac": achieved / FP32_UNPACKED_TFLOPS}
 
 # HBM bytes per launch of the integration kernel, from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
@@ -351,6 +359,7 @@ def bench_batch(args):
                    "config": "c4", "network_build_s": round(t_build, 1)},
         "roofline": {"bound": "valu", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "flops_per_launch": flops_launch,
+                     "unpacked_fp32": unpacked(achieved, precision),
                      "avg_launch_ms": kern["integrate"],
                      "kernels_ms_per_trial": {k: v * (1 if k in ("decay", "sync") else windows)
                                               for k, v in kern.items()}},
@@ -571,7 +580,7 @@ def main():
     achieved = flops_launch / avg_s / 1e12 if avg_s > 0 else 0.0
     traffic = TRAFFIC_PROFILED.get((args.config, precision, tuning["i_lanes"]))
     roofline = {"bound": "valu", "kernel": "integrate", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": traffic,
+                "frac": achieved / peak, "traffic": traffic, "unpacked_fp32": unpacked(achieved, precision),
                 "traffic_note": None if traffic is None else "HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
                                 "(scaled by a kernel of known traffic), profiles/integrate_c3_hbm_traffic.txt; "
                                 "algorithmic state traffic ~7.5 MB per launch",
