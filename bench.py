@@ -601,7 +601,7 @@ def main():
         wall64 = 0.0
         with Engine(ra_model=args.ra_model, i_model=args.i_model, precision=0, **kw) as e64:
             e64.upload(net)
-            apply_shape(e64, args)
+            tuning64 = apply_shape(e64, args)
             for k, syn in enumerate(states):
                 e64.upload_synapses(syn)
                 t0 = time.perf_counter()
@@ -618,6 +618,7 @@ def main():
                            "unpaired_spikes": sum(p["unpaired_spikes"] for p in per), "per_trial": per}
         fp64_form = {"value": (net.n_ra + net.n_i) * steps_per_trial * len(states) / wall64,
                      "ms_per_step": 1000.0 * wall64 / len(states),
+                     "tuning": tuning64,
                      "note": "the same trials in the plain FP64 GPU form (bit-exact with the oracle)"}
     growth_loop = bench_driver(net, kw, args, precision, tuning) if (args.config == "c3" and
                                                                       args.driver_trials > 0) else None
