@@ -72,16 +72,30 @@ struct DivGuard {
         emin = min(emin, __builtin_amdgcn_frexp_exp(q));
         big = __builtin_fma(__builtin_fabs(q), 0x1p-1000, big);
     }
+    // a value k that enters tableau terms (n k) / d with n / d in [2^-7, 2^5]
+    // and |n| < 2^36 only (DP8 stages 7-12 and the final combination): noting
+    // k once stands for noting each such quotient.  k = 0 or 2^-892 <= |k| <
+    // 2^985 keeps n k finite and every quotient in [2^-899, 2^990) or zero.
+    __device__ __forceinline__ void note_k(double k)
+    {
+        emin = min(emin, __builtin_amdgcn_frexp_exp(k) - 8);
+        big = __builtin_fma(__builtin_fabs(k), 0x1p-985, big);
+    }
     __device__ __forceinline__ bool failed() const { return emin < -899 || !(big < 1.0); }
 };
 
-__device__ __forceinline__ double cdiv_fast(double a, double B, double Y, DivGuard& g)
+// Markstein's correction with the rounded reciprocal Y = RN(1 / B)
+__device__ __forceinline__ double cdiv_plain(double a, double B, double Y)
 {
     double q = a * Y;
     double r = __builtin_fma(-q, B, a);
-    double q1 = __builtin_copysign(__builtin_fma(r, Y, q), a);
-    g.note(q);
-    return q1;
+    return __builtin_copysign(__builtin_fma(r, Y, q), a);
+}
+
+__device__ __forceinline__ double cdiv_fast(double a, double B, double Y, DivGuard& g)
+{
+    g.note(a * Y);
+    return cdiv_plain(a, B, Y);
 }
 
 // the same quotient with the IEEE fallback applied per value

@@ -38,11 +38,13 @@ typedef float real;
 struct Guard {
     static constexpr bool kActive = false;
     __device__ __forceinline__ bool failed() const { return false; }
+    __device__ __forceinline__ void note_k(real) const {}
 };
 #define HR(x) ((real)(x))
 #define CD(a, B) ((a) * (real)(1.0 / (double)(B)))
 #define CDV(a, B, invB) ((a) * (invB))
 #define CK(n, k, d) ((k) * (real)((double)(n) / (double)(d)))
+#define CKN(n, k, d) CK(n, k, d)
 #define DIVR(a, b) ((real)(a) * __builtin_amdgcn_rcpf((real)(b)))
 // exp(a / B) as 2^(a * RN(log2(e) / B)): the argument a / B is already only
 // known to |a / B| * 2^-24 in this form, and folding log2(e) into the constant
@@ -77,6 +79,8 @@ struct Guard : DivGuard {
 #define CDV(a, B, invB) (EXACT ? (a) / (B) : ::hvc::cdiv_fast((a), (B), (invB), bad))
 // tableau term (n k) / d, evaluated as the reference writes it
 #define CK(n, k, d) CD((n) * (k), d)
+// the same term for a k the step notes itself (DivGuard::note_k)
+#define CKN(n, k, d) (EXACT ? ((n) * (k)) / (double)(d) : ::hvc::cdiv_plain((n) * (k), (double)(d), 1.0 / (double)(d)))
 #define DIVR(a, b) ((a) / (b))
 #define EXPR(x) ::hvc::dexp(x)
 #define EXPCD(a, B) EXPR(CD(a, B))
@@ -881,37 +885,40 @@ __device__ __forceinline__ real i_lane_rhs(Guard& bad, const IState& s, int role
 }
 
 // DP8 stage values: the combination of stage derivatives P1.. of one
-// variable (HHI_buffer.cpp:426-691), shared by the five-lane and quad forms
+// variable (HHI_buffer.cpp:426-691), shared by the five-lane and quad forms;
+// the tableau terms of stages 7-12 and the final combination are HVC_CKX
+// (CK, or CKN where the step notes each stage derivative itself)
+#define HVC_CKX CK
 #define HVC_DP8_S2(P) HVC_F2(P##1)
 #define HVC_DP8_S3(P) HVC_F3(P##1 + 3 * P##2)
 #define HVC_DP8_S4(P) HVC_F4(P##1 + 3 * P##3)
 #define HVC_DP8_S5(P) HVC_F5(20 * P##1 - 75 * P##3 + 75 * P##4)
 #define HVC_DP8_S6(P) HVC_F6(3 * P##1 + 15 * P##4 + 12 * P##5)
-#define HVC_DP8_S7(P) dt * (CK(29443841, P##1, 614563906) + CK(77736538, P##4, 692538347) - \
-                            CK(28693883, P##5, 1125000000) + CK(23124283, P##6, 1800000000))
-#define HVC_DP8_S8(P) dt * (CK(16016141, P##1, 946692911) + CK(61564180, P##4, 158732637) + \
-                            CK(22789713, P##5, 633445777) + CK(545815736, P##6, 2771057229) - \
-                            CK(180193667, P##7, 1043307555))
-#define HVC_DP8_S9(P) dt * (CK(39632708, P##1, 573591083) - CK(433636366, P##4, 683701615) - \
-                            CK(421739975, P##5, 2616292301) + CK(100302831, P##6, 723423059) + \
-                            CK(790204164, P##7, 839813087) + CK(800635310, P##8, 3783071287))
-#define HVC_DP8_S10(P) dt * (CK(246121993, P##1, 1340847787) - CK(37695042795, P##4, 15268766246) - \
-                             CK(309121744, P##5, 1061227803) - CK(12992083, P##6, 490766935) + \
-                             CK(6005943493, P##7, 2108947869) + CK(393006217, P##8, 1396673457) + \
-                             CK(123872331, P##9, 1001029789))
-#define HVC_DP8_S11(P) dt * (CK(-1028468189, P##1, 846180014) + CK(8478235783, P##4, 508512852) + \
-                             CK(1311729495, P##5, 1432422823) - CK(10304129995, P##6, 1701304382) - \
-                             CK(48777925059, P##7, 3047939560) + CK(15336726248, P##8, 1032824649) - \
-                             CK(45442868181, P##9, 3398467696) + CK(3065993473, P##10, 597172653))
-#define HVC_DP8_S12(P) dt * (CK(185892177, P##1, 718116043) - CK(3185094517, P##4, 667107341) - \
-                             CK(477755414, P##5, 1098053517) - CK(703635378, P##6, 230739211) + \
-                             CK(5731566787, P##7, 1027545527) + CK(5232866602, P##8, 850066563) - \
-                             CK(4093664535, P##9, 808688257) + CK(3962137247, P##10, 1805957418) + \
-                             CK(65686358, P##11, 487910083))
-#define HVC_DP8_FIN(P) dt * (CK(13451932, P##1, 455176623) - CK(808719846, P##6, 976000145) + \
-                             CK(1757004468, P##7, 5645159321) + CK(656045339, P##8, 265891186) - \
-                             CK(3867574721, P##9, 1518517206) + CK(465885868, P##10, 322736535) + \
-                             CK(53011238, P##11, 667516719) + CK(2, P##12, 45))
+#define HVC_DP8_S7(P) dt * (HVC_CKX(29443841, P##1, 614563906) + HVC_CKX(77736538, P##4, 692538347) - \
+                            HVC_CKX(28693883, P##5, 1125000000) + HVC_CKX(23124283, P##6, 1800000000))
+#define HVC_DP8_S8(P) dt * (HVC_CKX(16016141, P##1, 946692911) + HVC_CKX(61564180, P##4, 158732637) + \
+                            HVC_CKX(22789713, P##5, 633445777) + HVC_CKX(545815736, P##6, 2771057229) - \
+                            HVC_CKX(180193667, P##7, 1043307555))
+#define HVC_DP8_S9(P) dt * (HVC_CKX(39632708, P##1, 573591083) - HVC_CKX(433636366, P##4, 683701615) - \
+                            HVC_CKX(421739975, P##5, 2616292301) + HVC_CKX(100302831, P##6, 723423059) + \
+                            HVC_CKX(790204164, P##7, 839813087) + HVC_CKX(800635310, P##8, 3783071287))
+#define HVC_DP8_S10(P) dt * (HVC_CKX(246121993, P##1, 1340847787) - HVC_CKX(37695042795, P##4, 15268766246) - \
+                             HVC_CKX(309121744, P##5, 1061227803) - HVC_CKX(12992083, P##6, 490766935) + \
+                             HVC_CKX(6005943493, P##7, 2108947869) + HVC_CKX(393006217, P##8, 1396673457) + \
+                             HVC_CKX(123872331, P##9, 1001029789))
+#define HVC_DP8_S11(P) dt * (HVC_CKX(-1028468189, P##1, 846180014) + HVC_CKX(8478235783, P##4, 508512852) + \
+                             HVC_CKX(1311729495, P##5, 1432422823) - HVC_CKX(10304129995, P##6, 1701304382) - \
+                             HVC_CKX(48777925059, P##7, 3047939560) + HVC_CKX(15336726248, P##8, 1032824649) - \
+                             HVC_CKX(45442868181, P##9, 3398467696) + HVC_CKX(3065993473, P##10, 597172653))
+#define HVC_DP8_S12(P) dt * (HVC_CKX(185892177, P##1, 718116043) - HVC_CKX(3185094517, P##4, 667107341) - \
+                             HVC_CKX(477755414, P##5, 1098053517) - HVC_CKX(703635378, P##6, 230739211) + \
+                             HVC_CKX(5731566787, P##7, 1027545527) + HVC_CKX(5232866602, P##8, 850066563) - \
+                             HVC_CKX(4093664535, P##9, 808688257) + HVC_CKX(3962137247, P##10, 1805957418) + \
+                             HVC_CKX(65686358, P##11, 487910083))
+#define HVC_DP8_FIN(P) dt * (HVC_CKX(13451932, P##1, 455176623) - HVC_CKX(808719846, P##6, 976000145) + \
+                             HVC_CKX(1757004468, P##7, 5645159321) + HVC_CKX(656045339, P##8, 265891186) - \
+                             HVC_CKX(3867574721, P##9, 1518517206) + HVC_CKX(465885868, P##10, 322736535) + \
+                             HVC_CKX(53011238, P##11, 667516719) + HVC_CKX(2, P##12, 45))
 
 #define HVC_BCAST5(x)                       \
     v = __shfl(x, base + 0);                \
@@ -1256,11 +1263,16 @@ __device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real&
 #define HVC_OE2(k, off) row.e2[off]
 #define HVC_OE5(k, off) row.e5[off]
 #endif
+// each stage derivative is noted once for all the tableau terms it enters
+#undef HVC_CKX
+#define HVC_CKX CKN
 #define HVC_OSTAGE(N, e2, e5, SX)                                                   \
     const real k_##N = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, e2, e5); \
+    bad.note_k(k_##N);                                                              \
     x = x0 + SX(k_);                                                                \
     HVC_BCAST8(x)
     const real k_1 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x0, HR(1.0), HR(1.0));
+    bad.note_k(k_1);
     x = x0 + HVC_DP8_S2(k_);
     HVC_BCAST8(x)
     HVC_OSTAGE(2, HVC_OE2(0, kOffDp2), HVC_OE5(0, kOffDp2), HVC_DP8_S3)
@@ -1275,6 +1287,7 @@ __device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real&
     HVC_OSTAGE(11, HVC_OE2(9, kOffDp11), HVC_OE5(9, kOffDp11), HVC_DP8_S12)
 #undef HVC_OSTAGE
     const real k_12 = i_oct_rhs<EXACT>(bad, s, q, g, v, n1, m1, h1, w1, x, HVC_OE2(10, kOffDt), HVC_OE5(10, kOffDt));
+    bad.note_k(k_12);
 #undef HVC_OE2
 #undef HVC_OE5
     {
@@ -1289,6 +1302,8 @@ __device__ __forceinline__ void i_dp8_oct(Guard& bad, IState& s, real& xo, real&
     s.Ge = s.Ge * row_e2dt(row);
     s.Gi = s.Gi * row_e5dt(row);
 }
+#undef HVC_CKX
+#define HVC_CKX CK
 #undef HVC_BCAST8
 
 }  // namespace p32 / p64
