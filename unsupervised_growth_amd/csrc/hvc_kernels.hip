@@ -1135,10 +1135,12 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuni
         return (int)((waves * 64 + bs - 1) / bs);
     };
     // a quad per HVC-I neuron shortens the chain of inbox atomics of its ~30
-    // I->RA edges when few waves are in flight (one network); an HVC-RA neuron's
-    // few RA->I edges take one lane.  With many networks the extra waves cost
-    // more than they save, and every push takes one lane.
-    if (nd.n_ra + nd.n_i <= kQuadPushNeurons) {
+    // I->RA edges; an HVC-RA neuron's few RA->I edges take one lane.  On the
+    // lists (at most kDeliverRoleWaves waves per role) quads serve at every
+    // size; the sweep's grid grows with the neurons, and above
+    // kQuadPushNeurons its extra waves cost more than they save, so every
+    // push takes one lane there.
+    if (tu.deliver_lists || nd.n_ra + nd.n_i <= kQuadPushNeurons) {
         const int b0 = role_blocks(nd.n_ra), b1 = role_blocks(nd.n_ra), b2 = role_blocks(4ll * nd.n_i);
         if (b0 + b1 + b2 == 0) return;
         if (tu.deliver_lists)
@@ -1148,10 +1150,7 @@ void launch_deliver(hipStream_t st, const NetDev& nd, int t0, int t1, const Tuni
     } else {
         const int b0 = role_blocks(nd.n_ra), b1 = role_blocks(nd.n_ra), b2 = role_blocks(nd.n_i);
         if (b0 + b1 + b2 == 0) return;
-        if (tu.deliver_lists)
-            k_deliver_list<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
-        else
-            k_deliver_sweep<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
+        k_deliver_sweep<1, 1><<<b0 + b1 + b2, bs, lds, st>>>(nd, t0, t1, b0, b1);
     }
 }
 
