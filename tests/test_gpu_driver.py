@@ -328,3 +328,39 @@ def test_replacement_every_25_trials_equals_oracle(tmp_path):
     assert last.replaced == len(R) > 0 and last.became_driven == driven and last.matured == matured
     assert replaced_ids[:len(R)].tolist() == R and (replaced_trials[:len(R)] == W).all()
     _assert_state_equal(d, "_after", s)
+
+
+def test_autotuned_shape_survives_replacement(tmp_path):
+    """The driver's engine autotuned once keeps its launch shape through the
+    engine rebuild of a replacement, and (FP64: every shape gives the same
+    bits) grows exactly as a driver on the default shape: same reports, same
+    state files after the replacement."""
+    from unsupervised_growth_amd import _abi
+    from unsupervised_growth_amd.driver import GrowthDriver
+    from unsupervised_growth_amd.engine import make_params
+    import ctypes
+    W = 3
+    t, net, p, mat = _setup()
+    params = make_params(precision=0, trial_ms=TRIAL_MS, seed=(7, 0))
+    reports, dirs = [], []
+    for tuned in (False, True):
+        d = str(tmp_path / ("tuned" if tuned else "default"))
+        os.makedirs(d)
+        with GrowthDriver(net, t.ra_xyz, t.i_xyz, t.model_interneuron_distance, p, mat, seed=SEED, params=params,
+                          activity_window=W) as drv:
+            shape = drv.autotune(windows=40) if tuned else None
+            print("tuned=%s shape %s" % (tuned, shape), flush=True)
+            rs = [drv.trial(training=True) for _ in range(W + 2)]
+            assert any(r.replaced > 0 for r in rs), "the window of 3 trials must replace neurons"
+            if tuned:
+                tu = _abi.Tuning()
+                assert drv.lib.hvc_get_tuning(drv.engine_ctx(), ctypes.byref(tu)) == _abi.HVC_OK
+                assert {f: getattr(tu, f) for f, _ in tu._fields_} == shape
+            drv.write_state(d, "_end", "full")
+        reports.append(rs)
+        dirs.append(d)
+    for a, b in zip(*reports):
+        assert a == b
+    for name in sorted(os.listdir(dirs[0])):
+        with open(os.path.join(dirs[0], name), "rb") as fa, open(os.path.join(dirs[1], name), "rb") as fb:
+            assert fa.read() == fb.read(), name
