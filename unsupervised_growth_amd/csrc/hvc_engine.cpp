@@ -1050,8 +1050,10 @@ static hvc_status sorted_log(hvc_ctx* ctx, int kind, const std::vector<int2>** o
         const int n = kind == 2 ? ctx->n_i : ctx->n_ra;
         std::vector<int>& start = ctx->h_log_start;
         start.assign((size_t)n + 1, 0);
+        const int nsteps = (int)ctx->clock.size();
         for (const int2& v : raw) {
             if (v.x < 0 || v.x >= n) return fail(ctx, HVC_E_HIP, "spike log holds a neuron id out of range");
+            if (v.y < 0 || v.y >= nsteps) return fail(ctx, HVC_E_HIP, "spike log holds a step out of range");
             ++start[v.x + 1];
         }
         for (int i = 0; i < n; ++i) start[i + 1] += start[i];
