@@ -779,20 +779,27 @@ __device__ __forceinline__ int rr_row(const NetDev& nd, const double* wclk, int 
             }
             cached = true;
         }
-        // drain in (delivery_time, target) order across all in-flight spikes
+        // drain in (delivery_time, target) order across all in-flight spikes;
+        // the pass that finds nothing due leaves the least head time (next_T)
+        double heads_min = HUGE_VAL;
         for (;;) {
             if (cached) {
                 int best = -1;
-                double bT = 0;
-                int btg = 0, be = 0;
+                double bT = 0, mT = HUGE_VAL;
+                int btg = 0;
                 for (int q = 0; q < sc; ++q) {
                     const double T = hd.T(q);
                     const int tg = hd.tg(q);
+                    mT = T < mT ? T : mT;
                     if (now >= T && (best < 0 || T < bT || (T == bT && tg < btg))) {
-                        best = q; bT = T; btg = tg; be = hd.e(q);
+                        best = q; bT = T; btg = tg;
                     }
                 }
-                if (best < 0) break;
+                if (best < 0) {
+                    heads_min = mT;
+                    break;
+                }
+                const int be = hd.e(best);
                 // the item's reads and the consumed stream's next item, loaded together
                 const ItemIn in = item_load(nd, be, btg);
                 const int c = hd.cur(best) + 1;
@@ -833,8 +840,7 @@ __device__ __forceinline__ int rr_row(const NetDev& nd, const double* wclk, int 
         if (cached) {
             // no stream ran out: the table is unchanged, next_T from the heads
             if (!exhausted) {
-                next_T = HUGE_VAL;
-                for (int q = 0; q < sc; ++q) next_T = hd.T(q) < next_T ? hd.T(q) : next_T;
+                next_T = heads_min;
                 continue;
             }
             heads_store(nd, i, sc, hd);
