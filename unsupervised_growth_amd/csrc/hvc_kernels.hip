@@ -787,13 +787,20 @@ __device__ __forceinline__ int rr_row(const NetDev& nd, const double* wclk, int 
                 int best = -1;
                 double bT = 0, mT = HUGE_VAL;
                 int btg = 0;
-                for (int q = 0; q < sc; ++q) {
-                    const double T = hd.T(q);
-                    const int tg = hd.tg(q);
+                auto consider = [&](int q, double T, int tg) {
                     mT = T < mT ? T : mT;
                     if (now >= T && (best < 0 || T < bT || (T == bT && tg < btg))) {
                         best = q; bT = T; btg = tg;
                     }
+                };
+                // heads read two at a time (the table holds kRowHeads columns, so
+                // a read beyond sc stays inside it and is unused)
+                static_assert(kRowHeads % 2 == 0, "heads read in pairs");
+                for (int q = 0; q < sc; q += 2) {
+                    const double Ta = hd.T(q), Tb = hd.T(q + 1);
+                    const int ga = hd.tg(q), gb = hd.tg(q + 1);
+                    consider(q, Ta, ga);
+                    if (q + 1 < sc) consider(q + 1, Tb, gb);
                 }
                 if (best < 0) {
                     heads_min = mT;
