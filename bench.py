@@ -42,9 +42,9 @@ def unpacked(achieved, precision):
 # HBM bytes per launch of the integration kernel, from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes (scaled by an elementwise kernel of known traffic) recorded in
 # profiles/integrate_c3_hbm_traffic.txt; keyed by (config, precision, HVC-I layout)
-TRAFFIC_PROFILED = {("c3", 1, 3): 7.245e6, ("c3", 0, 3): 7.705e6}
+TRAFFIC_PROFILED = {("c3", 1, 3): 7.244e6, ("c3", 0, 3): 7.680e6}
 # HBM bytes per launch of the delivery kernel (same passes), keyed by (config, delivery form: 0 sweep, 1 lists)
-DELIVER_TRAFFIC_PROFILED = {("c3", 0): 4.331e6, ("c3", 1): 1.873e6}
+DELIVER_TRAFFIC_PROFILED = {("c3", 0): 1.278e6, ("c3", 1): 1.873e6}
 DELIVER_TRAFFIC_NOTE = "profiles/integrate_c3_hbm_traffic.txt"
 HBM_PEAK_GBPS = 8000.0
 

@@ -755,6 +755,7 @@ static hvc_status alloc_device(hvc_ctx* ctx, const Caps& c)
         InFlight f;
         f.head = dalloc<FlHead>(ctx, (size_t)n, &st);
         f.more = dalloc<FlSlot>(ctx, (size_t)n * (kMaxInflight - kHeadSlots), &st);
+        f.held = dalloc<int>(ctx, (size_t)n, &st);
         return f;
     };
     nd.fl_ra2i = mk_inflight(nra);
@@ -1532,7 +1533,7 @@ static PoolCost pool_cost(const hvc_params* prm, int max_indeg, long long spikes
     const double NssA = std::max(prm->syn.Nss, 1);
     const double inbox = (double)(max_indeg + 8) * sizeof(Record) + sizeof(int);
     const double logs = 3.0 * spikes_per_neuron * sizeof(int2);
-    const double inflight = sizeof(FlHead) + (kMaxInflight - kHeadSlots) * sizeof(FlSlot);
+    const double inflight = sizeof(FlHead) + (kMaxInflight - kHeadSlots) * sizeof(FlSlot) + sizeof(int);
     PoolCost c;
     const double noise = (double)noise_cap(prm) * sizeof(NoiseEvent) + sizeof(int);
     c.ra = 12 * 8 + 4 + 4 * 8 + 4 + 2 * 8 + 4 + 2 * 8 + kPostRing * 8 + 16 + 2 * 4 + 4 + 3 * 4 + 4 * NssA +

@@ -146,6 +146,8 @@ static_assert(sizeof(FlHead) == 128, "one line per neuron");
 struct InFlight {
     FlHead* head;   // [n]
     FlSlot* more;   // [n][kMaxInflight - kHeadSlots]
+    int* held;      // [n] the head's count, apart: a window reads the head line
+                    // only of a neuron with spikes held or new
 };
 
 struct NetDev {

@@ -57,6 +57,7 @@ __global__ void k_decay_table(const double* clock, const double* offsets, int nu
 __device__ __forceinline__ void fl_clear(const InFlight& fl, int i)
 {
     fl.head[i].count = 0;
+    fl.held[i] = 0;
     for (int k = 0; k < kHeadSlots; ++k) fl.head[i].s[k].step = -1;
     FlSlot* m = fl.more + (size_t)i * (kMaxInflight - kHeadSlots);
     for (int k = 0; k < kMaxInflight - kHeadSlots; ++k) m[k].step = -1;
@@ -435,7 +436,10 @@ __device__ __forceinline__ int push_fixed(const NetDev& nd, const double* wclk, 
             }
         }
     }
-    if (r == 0) hp->count = now_held;
+    if (r == 0) {
+        hp->count = now_held;
+        fl.held[pre] = now_held;
+    }
     return now_held;
 }
 
@@ -904,8 +908,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     // spikes and the slot-table head: all issued before the block waits on the
     // clock
     const int nclk = t1 - t0 + 2;
-    const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
-    const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
     const bool ra = b < b0 + b1;
     const int q = (ra ? b - b0 : b - b0 - b1) * bs + tid;
     const int G = ra ? GA : GB;
@@ -913,15 +915,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const bool push = b >= b0 && pre < (ra ? nd.n_ra : nd.n_i);
     const FixedEdges fe = ra ? nd.ra2i : nd.i2ra;
     const InFlight fl = ra ? nd.fl_ra2i : nd.fl_i2ra;
-    int rb = 0, re = 0;
+    int rb = 0, re = 0, held = 0;
     uint64_t spikes = 0;
-    FlHead h;
     if (push) {
+        held = fl.held[pre];
         rb = fe.ptr[pre];
         re = fe.ptr[pre + 1];
         spikes = ra ? nd.soma_mask[pre] : nd.i_mask[pre];
-        h = fl.head[pre];
     }
+    const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
+    const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
+    // the slot-table head line only of a neuron with edges and spikes held or
+    // new (push_fixed does nothing for the others); in most windows few
+    // neurons have any, and reading every neuron's line (4 MB per window at
+    // 25 500 neurons) queued the ones that do behind the rest
+    const bool work = push && re > rb && (held > 0 || spikes != 0);
+    FlHead h;
+    if (work) h = fl.head[pre];
     if (tid < nclk) wclk[tid] = c0;
     if (tid + bs < nclk) wclk[tid + bs] = c1;
     for (int k = tid + 2 * bs; k < nclk; k += bs) wclk[k] = nd.clock[t0 - 1 + k];
@@ -943,7 +953,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         return;
     }
     // fixed-weight pushes: G lanes per presynaptic neuron
-    if (!push) return;
+    if (!work) return;
     if (ra)
         push_fixed<GA>(nd, wclk, pre, r, rb, re, nullptr, fe.tgt, fe.w, fe.d, fl, h, spikes, t0, t1, nd.in_ge_i);
     else
