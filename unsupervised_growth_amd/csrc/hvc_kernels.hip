@@ -926,9 +926,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double c0 = tid < nclk ? nd.clock[t0 - 1 + tid] : 0.0;
     const double c1 = tid + bs < nclk ? nd.clock[t0 - 1 + tid + bs] : 0.0;
     // the slot-table head line only of a neuron with edges and spikes held or
-    // new (push_fixed does nothing for the others); in most windows few
-    // neurons have any, and reading every neuron's line (4 MB per window at
-    // 25 500 neurons) queued the ones that do behind the rest
+    // new (push_fixed does nothing for the others): in most windows few
+    // neurons have any, so a window reads ~1.3 MB at 25 500 neurons instead of
+    // every neuron's 128-byte line (4.3 MB)
     const bool work = push && re > rb && (held > 0 || spikes != 0);
     FlHead h;
     if (work) h = fl.head[pre];
